@@ -1,0 +1,220 @@
+"""cageo_amd: MI355X (gfx950) Monte Carlo drainage-basin pipeline.
+
+Python view of the C ABI in ``include/cageo_hip.h`` (``libcageo_hip.so``),
+used by the tests and the benchmark.  Device buffers are passed as torch CUDA
+tensors (only their data pointers cross the boundary) or as raw integers.
+There is no CPU fallback: importing a stage without the built library raises.
+
+Stage methods mirror the reference's stage classes (jsoksane/cageo, src/):
+``random_field`` <- RandomFieldAlgorithm::execute, ``convolve`` <-
+ConvolutionAlgorithm::execute, ``realistic_dem`` <- RealisticDemAlgorithm::
+execute, ``stream_burn`` <- StreamBurningAlgorithm::execute, ``pit_fill`` <-
+PitFillingAlgorithm::execute, ``flow_routing`` <- FlowRoutingAlgorithm::
+execute, ``flow_tracing`` <- FlowTracingAlgorithm::execute,
+``catchment_border`` <- CatchmentBorderAlgorithm::execute, and
+``run_realizations`` <- the loop body of main.cpp:180-283.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+__all__ = [
+    "CageoError", "Context", "lib", "build", "create_filter", "filter_radius", "synth_dem",
+    "EXPONENTIAL", "GAUSSIAN", "OUTSIDE", "LAKE", "LIB_PATH",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcageo_hip.so")
+
+EXPONENTIAL = 0
+GAUSSIAN = 1
+OUTSIDE = -9999
+LAKE = -9998
+
+_STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error",
+           -3: "device allocation failed", -4: "iteration cap reached",
+           -5: "invalid call order", -6: "input outside the supported domain"}
+
+
+class CageoError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {_STATUS.get(status, 'unknown status')} ({status})")
+
+
+def build() -> None:
+    """Compile libcageo_hip.so for gfx950 (hipcc, in-tree)."""
+    subprocess.run(["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")], check=True)
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run cageo_amd.build() (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, f, u32, u64, i64 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
+                               ctypes.c_uint64, ctypes.c_int64)
+    sigs = {
+        "cageo_status_string": ([i], ctypes.c_char_p),
+        "cageo_version": ([], ctypes.c_char_p),
+        "cageo_context_create": ([ctypes.POINTER(vp), i, i, vp], i),
+        "cageo_context_destroy": ([vp], i),
+        "cageo_filter_radius": ([f, f], i),
+        "cageo_create_filter": ([i, f, f, i, vp], i),
+        "cageo_set_filter": ([vp, i, i, vp], i),
+        "cageo_random_field": ([vp, vp, u64, u32, u32, f, f], i),
+        "cageo_convolve": ([vp, vp], i),
+        "cageo_realistic_dem": ([vp, vp, vp, f], i),
+        "cageo_stream_burn": ([vp, vp, vp, i, f, f], i),
+        "cageo_pit_fill": ([vp, vp, vp, f], i),
+        "cageo_flow_routing": ([vp, vp, vp, f], i),
+        "cageo_flow_tracing": ([vp, vp, vp, vp, ctypes.POINTER(i64)], i),
+        "cageo_catchment_border": ([vp, vp, vp], i),
+        "cageo_mc_setup": ([vp, vp, vp, f, f, i, f, f], i),
+        "cageo_run_realizations": ([vp, vp, i, u64, u32], i),
+        "cageo_realization_from_noise": ([vp, vp, vp, vp, vp, vp], i),
+        "cageo_stage_times": ([vp, vp, i], i),
+        "cageo_synth_dem": ([i, i, u64, f, i, f, vp, vp], i),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise CageoError(status, what)
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor (or an int / None)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def filter_radius(practical_range: float, cell_size: float) -> int:
+    """main.cpp:110 -- ceil((PR / cell) * 2.5) in float."""
+    r = lib().cageo_filter_radius(practical_range, cell_size)
+    if r < 0:
+        raise CageoError(r, "filter_radius")
+    return r
+
+
+def create_filter(model: int, practical_range: float, cell_size: float, radius: int) -> np.ndarray:
+    """ConvolutionAlgorithm::createFilter (ConvolutionAlgorithm.cpp:18-45), host side."""
+    L = 2 * radius + 1
+    out = np.empty(L if model == GAUSSIAN else L * L, np.float32)
+    _check(lib().cageo_create_filter(model, practical_range, cell_size, radius,
+                                     out.ctypes.data), "create_filter")
+    return out if model == GAUSSIAN else out.reshape(L, L)
+
+
+def synth_dem(width: int, height: int, seed: int, cell_size: float = 10.0,
+              acc_threshold: int = 2000, nodata_stream: float = -9999.0):
+    """Seeded synthetic DEM and stream raster (host), as float32 numpy arrays."""
+    dem = np.empty((height, width), np.float32)
+    stream = np.empty((height, width), np.float32)
+    _check(lib().cageo_synth_dem(width, height, seed, cell_size, acc_threshold, nodata_stream,
+                                 dem.ctypes.data, stream.ctypes.data), "synth_dem")
+    return dem, stream
+
+
+class Context:
+    """Device workspace for one width x height grid (cageo_context)."""
+
+    def __init__(self, width: int, height: int, stream=None):
+        self.width, self.height = width, height
+        h = ctypes.c_void_p()
+        s = stream if isinstance(stream, (int, type(None))) else stream.cuda_stream
+        _check(lib().cageo_context_create(ctypes.byref(h), width, height, s), "context_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cageo_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stages -------------------------------------------------------------
+    def set_filter(self, model: int, radius: int, taps) -> None:
+        t = np.ascontiguousarray(taps, np.float32).ravel()
+        _check(lib().cageo_set_filter(self._h, model, radius, t.ctypes.data), "set_filter")
+
+    def random_field(self, field, seed, realization=0, stream_id=0, mean=0.0, std=1.0):
+        _check(lib().cageo_random_field(self._h, _ptr(field), seed, realization, stream_id,
+                                        mean, std), "random_field")
+
+    def convolve(self, field):
+        _check(lib().cageo_convolve(self._h, _ptr(field)), "convolve")
+
+    def realistic_dem(self, dem, field, nodata_dem):
+        _check(lib().cageo_realistic_dem(self._h, _ptr(dem), _ptr(field), nodata_dem),
+               "realistic_dem")
+
+    def stream_burn(self, z, stream, bevel_radius, nodata_dem, nodata_stream):
+        _check(lib().cageo_stream_burn(self._h, _ptr(z), _ptr(stream), bevel_radius,
+                                       nodata_dem, nodata_stream), "stream_burn")
+
+    def pit_fill(self, filled, dem, nodata_dem):
+        _check(lib().cageo_pit_fill(self._h, _ptr(filled), _ptr(dem), nodata_dem), "pit_fill")
+
+    def flow_routing(self, flowdir, filled, nodata_dem=-9999.0):
+        _check(lib().cageo_flow_routing(self._h, _ptr(flowdir), _ptr(filled), nodata_dem),
+               "flow_routing")
+
+    def flow_tracing(self, labels, flowdir, stream) -> int:
+        lp = ctypes.c_int64(0)
+        _check(lib().cageo_flow_tracing(self._h, _ptr(labels), _ptr(flowdir), _ptr(stream),
+                                        ctypes.byref(lp)), "flow_tracing")
+        return lp.value
+
+    def catchment_border(self, mc, labels):
+        _check(lib().cageo_catchment_border(self._h, _ptr(mc), _ptr(labels)), "catchment_border")
+
+    # -- Monte Carlo driver -------------------------------------------------
+    def mc_setup(self, dem, stream, nodata_dem, nodata_stream, bevel_radius=2, mean=0.0,
+                 std=1.0):
+        self._inputs = (dem, stream)  # keep the device buffers alive
+        _check(lib().cageo_mc_setup(self._h, _ptr(dem), _ptr(stream), nodata_dem, nodata_stream,
+                                    bevel_radius, mean, std), "mc_setup")
+
+    def run_realizations(self, mc, n, seed, first=0):
+        _check(lib().cageo_run_realizations(self._h, _ptr(mc), n, seed, first),
+               "run_realizations")
+
+    def realization_from_noise(self, noise, mc=None, filled=None, flowdir=None, labels=None):
+        _check(lib().cageo_realization_from_noise(self._h, _ptr(noise), _ptr(mc), _ptr(filled),
+                                                  _ptr(flowdir), _ptr(labels)),
+               "realization_from_noise")
+
+    def stage_times(self, enable=True):
+        out = np.zeros(7, np.float32)
+        _check(lib().cageo_stage_times(self._h, out.ctypes.data, int(enable)), "stage_times")
+        return out

@@ -1,0 +1,363 @@
+// C ABI of libcageo_hip: context (device workspace), stage entry points that
+// replace the reference's XAlgorithm::execute calls, and the fused Monte Carlo
+// driver that replaces the loop body of main.cpp:180-283.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cageo {
+
+namespace {
+
+__global__ void k_iota(int* __restrict__ out, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = i;
+}
+
+int reset_minmax(Context* c) {
+    CAGEO_HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->d_scal + SC_MIN),
+                                    0xffffffffu, 1, c->stream));
+    CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_MAX, 0, 2 * sizeof(uint32_t), c->stream));
+    return CAGEO_OK;
+}
+
+int sync(Context* c) {
+    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    return CAGEO_OK;
+}
+
+template <class T>
+int alloc(T** p, size_t count) {
+    if (hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)) !=
+        hipSuccess) {
+        *p = nullptr;
+        return CAGEO_ERR_NOMEM;
+    }
+    return CAGEO_OK;
+}
+
+void free_all(Context* c) {
+    void* ptrs[] = {c->d_work[0], c->d_work[1], c->d_mark, c->d_scal, c->d_filter,
+                    c->d_sflags, c->d_slabel, c->d_off, c->d_factor, c->d_a, c->d_b, c->d_z,
+                    c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->h_scal) (void)hipHostFree(c->h_scal);
+    for (hipEvent_t& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+// One realization from the error field already in c->d_a (stages a3..a9).
+int realization_body(Context* c, float* mc, bool timed, int first_event) {
+    int ev = first_event;
+    auto mark = [&]() -> int {
+        if (timed) CAGEO_HIP_TRY(hipEventRecord(c->ev[ev++], c->stream));
+        return CAGEO_OK;
+    };
+    CAGEO_TRY(launch_convolve(c, c->d_a, c->d_b));
+    CAGEO_TRY(mark());
+    CAGEO_TRY(reset_minmax(c));
+    CAGEO_TRY(launch_realistic_burn_quant(c, c->dem, c->d_a, c->d_z));
+    CAGEO_TRY(mark());
+    CAGEO_TRY(pit_fill_quantized(c, c->d_z, c->d_w, c->nodata_dem, true));
+    CAGEO_TRY(mark());
+    CAGEO_TRY(flow_routing(c, c->d_w, c->d_fd));
+    CAGEO_TRY(mark());
+    CAGEO_TRY(flow_tracing(c, c->d_fd, c->d_sflags, c->d_slabel, c->d_lab, nullptr));
+    CAGEO_TRY(mark());
+    if (mc) CAGEO_TRY(launch_border(c, mc, c->d_lab));
+    CAGEO_TRY(mark());
+    return CAGEO_OK;
+}
+
+}  // namespace
+
+int worklist_all_tiles(Context* c, int buf) {
+    hipLaunchKernelGGL(k_iota, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
+                       c->d_work[buf], c->ntiles);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+int read_scalars(Context* c) {
+    CAGEO_HIP_TRY(hipMemcpyAsync(c->h_scal, c->d_scal, SC_NSLOTS * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    return CAGEO_OK;
+}
+
+uint32_t next_epoch(Context* c) {
+    if (++c->epoch == 0) {
+        (void)hipMemsetAsync(c->d_mark, 0, (size_t)c->ntiles * sizeof(uint32_t), c->stream);
+        c->epoch = 1;
+    }
+    return c->epoch;
+}
+
+}  // namespace cageo
+
+using cageo::Context;
+
+extern "C" {
+
+const char* cageo_status_string(int s) {
+    switch (s) {
+        case CAGEO_OK: return "ok";
+        case CAGEO_ERR_ARG: return "invalid argument";
+        case CAGEO_ERR_HIP: return "HIP runtime error";
+        case CAGEO_ERR_NOMEM: return "device allocation failed";
+        case CAGEO_ERR_NOT_CONVERGED: return "iteration cap reached";
+        case CAGEO_ERR_STATE: return "invalid call order";
+        case CAGEO_ERR_RANGE: return "input outside the supported domain";
+        default: return "unknown status";
+    }
+}
+
+const char* cageo_version(void) { return "cageo_hip 0.1 gfx950"; }
+
+int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
+    if (!out || W <= 0 || H <= 0 || (int64_t)W * H >= (int64_t)1 << 31) return CAGEO_ERR_ARG;
+    cageo_context* ctx = new cageo_context();
+    Context* c = &ctx->c;
+    c->W = W;
+    c->H = H;
+    c->n = (size_t)W * H;
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete ctx;
+        return CAGEO_ERR_HIP;
+    }
+    c->ntx = (W + cageo::TILE - 1) / cageo::TILE;
+    c->nty = (H + cageo::TILE - 1) / cageo::TILE;
+    c->ntiles = c->ntx * c->nty;
+    const size_t n = c->n;
+    int st = CAGEO_OK;
+    auto A = [&](int s) {
+        if (st == CAGEO_OK) st = s;
+    };
+    A(cageo::alloc(&c->d_work[0], c->ntiles));
+    A(cageo::alloc(&c->d_work[1], c->ntiles));
+    A(cageo::alloc(&c->d_mark, c->ntiles));
+    A(cageo::alloc(&c->d_scal, cageo::SC_NSLOTS));
+    A(cageo::alloc(&c->d_sflags, n));
+    A(cageo::alloc(&c->d_slabel, n));
+    const int maxoff = (2 * CAGEO_MAX_BEVEL_RADIUS + 1) * (2 * CAGEO_MAX_BEVEL_RADIUS + 1);
+    A(cageo::alloc(&c->d_off, 2 * maxoff));
+    A(cageo::alloc(&c->d_factor, maxoff));
+    A(cageo::alloc(&c->d_a, n));
+    A(cageo::alloc(&c->d_b, n));
+    A(cageo::alloc(&c->d_z, n));
+    A(cageo::alloc(&c->d_w, n));
+    A(cageo::alloc(&c->d_st, n));
+    A(cageo::alloc(&c->d_fd, 2 * n));
+    A(cageo::alloc(&c->d_dist, n));
+    A(cageo::alloc(&c->d_lab, n));
+    A(cageo::alloc(&c->d_ptr, n));
+    if (st == CAGEO_OK &&
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_scal), cageo::SC_NSLOTS * sizeof(uint32_t)) !=
+            hipSuccess)
+        st = CAGEO_ERR_NOMEM;
+    if (st == CAGEO_OK &&
+        (hipMemsetAsync(c->d_mark, 0, (size_t)c->ntiles * sizeof(uint32_t), c->stream) != hipSuccess ||
+         hipMemsetAsync(c->d_scal, 0, cageo::SC_NSLOTS * sizeof(uint32_t), c->stream) != hipSuccess ||
+         hipStreamSynchronize(c->stream) != hipSuccess))
+        st = CAGEO_ERR_HIP;
+    for (hipEvent_t& e : c->ev)
+        if (st == CAGEO_OK && hipEventCreate(&e) != hipSuccess) st = CAGEO_ERR_HIP;
+    if (st != CAGEO_OK) {
+        cageo::free_all(c);
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return CAGEO_OK;
+}
+
+int cageo_context_destroy(cageo_context* ctx) {
+    if (!ctx) return CAGEO_ERR_ARG;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    cageo::free_all(&ctx->c);
+    delete ctx;
+    return CAGEO_OK;
+}
+
+int cageo_set_filter(cageo_context* ctx, int model, int radius, const float* taps) {
+    if (!ctx || !taps || radius < 0) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    // The reference wraps a source index once (ConvolutionAlgorithmCPUOpenMP.h:
+    // 127-130), which stays in the grid only for radius <= width, height.
+    if (radius > c->W || radius > c->H) return CAGEO_ERR_ARG;
+    if (model != CAGEO_FILTER_GAUSSIAN && model != CAGEO_FILTER_EXPONENTIAL) return CAGEO_ERR_ARG;
+    const int L = 2 * radius + 1;
+    std::vector<float> host;
+    int fs = L;
+    if (model == CAGEO_FILTER_GAUSSIAN) {
+        if ((size_t)(cageo::TILE + 2 * radius) * cageo::TILE * sizeof(float) > 160 * 1024)
+            return CAGEO_ERR_ARG;
+        host.assign(taps, taps + L);
+    } else {
+        int ty, S;
+        size_t lds;
+        if (cageo::conv_2d_config(radius, &ty, &S, &lds) != CAGEO_OK) return CAGEO_ERR_ARG;
+        fs = (L + 3 + 3) & ~3;
+        host.assign((size_t)L * fs, 0.0f);
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) host[(size_t)y * fs + x] = taps[(size_t)y * L + x];
+    }
+    if (c->d_filter) (void)hipFree(c->d_filter);
+    c->d_filter = nullptr;
+    CAGEO_TRY(cageo::alloc(&c->d_filter, host.size()));
+    CAGEO_HIP_TRY(hipMemcpyAsync(c->d_filter, host.data(), host.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->model = model;
+    c->radius = radius;
+    c->filter_len = (int)host.size();
+    return CAGEO_OK;
+}
+
+int cageo_random_field(cageo_context* ctx, float* d_field, uint64_t seed, uint32_t realization,
+                       uint32_t stream_id, float mean, float std_dev) {
+    if (!ctx || !d_field) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::launch_random_field(c, d_field, seed, realization, stream_id, mean, std_dev));
+    return cageo::sync(c);
+}
+
+int cageo_convolve(cageo_context* ctx, float* d_field) {
+    if (!ctx || !d_field) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::launch_convolve(c, d_field, c->d_b));
+    return cageo::sync(c);
+}
+
+int cageo_realistic_dem(cageo_context* ctx, const float* d_dem, float* d_field, float nodata) {
+    if (!ctx || !d_dem || !d_field) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::launch_realistic(c, d_dem, d_field, nodata));
+    return cageo::sync(c);
+}
+
+int cageo_stream_burn(cageo_context* ctx, float* d_z, const float* d_stream, int bevel_radius,
+                      float nodata_dem, float nodata_stream) {
+    if (!ctx || !d_z || !d_stream) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::prepare_offsets(c, bevel_radius));
+    // scratch: d_st holds the stream flags, d_dist the stream labels
+    CAGEO_TRY(cageo::launch_stream_flags(c, d_stream, nodata_stream, c->d_st, c->d_dist));
+    CAGEO_HIP_TRY(hipMemcpyAsync(c->d_b, d_z, c->n * sizeof(float), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    CAGEO_TRY(cageo::launch_burn(c, c->d_b, d_z, c->d_st, nodata_dem));
+    return cageo::sync(c);
+}
+
+int cageo_pit_fill(cageo_context* ctx, float* d_filled, float* d_dem, float nodata) {
+    if (!ctx || !d_filled || !d_dem || d_filled == d_dem) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::reset_minmax(c));
+    CAGEO_TRY(cageo::launch_quantize(c, d_dem, nodata));
+    CAGEO_TRY(cageo::pit_fill_quantized(c, d_dem, d_filled, nodata, true));
+    return cageo::sync(c);
+}
+
+int cageo_flow_routing(cageo_context* ctx, int8_t* d_flowdir, const float* d_filled,
+                       float /*nodata_dem*/) {
+    if (!ctx || !d_flowdir || !d_filled) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::flow_routing(c, d_filled, d_flowdir));
+    return cageo::sync(c);
+}
+
+int cageo_flow_tracing(cageo_context* ctx, int32_t* d_labels, const int8_t* d_flowdir,
+                       const float* d_stream, int64_t* long_paths) {
+    if (!ctx || !d_labels || !d_flowdir || !d_stream) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    // The trace rule tests stream >= 0 only; the nodata value is irrelevant.
+    CAGEO_TRY(cageo::launch_stream_flags(c, d_stream, -__builtin_huge_valf(), c->d_st, c->d_dist));
+    CAGEO_TRY(cageo::flow_tracing(c, d_flowdir, c->d_st, c->d_dist, d_labels, long_paths));
+    return cageo::sync(c);
+}
+
+int cageo_catchment_border(cageo_context* ctx, float* d_mc, const int32_t* d_labels) {
+    if (!ctx || !d_mc || !d_labels) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::launch_border(c, d_mc, d_labels));
+    return cageo::sync(c);
+}
+
+int cageo_mc_setup(cageo_context* ctx, const float* d_dem, const float* d_stream,
+                   float nodata_dem, float nodata_stream, int bevel_radius, float mean,
+                   float std_dev) {
+    if (!ctx || !d_dem || !d_stream) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::prepare_offsets(c, bevel_radius));
+    CAGEO_TRY(cageo::launch_stream_flags(c, d_stream, nodata_stream, c->d_sflags, c->d_slabel));
+    c->dem = d_dem;
+    c->streamr = d_stream;
+    c->nodata_dem = nodata_dem;
+    c->nodata_stream = nodata_stream;
+    c->bevel = bevel_radius;
+    c->mean = mean;
+    c->std_dev = std_dev;
+    c->mc_ready = true;
+    return cageo::sync(c);
+}
+
+int cageo_run_realizations(cageo_context* ctx, float* d_mc, int n, uint64_t seed,
+                           uint32_t first) {
+    if (!ctx || !d_mc || n < 0) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    if (!c->mc_ready || c->model < 0) return CAGEO_ERR_STATE;
+    const bool timed = c->timing;
+    if (timed) std::fill(c->stage_ms, c->stage_ms + 7, 0.0f);
+    for (int r = 0; r < n; ++r) {
+        if (timed) CAGEO_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        // RandomFieldAlgorithm::execute(rand, seed), main.cpp:189-192
+        CAGEO_TRY(cageo::launch_random_field(c, c->d_a, seed, first + (uint32_t)r, 0u, c->mean,
+                                             c->std_dev));
+        if (timed) CAGEO_HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+        CAGEO_TRY(cageo::realization_body(c, d_mc, timed, 2));
+        if (timed) {
+            CAGEO_HIP_TRY(hipEventSynchronize(c->ev[7]));
+            for (int s = 0; s < 7; ++s) {
+                float ms = 0.0f;
+                CAGEO_HIP_TRY(hipEventElapsedTime(&ms, c->ev[s], c->ev[s + 1]));
+                c->stage_ms[s] += ms;
+            }
+        }
+    }
+    return cageo::sync(c);
+}
+
+int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc,
+                                 float* d_filled_out, int8_t* d_flowdir_out,
+                                 int32_t* d_labels_out) {
+    if (!ctx || !d_noise) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    if (!c->mc_ready || c->model < 0) return CAGEO_ERR_STATE;
+    CAGEO_HIP_TRY(hipMemcpyAsync(c->d_a, d_noise, c->n * sizeof(float), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    CAGEO_TRY(cageo::realization_body(c, d_mc, false, 0));
+    if (d_filled_out)
+        CAGEO_HIP_TRY(hipMemcpyAsync(d_filled_out, c->d_w, c->n * sizeof(float),
+                                     hipMemcpyDeviceToDevice, c->stream));
+    if (d_flowdir_out)
+        CAGEO_HIP_TRY(hipMemcpyAsync(d_flowdir_out, c->d_fd, 2 * c->n, hipMemcpyDeviceToDevice,
+                                     c->stream));
+    if (d_labels_out)
+        CAGEO_HIP_TRY(hipMemcpyAsync(d_labels_out, c->d_lab, c->n * sizeof(int32_t),
+                                     hipMemcpyDeviceToDevice, c->stream));
+    return cageo::sync(c);
+}
+
+int cageo_stage_times(cageo_context* ctx, float* out, int enable) {
+    if (!ctx) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    if (out) std::memcpy(out, c->stage_ms, sizeof(c->stage_ms));
+    c->timing = enable != 0;
+    return CAGEO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// Shared definitions of the gfx950 library: the context (device workspace),
+// launch helpers and small device utilities.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "cageo_hip.h"
+
+namespace cageo {
+
+// Tiles of the iterative stencil kernels: 64 x 64 cells, 256 threads, each
+// thread owning a vertical strip of 16 consecutive cells of one column so a
+// wavefront reads 64 consecutive LDS words per step.
+constexpr int TILE = 64;
+constexpr int TILE_THREADS = 256;
+constexpr int STRIP = TILE * TILE / TILE_THREADS;  // 16
+constexpr int HALO_W = TILE + 2;                   // tile + 1-cell halo
+
+// Pit-fill cell states.
+enum : uint8_t { PF_NODATA = 0, PF_FIXED_LIVE = 1, PF_FIXED_DEAD = 2, PF_OPEN = 3 };
+
+// Stream classification bits (precomputed once per run from the stream raster).
+enum : uint8_t { SF_BURN = 1, SF_TRACE = 2 };
+
+// Scalars kept in device memory (one 64-word block, read back in one copy).
+enum ScalarSlot : int {
+    SC_MIN = 0,        // ordered-uint min of the quantized DEM (incl. nodata)
+    SC_MAX = 1,        // ordered-uint max
+    SC_BAD = 2,        // negative / NaN elevation seen
+    SC_COUNT0 = 3,     // worklist count, buffer 0
+    SC_COUNT1 = 4,     // worklist count, buffer 1
+    SC_CERTIFIED = 5,  // cells certified in this round
+    SC_OPEN = 6,       // OPEN cells left
+    SC_CHANGED = 7,    // generic "something changed" flag
+    SC_LONGPATH = 8,   // cells whose trace is longer than the walk guard
+    SC_EPOCH = 9,      // unused on device; reserved
+    SC_NSLOTS = 64
+};
+
+struct Context {
+    int W = 0, H = 0;
+    size_t n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // tiles
+    int ntx = 0, nty = 0, ntiles = 0;
+    int* d_work[2] = {nullptr, nullptr};
+    uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
+    uint32_t epoch = 0;
+
+    // scalars
+    uint32_t* d_scal = nullptr;
+    uint32_t* h_scal = nullptr;  // pinned mirror
+
+    // filter
+    int model = -1, radius = 0, filter_len = 0;
+    float* d_filter = nullptr;
+
+    // bound Monte Carlo inputs
+    const float* dem = nullptr;
+    const float* streamr = nullptr;
+    float nodata_dem = 0.0f, nodata_stream = 0.0f, mean = 0.0f, std_dev = 1.0f;
+    int bevel = 2;
+    bool mc_ready = false;
+    uint8_t* d_sflags = nullptr;
+    int32_t* d_slabel = nullptr;
+
+    // burn offsets (x, y, factor), ordered as the reference's search array
+    int n_offsets = 0;
+    int* d_off = nullptr;      // 2 ints per offset
+    float* d_factor = nullptr;  // one factor per offset
+    int off_radius = -1;
+
+    // work rasters
+    float* d_a = nullptr;  // noise / error field
+    float* d_b = nullptr;  // convolution buffer
+    float* d_z = nullptr;  // quantized burned DEM
+    float* d_w = nullptr;  // filled DEM
+    uint8_t* d_st = nullptr;
+    int8_t* d_fd = nullptr;
+    int32_t* d_dist = nullptr;
+    int32_t* d_lab = nullptr;
+    uint64_t* d_ptr = nullptr;
+
+    // timing
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    float stage_ms[7] = {};
+};
+
+#define CAGEO_HIP_TRY(expr)                                                        \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            std::fprintf(stderr, "cageo: %s failed: %s (%s:%d)\n", #expr,          \
+                         hipGetErrorString(_e), __FILE__, __LINE__);               \
+            return CAGEO_ERR_HIP;                                                  \
+        }                                                                          \
+    } while (0)
+
+#define CAGEO_TRY(expr)             \
+    do {                            \
+        int _s = (expr);            \
+        if (_s != CAGEO_OK) return _s; \
+    } while (0)
+
+// Float <-> order-preserving uint (for atomic min/max of floats).
+__host__ __device__ inline uint32_t float_to_ordered(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float ordered_to_float(uint32_t u) {
+    uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    return __builtin_bit_cast(float, b);
+}
+
+// toIndex / toElev of PitFillingAlgorithmCPUOpenMP.h:55-65 (PREC = 1000).
+__host__ __device__ inline uint64_t pf_to_index(float elev, float min_elev) {
+    return (uint64_t)(1000.0f * (elev - min_elev));
+}
+__host__ __device__ inline float pf_to_elev(uint64_t index, float min_elev) {
+    return (float)index / 1000.0f + min_elev;
+}
+// A cell holding value v expands its neighbours in the reference's bucket
+// flood only if v survives the bucket round trip.
+__host__ __device__ inline bool pf_live(float v, float min_elev) {
+    return pf_to_elev(pf_to_index(v, min_elev), min_elev) == v;
+}
+
+// Worklist helpers --------------------------------------------------------
+int worklist_all_tiles(Context* c, int buf);
+int read_scalars(Context* c);
+uint32_t next_epoch(Context* c);
+
+// Stage drivers implemented in the stage translation units -----------------
+int launch_random_field(Context* c, float* out, uint64_t seed, uint32_t realization,
+                        uint32_t stream_id, float mean, float std_dev);
+int launch_convolve(Context* c, float* field, float* buf);
+int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes);
+int launch_realistic(Context* c, const float* dem, float* field, float nodata);
+int launch_burn(Context* c, const float* zin, float* zout, const uint8_t* sflags,
+                float nodata_dem);
+int prepare_offsets(Context* c, int radius);
+int launch_stream_flags(Context* c, const float* stream, float nodata_stream, uint8_t* sflags,
+                        int32_t* slabel);
+int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq);
+int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool stats_ready);
+int launch_quantize(Context* c, float* dem, float nodata);
+int flow_routing(Context* c, const float* z, int8_t* fd);
+int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,
+                 int32_t* lab, int64_t* long_paths);
+int launch_border(Context* c, float* mc, const int32_t* lab);
+
+}  // namespace cageo
+
+struct cageo_context {
+    cageo::Context c;
+};

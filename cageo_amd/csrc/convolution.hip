@@ -1,0 +1,252 @@
+// Convolution stage: ConvolutionAlgorithm<T>::execute, ConvolutionAlgorithm.h:
+// 296-312, with the accumulation order of the reference's CPU loops
+// (ConvolutionAlgorithmCPUOpenMP.h:105-248): sum starts at 0.0f and adds
+// in[..] * f[k] for ascending taps (2-D: filter row outer, column inner),
+// every product and sum rounded separately (this file is compiled with
+// -ffp-contract=off), periodic wrap of the source index.
+//
+// Gaussian: columns pass into a buffer, then rows pass back into the field
+// (ConvolutionAlgorithmCPUOpenMP.h:80-92).  Exponential: one 2-D pass,
+// out of place (:94-102).
+//
+// Layout: output tiles of 64 x 64 cells staged through LDS with their
+// wrapped halo; taps are wave-uniform and come through scalar loads.
+#include "common.hpp"
+
+namespace cageo {
+
+namespace {
+
+__device__ inline int wrap_index(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// Columns pass: out[y][x] = sum_fy in[wrap(y - r + fy)][x] * f[fy].
+__global__ __launch_bounds__(256) void k_conv_cols(float* __restrict__ out,
+                                                   const float* __restrict__ in,
+                                                   const float* __restrict__ f, int r, int W,
+                                                   int H) {
+    extern __shared__ float lds[];  // (64 + 2r) rows x 64
+    const int L = 2 * r + 1;
+    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int rows = TILE + 2 * r;
+    const int x = x0 + tx;
+    for (int k = ty; k < rows; k += 4) {
+        int gy = wrap_index(y0 - r + k, H);
+        lds[k * TILE + tx] = (x < W) ? in[(size_t)gy * W + x] : 0.0f;
+    }
+    __syncthreads();
+    float acc[STRIP];
+#pragma unroll
+    for (int j = 0; j < STRIP; ++j) acc[j] = 0.0f;
+    const float* base = lds + (ty * STRIP) * TILE + tx;
+    for (int fy = 0; fy < L; ++fy) {
+        const float t = f[fy];
+        const float* p = base + fy * TILE;
+#pragma unroll
+        for (int j = 0; j < STRIP; ++j) {
+            float prod = p[j * TILE] * t;
+            acc[j] = acc[j] + prod;
+        }
+    }
+    if (x < W) {
+#pragma unroll
+        for (int j = 0; j < STRIP; ++j) {
+            int y = y0 + ty * STRIP + j;
+            if (y < H) out[(size_t)y * W + x] = acc[j];
+        }
+    }
+}
+
+// Rows pass: out[y][x] = sum_fx in[y][wrap(x - r + fx)] * f[fx].
+__global__ __launch_bounds__(256) void k_conv_rows(float* __restrict__ out,
+                                                   const float* __restrict__ in,
+                                                   const float* __restrict__ f, int r, int W,
+                                                   int H) {
+    extern __shared__ float lds[];  // 64 rows x (64 + 2r)
+    const int L = 2 * r + 1;
+    const int S = TILE + 2 * r;
+    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int idx = threadIdx.x; idx < TILE * S; idx += 256) {
+        int row = idx / S, k = idx - row * S;
+        int gy = y0 + row;
+        int gx = wrap_index(x0 - r + k, W);
+        lds[idx] = (gy < H) ? in[(size_t)gy * W + gx] : 0.0f;
+    }
+    __syncthreads();
+    float acc[STRIP];
+#pragma unroll
+    for (int j = 0; j < STRIP; ++j) acc[j] = 0.0f;
+    const float* base = lds + (ty * STRIP) * S + tx;
+    for (int fx = 0; fx < L; ++fx) {
+        const float t = f[fx];
+#pragma unroll
+        for (int j = 0; j < STRIP; ++j) {
+            float prod = base[j * S + fx] * t;
+            acc[j] = acc[j] + prod;
+        }
+    }
+    const int x = x0 + tx;
+    if (x < W) {
+#pragma unroll
+        for (int j = 0; j < STRIP; ++j) {
+            int y = y0 + ty * STRIP + j;
+            if (y < H) out[(size_t)y * W + x] = acc[j];
+        }
+    }
+}
+
+// Full 2-D pass (exponential model).  Each thread owns a 4 x 4 block of
+// outputs; for every filter row the 4 x 4 outputs slide over one LDS row
+// segment read as float4, with the taps of that row held in scalar
+// registers.  The filter is stored with a row stride `fs` (multiple of 4,
+// zero padded) so a tap quadruple is one aligned scalar load.
+template <int TY>
+__global__ __launch_bounds__(16 * TY) void k_conv_2d(float* __restrict__ out,
+                                                     const float* __restrict__ in,
+                                                     const float* __restrict__ f, int r, int fs,
+                                                     int S, int W, int H) {
+    extern __shared__ float lds[];  // (4*TY + 2r) rows x S
+    constexpr int NT = 16 * TY;
+    constexpr int OUT_H = 4 * TY;
+    const int L = 2 * r + 1;
+    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * OUT_H;
+    const int rows = OUT_H + 2 * r;
+    const int cols = TILE + 2 * r;
+    for (int k = threadIdx.y * 16 + threadIdx.x; k < rows * S; k += NT) {
+        int rr = k / S, cc = k - rr * S;
+        float v = 0.0f;
+        if (cc < cols) {
+            int gy = wrap_index(y0 - r + rr, H);
+            int gx = wrap_index(x0 - r + cc, W);
+            v = in[(size_t)gy * W + gx];
+        }
+        lds[k] = v;
+    }
+    __syncthreads();
+
+    const int tx4 = threadIdx.x, ty4 = threadIdx.y;
+    float acc[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[j][k] = 0.0f;
+
+    const int nb = L >> 2;      // full tap quadruples
+    const int rem = L & 3;      // 1 or 3 (L is odd)
+    for (int fy = 0; fy < L; ++fy) {
+        const float* frow = f + (size_t)fy * fs;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float* row = lds + (ty4 * 4 + j + fy) * S + tx4 * 4;
+            float4 cur = *reinterpret_cast<const float4*>(row);
+            for (int b = 0; b < nb; ++b) {
+                float4 nxt = *reinterpret_cast<const float4*>(row + 4 * b + 4);
+                float4 t = *reinterpret_cast<const float4*>(frow + 4 * b);
+                float win[8] = {cur.x, cur.y, cur.z, cur.w, nxt.x, nxt.y, nxt.z, nxt.w};
+                float tap[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        float prod = win[k + i] * tap[i];
+                        acc[j][k] = acc[j][k] + prod;
+                    }
+                cur = nxt;
+            }
+            if (rem) {
+                float4 nxt = *reinterpret_cast<const float4*>(row + 4 * nb + 4);
+                float4 t = *reinterpret_cast<const float4*>(frow + 4 * nb);
+                float win[8] = {cur.x, cur.y, cur.z, cur.w, nxt.x, nxt.y, nxt.z, nxt.w};
+                float tap[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    if (i < rem) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            float prod = win[k + i] * tap[i];
+                            acc[j][k] = acc[j][k] + prod;
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int y = y0 + ty4 * 4 + j;
+        if (y >= H) continue;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int x = x0 + tx4 * 4 + k;
+            if (x < W) out[(size_t)y * W + x] = acc[j][k];
+        }
+    }
+}
+
+}  // namespace
+
+// Largest LDS allocation a workgroup may request on gfx950.
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
+int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes) {
+    // S: LDS row stride, covering 64 + 2r columns plus one float4 of slack
+    // for the remainder read, rounded to a multiple of 4.
+    const int s = ((TILE + 2 * r + 4) + 3) & ~3;
+    const int tys[3] = {16, 8, 4};
+    for (int t : tys) {
+        size_t bytes = (size_t)(4 * t + 2 * r) * s * sizeof(float);
+        if (bytes <= LDS_LIMIT) {
+            *ty = t;
+            *S = s;
+            *lds_bytes = bytes;
+            return CAGEO_OK;
+        }
+    }
+    return CAGEO_ERR_ARG;
+}
+
+int launch_convolve(Context* c, float* field, float* buf) {
+    const int W = c->W, H = c->H, r = c->radius;
+    if (c->model < 0 || !c->d_filter) return CAGEO_ERR_STATE;
+    dim3 grid((W + TILE - 1) / TILE, (H + TILE - 1) / TILE);
+    if (c->model == CAGEO_FILTER_GAUSSIAN) {
+        size_t lds = (size_t)(TILE + 2 * r) * TILE * sizeof(float);
+        if (lds > LDS_LIMIT) return CAGEO_ERR_ARG;
+        hipLaunchKernelGGL(k_conv_cols, grid, dim3(256), lds, c->stream, buf, field,
+                           c->d_filter, r, W, H);
+        CAGEO_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_conv_rows, grid, dim3(256), lds, c->stream, field, buf,
+                           c->d_filter, r, W, H);
+        CAGEO_HIP_TRY(hipGetLastError());
+        return CAGEO_OK;
+    }
+    int ty, S;
+    size_t lds;
+    CAGEO_TRY(conv_2d_config(r, &ty, &S, &lds));
+    const int fs = (2 * r + 1 + 3 + 3) & ~3;
+    dim3 g2((W + TILE - 1) / TILE, (H + 4 * ty - 1) / (4 * ty));
+    switch (ty) {
+        case 16:
+            hipLaunchKernelGGL(k_conv_2d<16>, g2, dim3(16, 16), lds, c->stream, buf, field,
+                               c->d_filter, r, fs, S, W, H);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_conv_2d<8>, g2, dim3(16, 8), lds, c->stream, buf, field,
+                               c->d_filter, r, fs, S, W, H);
+            break;
+        default:
+            hipLaunchKernelGGL(k_conv_2d<4>, g2, dim3(16, 4), lds, c->stream, buf, field,
+                               c->d_filter, r, fs, S, W, H);
+            break;
+    }
+    CAGEO_HIP_TRY(hipGetLastError());
+    CAGEO_HIP_TRY(hipMemcpyAsync(field, buf, c->n * sizeof(float), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    return CAGEO_OK;
+}
+
+}  // namespace cageo

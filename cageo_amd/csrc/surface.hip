@@ -1,0 +1,254 @@
+// Surface stages that turn the error field into the DEM the pit fill sees:
+//  - RealisticDemAlgorithm<T,U>::execute, RealisticDemAlgorithm.h:49-70
+//    (AddSurfacesAlgorithmCPUOpenMP.h:36-56 then
+//    RemoveNegativeElevationsAlgorithmCPUOpenMP.h:34-48);
+//  - StreamBurningAlgorithm<T,U>::execute, StreamBurningAlgorithm.h:116-134
+//    with performStreamBurning, StreamBurningAlgorithmCPUOpenMP.h:32-82;
+//  - the mm quantisation that opens performPitFilling,
+//    PitFillingAlgorithmCPUOpenMP.h:84-106.
+//
+// Burning semantics.  The reference updates z in place while scanning each
+// cell's circular search array; with one thread in raster order a stream
+// neighbour n that precedes the cell has already been set to 0, one that
+// follows still holds its realistic-DEM value.  Non-stream cell i takes
+//   z_i * factor(o)  for the first offset o (order of
+//                    StreamBurningAlgorithm.h:140-167) whose cell n is in the
+//                    grid, is a stream cell, and has seen(n) != nodata_dem,
+//                    seen(n) = n < i ? 0 : z_n;
+//   z_i              otherwise;
+// stream cells become 0.  Evaluated out of place here so every cell reads the
+// unburned values.
+#include <cmath>
+#include <list>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cageo {
+
+namespace {
+
+
+__global__ __launch_bounds__(256) void k_stream_flags(const float* __restrict__ stream,
+                                                      int64_t n, float nodata_stream,
+                                                      uint8_t* __restrict__ sflags,
+                                                      int32_t* __restrict__ slabel) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float s = stream[i];
+        uint8_t f = 0;
+        if (s != nodata_stream) f |= SF_BURN;  // StreamBurningAlgorithmCPUOpenMP.h:46
+        if (s >= 0.0f) f |= SF_TRACE;          // FlowTracingAlgorithmCPUOpenMP.h:52,88
+        sflags[i] = f;
+        // FlowTracingAlgorithmCPUOpenMP.h:89: label = -stream converted to int
+        slabel[i] = (s >= 0.0f) ? (int32_t)(-s) : 0;
+    }
+}
+
+__device__ inline float realistic_value(float d, float e, float nodata) {
+    float v = (d != nodata) ? d + e : e;
+    if (v <= 0.0f && v != nodata) v = 1.0f;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_realistic(const float* __restrict__ dem,
+                                                   float* __restrict__ field, int64_t n,
+                                                   float nodata) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        field[i] = realistic_value(dem[i], field[i], nodata);
+}
+
+// Burned value of cell (x, y) given a source of unburned values `zr(n)`.
+template <class ZR>
+__device__ inline float burn_value(int x, int y, int W, int H, const uint8_t* __restrict__ sf,
+                                   const int* __restrict__ off, const float* __restrict__ fac,
+                                   int n_off, float nodata_dem, ZR zr) {
+    const int64_t idx = (int64_t)y * W + x;
+    if (sf[idx] & SF_BURN) return 0.0f;
+    for (int o = 0; o < n_off; ++o) {
+        int nx = x + off[2 * o], ny = y + off[2 * o + 1];
+        if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
+        int64_t ni = (int64_t)ny * W + nx;
+        if (!(sf[ni] & SF_BURN)) continue;
+        float seen = (ni < idx) ? 0.0f : zr(ni);
+        if (seen != nodata_dem) return zr(idx) * fac[o];
+    }
+    return zr(idx);
+}
+
+__global__ __launch_bounds__(256) void k_burn(const float* __restrict__ zin,
+                                              float* __restrict__ zout,
+                                              const uint8_t* __restrict__ sf,
+                                              const int* __restrict__ off,
+                                              const float* __restrict__ fac, int n_off, int W,
+                                              int H, float nodata_dem) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    zout[(size_t)y * W + x] = burn_value(x, y, W, H, sf, off, fac, n_off, nodata_dem,
+                                         [&](int64_t i) { return zin[i]; });
+}
+
+// Quantisation of PitFillingAlgorithmCPUOpenMP.h:84-106 for one value.
+__device__ inline float quantize_mm(float v, float nodata, uint32_t* bad) {
+    if (v == nodata) return v;
+    if (v < 0.0f || v != v) {
+        *bad = 1u;
+        return v;
+    }
+    uint64_t q = pf_to_index(v, 0.0f);
+    if (v != 0.0f && q == 0) q = 1;
+    return pf_to_elev(q, 0.0f);
+}
+
+// Block-wide min/max of ordered uints, then one atomic per block.
+__device__ inline void block_minmax(uint32_t mn, uint32_t mx, uint32_t bad, uint32_t* scal) {
+    __shared__ uint32_t s_mn, s_mx, s_bad;
+    if (threadIdx.x == 0) {
+        s_mn = 0xffffffffu;
+        s_mx = 0u;
+        s_bad = 0u;
+    }
+    __syncthreads();
+    // wave-level reduction first
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+        bad |= (uint32_t)__shfl_xor((int)bad, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&s_mn, mn);
+        atomicMax(&s_mx, mx);
+        atomicOr(&s_bad, bad);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMin(&scal[SC_MIN], s_mn);
+        atomicMax(&scal[SC_MAX], s_mx);
+        if (s_bad) atomicOr(&scal[SC_BAD], s_bad);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_quantize_minmax(float* __restrict__ z, int64_t n,
+                                                         float nodata,
+                                                         uint32_t* __restrict__ scal) {
+    uint32_t mn = 0xffffffffu, mx = 0u, bad = 0u;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float q = quantize_mm(z[i], nodata, &bad);
+        z[i] = q;
+        uint32_t o = float_to_ordered(q);
+        mn = min(mn, o);
+        mx = max(mx, o);
+    }
+    block_minmax(mn, mx, bad, scal);
+}
+
+// Fused realistic DEM + burn + quantisation + min/max for the Monte Carlo
+// driver: one pass over dem, error field and stream flags.
+__global__ __launch_bounds__(256) void k_realistic_burn_quant(
+    const float* __restrict__ dem, const float* __restrict__ e, const uint8_t* __restrict__ sf,
+    const int* __restrict__ off, const float* __restrict__ fac, int n_off, int W, int H,
+    float nodata_dem, float* __restrict__ zq, uint32_t* __restrict__ scal) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    uint32_t mn = 0xffffffffu, mx = 0u, bad = 0u;
+    if (x < W && y < H) {
+        float zb = burn_value(x, y, W, H, sf, off, fac, n_off, nodata_dem, [&](int64_t i) {
+            return realistic_value(dem[i], e[i], nodata_dem);
+        });
+        float q = quantize_mm(zb, nodata_dem, &bad);
+        zq[(size_t)y * W + x] = q;
+        mn = mx = float_to_ordered(q);
+    }
+    block_minmax(mn, mx, bad, scal);
+}
+
+inline unsigned grid_1d(size_t n) {
+    size_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    return (unsigned)(g ? g : 1);
+}
+
+}  // namespace
+
+// Circular search array of StreamBurningAlgorithm.h:140-167 ordered by
+// (d^2, y, x) as coordDistCompare (StreamBurningCommon.cpp:13-28), with the
+// bevel factor sqrt(d^2)/R of StreamBurningAlgorithmCPUOpenMP.h:67-71
+// evaluated in float on the host.
+int prepare_offsets(Context* c, int R) {
+    if (R < 0 || R > CAGEO_MAX_BEVEL_RADIUS) return CAGEO_ERR_ARG;
+    if (c->off_radius == R) return CAGEO_OK;
+    struct O {
+        int d, x, y;
+    };
+    std::list<O> l;
+    for (int y = -R; y <= R; ++y)
+        for (int x = -R; x <= R; ++x)
+            if ((x || y) && x * x + y * y <= R * R) l.push_back({x * x + y * y, x, y});
+    l.sort([](const O& a, const O& b) {
+        return a.d < b.d || (a.d == b.d && (a.y < b.y || (a.y == b.y && a.x < b.x)));
+    });
+    std::vector<int> off;
+    std::vector<float> fac;
+    for (const O& o : l) {
+        off.push_back(o.x);
+        off.push_back(o.y);
+        float xd = (float)(-o.x), yd = (float)(-o.y);
+        float dist = std::sqrt(xd * xd + yd * yd);
+        fac.push_back(dist / (float)R);
+    }
+    c->n_offsets = (int)fac.size();
+    if (c->n_offsets > 0) {
+        CAGEO_HIP_TRY(hipMemcpyAsync(c->d_off, off.data(), off.size() * sizeof(int),
+                                     hipMemcpyHostToDevice, c->stream));
+        CAGEO_HIP_TRY(hipMemcpyAsync(c->d_factor, fac.data(), fac.size() * sizeof(float),
+                                     hipMemcpyHostToDevice, c->stream));
+        CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    c->off_radius = R;
+    return CAGEO_OK;
+}
+
+int launch_stream_flags(Context* c, const float* stream, float nodata_stream, uint8_t* sflags,
+                        int32_t* slabel) {
+    hipLaunchKernelGGL(k_stream_flags, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, stream,
+                       (int64_t)c->n, nodata_stream, sflags, slabel);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+int launch_realistic(Context* c, const float* dem, float* field, float nodata) {
+    hipLaunchKernelGGL(k_realistic, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, dem, field,
+                       (int64_t)c->n, nodata);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+int launch_burn(Context* c, const float* zin, float* zout, const uint8_t* sflags,
+                float nodata_dem) {
+    dim3 grid((c->W + 63) / 64, (c->H + 3) / 4);
+    hipLaunchKernelGGL(k_burn, grid, dim3(256), 0, c->stream, zin, zout, sflags, c->d_off,
+                       c->d_factor, c->n_offsets, c->W, c->H, nodata_dem);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+int launch_quantize(Context* c, float* dem, float nodata) {
+    hipLaunchKernelGGL(k_quantize_minmax, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, dem,
+                       (int64_t)c->n, nodata, c->d_scal);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq) {
+    dim3 grid((c->W + 63) / 64, (c->H + 3) / 4);
+    hipLaunchKernelGGL(k_realistic_burn_quant, grid, dim3(256), 0, c->stream, dem, e,
+                       c->d_sflags, c->d_off, c->d_factor, c->n_offsets, c->W, c->H,
+                       c->nodata_dem, zq, c->d_scal);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+}  // namespace cageo

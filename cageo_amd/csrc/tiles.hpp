@@ -1,0 +1,83 @@
+// Tile worklists for the iterative stencil stages (pit fill, certification,
+// flat resolution).  A pass launches one 256-thread workgroup per listed
+// 64x64 tile; the tile iterates to a local fixed point in LDS against the
+// halo it loaded, writes back, and lists the neighbour tiles whose halo it
+// changed (border cells only) for the next pass.  A per-tile epoch stamp
+// de-duplicates the next list without clearing it.  The host loop stops when
+// a pass lists nothing: then every tile is at a fixed point with respect to
+// its neighbours' final values.
+#pragma once
+
+#include "common.hpp"
+
+namespace cageo {
+
+// Neighbour-tile bits: N, NE, E, SE, S, SW, W, NW.
+__device__ constexpr int TDX[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+__device__ constexpr int TDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+
+// Bits of the neighbour tiles whose halo contains local cell (lx, ly).
+__device__ inline uint32_t border_bits(int lx, int ly) {
+    uint32_t b = 0;
+    const bool n = ly == 0, s = ly == TILE - 1, w = lx == 0, e = lx == TILE - 1;
+    if (n) b |= 1u << 0;
+    if (n && e) b |= 1u << 1;
+    if (e) b |= 1u << 2;
+    if (s && e) b |= 1u << 3;
+    if (s) b |= 1u << 4;
+    if (s && w) b |= 1u << 5;
+    if (w) b |= 1u << 6;
+    if (n && w) b |= 1u << 7;
+    return b;
+}
+
+// Called by one thread per workgroup.
+__device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty,
+                                      int* __restrict__ work_out, uint32_t* count_out,
+                                      uint32_t* __restrict__ mark, uint32_t epoch) {
+    for (int d = 0; d < 8; ++d) {
+        if (!(bits & (1u << d))) continue;
+        int nx = tx + TDX[d], ny = ty + TDY[d];
+        if (nx < 0 || ny < 0 || nx >= ntx || ny >= nty) continue;
+        int t = ny * ntx + nx;
+        if (atomicExch(&mark[t], epoch) != epoch) {
+            uint32_t slot = atomicAdd(count_out, 1u);
+            work_out[slot] = t;
+        }
+    }
+}
+
+// Block-wide OR of per-thread border bits; returns the result in all threads.
+__device__ inline uint32_t block_or_bits(uint32_t bits) {
+    __shared__ uint32_t s_bits;
+    if (threadIdx.x == 0) s_bits = 0;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) bits |= (uint32_t)__shfl_xor((int)bits, off);
+    if ((threadIdx.x & 63) == 0 && bits) atomicOr(&s_bits, bits);
+    __syncthreads();
+    return s_bits;
+}
+
+// Host side: run passes until no tile is listed.  `launch(work_in, count,
+// work_out, count_slot, epoch)` enqueues one pass.  `count` tiles are in
+// d_work[buf] on entry.
+template <class Launch>
+int run_tile_passes(Context* c, int buf, uint32_t count, int max_passes, Launch launch,
+                    int* passes_out) {
+    int passes = 0;
+    while (count > 0) {
+        if (passes >= max_passes) return CAGEO_ERR_NOT_CONVERGED;
+        const int out = 1 - buf;
+        const uint32_t epoch = next_epoch(c);
+        CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_COUNT0 + out, 0, sizeof(uint32_t), c->stream));
+        CAGEO_TRY(launch(c->d_work[buf], count, c->d_work[out], c->d_scal + SC_COUNT0 + out, epoch));
+        CAGEO_TRY(read_scalars(c));
+        count = c->h_scal[SC_COUNT0 + out];
+        buf = out;
+        ++passes;
+    }
+    if (passes_out) *passes_out += passes;
+    return CAGEO_OK;
+}
+
+}  // namespace cageo

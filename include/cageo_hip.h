@@ -1,0 +1,165 @@
+/*
+ * libcageo_hip: MI355X (gfx950) implementation of the per-realization Monte
+ * Carlo drainage-basin pipeline of jsoksane/cageo (`drainage`).
+ *
+ * Plain C ABI: device pointers and sizes, no C++ or framework types.  Each
+ * stage entry point replaces one reference stage's execute()/perform*() call
+ * (cited as reference-file:line, paths relative to the reference's src/).
+ *
+ * Conventions
+ *  - Rasters are row-major W x H, index = y*W + x, resident in device memory.
+ *  - Flow directions are `char2 {x, y}` (FlowDirClass.h:44): 2 int8 per cell.
+ *  - Every call is enqueued on the context's HIP stream and returns after the
+ *    work has completed (the reference's stages are blocking too); the return
+ *    value is a CAGEO_* status code, never an exit().
+ *  - A context owns the device workspace for one grid size, so no stage
+ *    allocates memory per call.  One context must not be used from two host
+ *    threads at once.
+ */
+#ifndef CAGEO_HIP_H_
+#define CAGEO_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CAGEO_OK 0
+#define CAGEO_ERR_ARG (-1)           /* bad argument or shape                */
+#define CAGEO_ERR_HIP (-2)           /* a HIP runtime call failed            */
+#define CAGEO_ERR_NOMEM (-3)         /* device allocation failed             */
+#define CAGEO_ERR_NOT_CONVERGED (-4) /* an iteration cap was reached         */
+#define CAGEO_ERR_STATE (-5)         /* call order (e.g. no filter set)      */
+#define CAGEO_ERR_RANGE (-6)         /* input outside the reference's domain */
+
+/* ---- constants shared with the reference ------------------------------- */
+#define CAGEO_FILTER_EXPONENTIAL 0 /* ConvolutionCommon.h:18-21 */
+#define CAGEO_FILTER_GAUSSIAN 1
+#define CAGEO_FLOWTRACE_OUTSIDE (-9999) /* FlowTracingCommon.h:18-21 */
+#define CAGEO_FLOWTRACE_LAKE (-9998)
+#define CAGEO_MAX_BEVEL_RADIUS 5 /* StreamBurningCommon.h:17 */
+
+typedef struct cageo_context cageo_context;
+
+/* Human-readable name of a status code. */
+const char* cageo_status_string(int status);
+
+/* Version string of the library build. */
+const char* cageo_version(void);
+
+/* ---- context ------------------------------------------------------------
+ * Allocates the device workspace for a width x height grid on the current
+ * HIP device.  `hip_stream` is a hipStream_t (NULL = the null stream).
+ * Replaces the per-stage cudaMalloc/cudaFree temporaries of the reference
+ * (e.g. ConvolutionAlgorithmGPUCUDA.cu:27-33, PitFillingAlgorithmGPUCUDA.cu)
+ * and the ManagedData device copies (ManagedDataCUDA.h:138-213). */
+int cageo_context_create(cageo_context** out, int width, int height, void* hip_stream);
+int cageo_context_destroy(cageo_context* ctx);
+
+/* ---- filter construction (host, once per run) ----------------------------
+ * filterRadius = ceil((PR / cell) * 2.5) in float, main.cpp:110. */
+int cageo_filter_radius(float practical_range, float cell_size);
+
+/* ConvolutionAlgorithm<T>::createFilter, ConvolutionAlgorithm.cpp:18-45
+ * (createCorrelogram :49-82, createFilterFromCorrelogram :84-148,
+ * createKernelVector :150-172).  Writes L = 2*radius+1 taps for the Gaussian
+ * model (separable vector) or L*L taps for the exponential model. */
+int cageo_create_filter(int model, float practical_range, float cell_size, int radius,
+                        float* taps_out);
+
+/* ConvolutionAlgorithm<T>::setFilterModel/setFilterRadius + the per-call
+ * H2D filter copy of ConvolutionAlgorithmGPUCUDA.cu:32-33, done once. */
+int cageo_set_filter(cageo_context* ctx, int model, int radius, const float* host_taps);
+
+/* ---- stages (device pointers) ------------------------------------------ */
+
+/* RandomFieldAlgorithm<T>::execute(field, seed), RandomFieldAlgorithm.h:82-93
+ * (GPU: RandomFieldAlgorithmGPUCUDA.cu:30-88).  field = mean + std * N(0,1),
+ * Philox4x32-10 keyed by `seed`, counter (cell/4, realization, stream_id). */
+int cageo_random_field(cageo_context* ctx, float* d_field, uint64_t seed,
+                       uint32_t realization, uint32_t stream_id, float mean, float std_dev);
+
+/* ConvolutionAlgorithm<T>::execute(field), ConvolutionAlgorithm.h:296-312
+ * (CPU order: ConvolutionAlgorithmCPUOpenMP.h:70-248).  In place, periodic
+ * wrap, taps accumulated in the reference's order with every product and sum
+ * rounded separately (bit-identical to the CPU implementation). */
+int cageo_convolve(cageo_context* ctx, float* d_field);
+
+/* RealisticDemAlgorithm<T,U>::execute(dem, field, field),
+ * RealisticDemAlgorithm.h:49-70: field = dem + field where dem != nodata,
+ * then v <= 0 && v != nodata -> 1.0.  The output aliases the field as in
+ * main.cpp:200-204. */
+int cageo_realistic_dem(cageo_context* ctx, const float* d_dem, float* d_field,
+                        float nodata_dem);
+
+/* StreamBurningAlgorithm<T,U>::execute(z, streams), StreamBurningAlgorithm.h:
+ * 116-134 with performStreamBurning, StreamBurningAlgorithmCPUOpenMP.h:32-82.
+ * In place on d_z; bevel_radius <= CAGEO_MAX_BEVEL_RADIUS. */
+int cageo_stream_burn(cageo_context* ctx, float* d_z, const float* d_stream,
+                      int bevel_radius, float nodata_dem, float nodata_stream);
+
+/* PitFillingAlgorithm<T>::execute(filled, dem), PitFillingAlgorithm.h:88-98
+ * (performPitFilling, PitFillingAlgorithmCPUOpenMP.h:68-242).  Quantizes
+ * d_dem in place to mm as the reference does and writes the filled surface
+ * to d_filled. */
+int cageo_pit_fill(cageo_context* ctx, float* d_filled, float* d_dem, float nodata_dem);
+
+/* FlowRoutingAlgorithm<T,U>::execute(flowDir, filled), FlowRoutingAlgorithm.h:
+ * 69-94 (D8 FlowRoutingAlgorithmCPUOpenMP.h:53-151, flats :153-249).
+ * d_flowdir: 2*W*H int8 (char2 {x, y}).  nodata_dem is accepted for the
+ * interface; the reference's CPU routing does not test it. */
+int cageo_flow_routing(cageo_context* ctx, int8_t* d_flowdir, const float* d_filled,
+                       float nodata_dem);
+
+/* FlowTracingAlgorithm<T,U,V>::execute(trace, flowDir, streams),
+ * FlowTracingAlgorithm.h:52-69 (FlowTracingAlgorithmCPUOpenMP.h:34-134).
+ * If `long_paths` is not NULL it receives the number of cells whose path to
+ * its terminal is longer than the reference's 100 001-step walk guard. */
+int cageo_flow_tracing(cageo_context* ctx, int32_t* d_labels, const int8_t* d_flowdir,
+                       const float* d_stream, int64_t* long_paths);
+
+/* CatchmentBorderAlgorithm<T,U>::execute(mc, trace), CatchmentBorderAlgorithm.h:
+ * 53-74 (CatchmentBorderAlgorithmCPUOpenMP.h:30-82): mc += border where
+ * mc != -1. */
+int cageo_catchment_border(cageo_context* ctx, float* d_mc, const int32_t* d_labels);
+
+/* ---- fused Monte Carlo driver (main.cpp:180-283) -------------------------
+ * cageo_mc_setup binds the resident inputs and the error model; the filter
+ * comes from cageo_set_filter.  cageo_run_realizations runs n realizations
+ * r = first .. first+n-1 (random field seed `seed`, counter realization r)
+ * and accumulates border counts into d_mc. */
+int cageo_mc_setup(cageo_context* ctx, const float* d_dem, const float* d_stream,
+                   float nodata_dem, float nodata_stream, int bevel_radius, float mean,
+                   float std_dev);
+int cageo_run_realizations(cageo_context* ctx, float* d_mc, int n, uint64_t seed,
+                           uint32_t first_realization);
+
+/* Run one realization from an injected white-noise field (stages a3..a9),
+ * leaving the intermediate rasters readable for parity checks.  d_noise is
+ * consumed.  Any of the optional outputs may be NULL. */
+int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc,
+                                 float* d_filled_out, int8_t* d_flowdir_out,
+                                 int32_t* d_labels_out);
+
+/* Per-stage device time (ms, hipEvent) of the last cageo_run_realizations
+ * call, summed over its realizations: random field, convolution, realistic +
+ * burn, pit fill, flow routing, flow tracing, border.  `out` holds 7 floats. */
+int cageo_stage_times(cageo_context* ctx, float* out, int enable);
+
+/* ---- synthetic inputs (host) -----------------------------------------------
+ * Seeded DEM: 100 m base + 0.5 % tilt along +x + 40 Gaussian hills
+ * (height U(5,60) m, sigma U(50,400) m) + N(0, 0.3^2) noise; streams where
+ * D8 flow accumulation on the filled 5x5-smoothed DEM exceeds acc_threshold
+ * cells, float ids >= 1 per stream link, nodata_stream elsewhere. */
+int cageo_synth_dem(int width, int height, uint64_t seed, float cell_size,
+                    int acc_threshold, float nodata_stream, float* dem_out,
+                    float* stream_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAGEO_HIP_H_ */
