@@ -51,6 +51,7 @@ struct Context {
     int ntx = 0, nty = 0, ntiles = 0;
     int* d_work[2] = {nullptr, nullptr};
     uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
+    uint32_t* d_topen = nullptr; // per-tile flag: tile holds OPEN pit-fill cells
     uint32_t epoch = 0;
 
     // scalars
@@ -86,6 +87,7 @@ struct Context {
     int32_t* d_dist = nullptr;
     int32_t* d_lab = nullptr;
     uint64_t* d_ptr = nullptr;
+    uint64_t* d_ptr2 = nullptr;
 
     // timing
     bool timing = false;

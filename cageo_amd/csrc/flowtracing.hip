@@ -6,87 +6,140 @@
 // int) or onto an already labelled cell (copies it); stream cells themselves
 // are LAKE; a walk longer than 100 001 steps (a cell with no direction, or a
 // cycle) leaves its path at 0.  A cell's label is therefore the label of its
-// path's terminal, which pointer jumping computes in O(log path) passes:
-//   entry(i) = terminal label    if i's next cell decides it, else
-//              pointer to next cell;  entry(i) <- entry(entry(i)).
-// Entries are packed in 64 bits (flags | steps | value) so one atomic load
-// reads a consistent pointer/step pair while other lanes update in place.
-// A pointer that stops advancing points into a cell without a direction
-// (stuck, label 0); pointers still moving after ceil(log2 n)+1 passes are on
-// a cycle (label 0).  Cells whose terminal is more than 100 001 steps away
-// are counted: there the reference's result depends on its raster order.
-// performConnectFlowPath (:115-134) then maps label >= 0 to label[label]
-// when that is negative.
+// path's terminal, computed here by pointer jumping in two levels:
+//   1. per 64x64 tile, in LDS: every cell's pointer jumps along the in-tile
+//      part of its path (12 doubling rounds cover 4096 cells) and ends as a
+//      terminal label, "stuck" (its path cycles inside the tile, e.g. a cell
+//      without a direction), or an exit: the first cell outside the tile;
+//   2. over the grid: exits jump to the entries of their exit cells
+//      (ping-pong buffers) until no exit is left; a chain still moving after
+//      ceil(log2 n)+1 rounds is a cycle through several tiles (stuck).
+// Entries are 64 bits: kind (2 bits) | steps (30 bits) | value (32 bits), so
+// the step count to the terminal is known and cells whose terminal is more
+// than 100 001 steps away (where the reference's result depends on its
+// raster order) are counted.  performConnectFlowPath (:115-134) then maps a
+// label >= 0 to label[label] when that is negative.
 #include "common.hpp"
 
 namespace cageo {
 
 namespace {
 
-constexpr uint64_t E_RESOLVED = 1ull << 63;
-constexpr uint64_t E_STUCK = 1ull << 62;
-constexpr int STEP_SHIFT = 32;
+constexpr uint32_t K_LOCAL = 0, K_RESOLVED = 1, K_EXIT = 2, K_STUCK = 3;
 constexpr uint64_t STEP_MASK = (1ull << 30) - 1;
 
-__device__ inline uint64_t pack(bool resolved, uint32_t steps, int32_t value) {
-    return (resolved ? E_RESOLVED : 0ull) | ((uint64_t)(steps & STEP_MASK) << STEP_SHIFT) |
+__device__ inline uint64_t pack(uint32_t kind, uint32_t steps, int32_t value) {
+    return ((uint64_t)kind << 62) | ((uint64_t)(steps & STEP_MASK) << 32) |
            (uint64_t)(uint32_t)value;
 }
-__device__ inline uint32_t steps_of(uint64_t e) { return (uint32_t)((e >> STEP_SHIFT) & STEP_MASK); }
+__device__ inline uint32_t kind_of(uint64_t e) { return (uint32_t)(e >> 62); }
+__device__ inline uint32_t steps_of(uint64_t e) { return (uint32_t)((e >> 32) & STEP_MASK); }
 __device__ inline int32_t value_of(uint64_t e) { return (int32_t)(uint32_t)e; }
 
-__global__ __launch_bounds__(256) void k_trace_init(const int8_t* __restrict__ fd,
-                                                    const uint8_t* __restrict__ sf,
-                                                    const int32_t* __restrict__ slabel,
-                                                    uint64_t* __restrict__ ent, int W, int H) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const size_t i = (size_t)y * W + x;
-    uint64_t e;
-    if (sf[i] & SF_TRACE) {
-        e = pack(true, 0, CAGEO_FLOWTRACE_LAKE);
-    } else {
-        const char2 d = reinterpret_cast<const char2*>(fd)[i];
-        const int nx = x + d.x, ny = y + d.y;
-        if (nx < 0 || ny < 0 || nx >= W || ny >= H) {
-            e = pack(true, 1, CAGEO_FLOWTRACE_OUTSIDE);
-        } else {
-            const size_t ni = (size_t)ny * W + nx;
-            if (sf[ni] & SF_TRACE)
-                e = pack(true, 1, slabel[ni]);
-            else
-                e = pack(false, 1, (int32_t)ni);
-        }
-    }
-    ent[i] = e;
+// Follow entry e one hop through entry f (the entry of e's target).
+__device__ inline uint64_t hop(uint64_t e, uint64_t f) {
+    uint32_t s = steps_of(e) + steps_of(f);
+    if (s > STEP_MASK) s = STEP_MASK;
+    const uint32_t k = kind_of(f);
+    if (k == K_STUCK) return pack(K_STUCK, 0, 0);
+    return pack(k, s, value_of(f));
 }
 
-__global__ __launch_bounds__(256) void k_trace_jump(uint64_t* __restrict__ ent, int64_t n,
-                                                    uint32_t* __restrict__ scal) {
-    uint32_t changed = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t e = __hip_atomic_load(&ent[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (e & E_RESOLVED) continue;
-        const int32_t p = value_of(e);
-        const uint64_t f = __hip_atomic_load(&ent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t ne;
-        uint32_t s = steps_of(e) + steps_of(f);
-        if (s > STEP_MASK) s = STEP_MASK;
-        if (f & E_RESOLVED) {
-            ne = (f & E_STUCK) ? (E_RESOLVED | E_STUCK) : pack(true, s, value_of(f));
-        } else if (value_of(f) == p) {
-            ne = E_RESOLVED | E_STUCK;  // next cell points at itself
-        } else {
-            ne = pack(false, s, value_of(f));
+__global__ __launch_bounds__(256) void k_trace_tile(const int8_t* __restrict__ fd,
+                                                    const uint8_t* __restrict__ sf,
+                                                    const int32_t* __restrict__ slabel,
+                                                    uint64_t* __restrict__ ent, int W, int H,
+                                                    int ntx) {
+    __shared__ uint64_t L[TILE * TILE];
+    const int t = blockIdx.x;
+    const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int gx = x0 + lx;
+
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) {
+        const int ly = strip * STRIP + k;
+        const int gy = y0 + ly;
+        uint64_t e = pack(K_STUCK, 0, 0);
+        if (gx < W && gy < H) {
+            const size_t i = (size_t)gy * W + gx;
+            if (sf[i] & SF_TRACE) {
+                e = pack(K_RESOLVED, 0, CAGEO_FLOWTRACE_LAKE);
+            } else {
+                const char2 d = reinterpret_cast<const char2*>(fd)[i];
+                const int nx = gx + d.x, ny = gy + d.y;
+                if (nx < 0 || ny < 0 || nx >= W || ny >= H) {
+                    e = pack(K_RESOLVED, 1, CAGEO_FLOWTRACE_OUTSIDE);
+                } else {
+                    const size_t ni = (size_t)ny * W + nx;
+                    if (sf[ni] & SF_TRACE) {
+                        e = pack(K_RESOLVED, 1, slabel[ni]);
+                    } else if (d.x == 0 && d.y == 0) {
+                        e = pack(K_STUCK, 0, 0);
+                    } else if (nx >= x0 && ny >= y0 && nx < x0 + TILE && ny < y0 + TILE) {
+                        e = pack(K_LOCAL, 1, (ny - y0) * TILE + (nx - x0));
+                        local |= 1u << k;
+                    } else {
+                        e = pack(K_EXIT, 1, (int32_t)ni);
+                    }
+                }
+            }
         }
-        if (ne != e) {
-            __hip_atomic_store(&ent[i], ne, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            changed = 1;
+        L[ly * TILE + lx] = e;
+    }
+    __syncthreads();
+
+    // Jacobi doubling rounds: read all targets, barrier, write.
+    for (int round = 0; round < 13; ++round) {
+        if (!__syncthreads_or(local != 0)) break;
+        uint64_t nv[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; ++k) {
+            const int li = (strip * STRIP + k) * TILE + lx;
+            nv[k] = 0;
+            if (local & (1u << k)) {
+                const uint64_t e = L[li];
+                nv[k] = hop(e, L[value_of(e)]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < STRIP; ++k) {
+            if (local & (1u << k)) {
+                const int li = (strip * STRIP + k) * TILE + lx;
+                L[li] = nv[k];
+                if (kind_of(nv[k]) != K_LOCAL) local &= ~(1u << k);
+            }
         }
     }
-    if (changed) atomicOr(&scal[SC_CHANGED], 1u);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) {
+        const int ly = strip * STRIP + k;
+        const int gy = y0 + ly;
+        if (gx < W && gy < H) {
+            uint64_t e = L[ly * TILE + lx];
+            if (kind_of(e) == K_LOCAL) e = pack(K_STUCK, 0, 0);  // cycle inside the tile
+            ent[(size_t)gy * W + gx] = e;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_trace_exit_jump(const uint64_t* __restrict__ src,
+                                                         uint64_t* __restrict__ dst, int64_t n,
+                                                         uint32_t* __restrict__ scal) {
+    uint32_t pending = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t e = src[i];
+        if (kind_of(e) == K_EXIT) {
+            e = hop(e, src[value_of(e)]);
+            pending |= kind_of(e) == K_EXIT;
+        }
+        dst[i] = e;
+    }
+    if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&scal[SC_CHANGED], 1u);
 }
 
 __global__ __launch_bounds__(256) void k_trace_final(const uint64_t* __restrict__ ent,
@@ -97,7 +150,7 @@ __global__ __launch_bounds__(256) void k_trace_final(const uint64_t* __restrict_
          i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t e = ent[i];
         int32_t v = 0;
-        if ((e & E_RESOLVED) && !(e & E_STUCK)) {
+        if (kind_of(e) == K_RESOLVED) {
             v = value_of(e);
             if (steps_of(e) > 100001u) ++longp;
         }
@@ -119,31 +172,34 @@ __global__ __launch_bounds__(256) void k_trace_connect(int32_t* __restrict__ lab
 
 inline unsigned grid_1d(size_t n) {
     size_t g = (n + 255) / 256;
-    if (g > 65536) g = 65536;
+    if (g > 32768) g = 32768;
     return (unsigned)(g ? g : 1);
 }
 
 }  // namespace
 
+// Needs two 64-bit rasters: c->d_ptr and the scratch `alt` (same size).
 int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,
                  int32_t* lab, int64_t* long_paths) {
-    const int W = c->W, H = c->H;
-    hipLaunchKernelGGL(k_trace_init, dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, c->stream,
-                       fd, sflags, slabel, c->d_ptr, W, H);
+    hipLaunchKernelGGL(k_trace_tile, dim3(c->ntiles), dim3(TILE_THREADS), 0, c->stream, fd,
+                       sflags, slabel, c->d_ptr, c->W, c->H, c->ntx);
     CAGEO_HIP_TRY(hipGetLastError());
+    uint64_t* cur = c->d_ptr;
+    uint64_t* alt = c->d_ptr2;
     int max_rounds = 2;
     for (size_t m = 1; m < c->n; m <<= 1) ++max_rounds;
     for (int r = 0; r < max_rounds; ++r) {
         CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_CHANGED, 0, sizeof(uint32_t), c->stream));
-        hipLaunchKernelGGL(k_trace_jump, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->d_ptr,
-                           (int64_t)c->n, c->d_scal);
+        hipLaunchKernelGGL(k_trace_exit_jump, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, cur,
+                           alt, (int64_t)c->n, c->d_scal);
         CAGEO_HIP_TRY(hipGetLastError());
+        std::swap(cur, alt);
         CAGEO_TRY(read_scalars(c));
         if (!c->h_scal[SC_CHANGED]) break;
     }
     CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_LONGPATH, 0, sizeof(uint32_t), c->stream));
-    hipLaunchKernelGGL(k_trace_final, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, c->d_ptr,
-                       lab, (int64_t)c->n, c->d_scal);
+    hipLaunchKernelGGL(k_trace_final, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, cur, lab,
+                       (int64_t)c->n, c->d_scal);
     CAGEO_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_trace_connect, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, lab,
                        (int64_t)c->n);

@@ -7,25 +7,26 @@
 // unprocessed 8-neighbours n to max(z_n, spill_c); every cell is pushed once,
 // into bucket toIndex(v) of its value v, and when popped it expands only if
 // toElev(toIndex(v)) == v (:178-181).  Values that fail that round trip
-// ("dead" values, e.g. 0.251 m) never expand, so
-//   W(n) = max(Z(n), min { W(c) : c an 8-neighbour, reached, live(W(c)) }),
+// ("dead" values, e.g. 0.251 m) never expand, so with P the result
+//   P(n) = max(Z(n), min { P(c) : c an 8-neighbour, reached, live(P(c)) }),
 // seeds fixed at Z, unreached cells keep Z.
 //
-// How it is computed here, without a global priority queue:
-//   round k: (1) fill: the greatest fixed point of
-//                W(n) = max(Z(n), min relay(c)),  relay(c) = W(c) for OPEN
-//                cells and live FIXED cells, +inf otherwise,
-//            by monotone relaxation from +inf over the OPEN cells.  Every
-//            relay of the reference is allowed, so W <= the reference's
-//            value on every reached OPEN cell;
-//            (2) certify: an OPEN cell n with a live FIXED neighbour c and
-//                W(c) <= W(n) has the reference's value (the relay through c
-//                bounds it from above, the fill from below); certification
-//                propagates until closed;
-//            (3) reset the still-OPEN cells to +inf and repeat.
-// Each round certifies at least the first cell the reference reaches among
-// the OPEN ones; a round that certifies nothing leaves only cells the
-// reference never reaches, which keep their quantized elevation.
+// How it is computed here, without a global priority queue.  Cells are
+// FIXED (value known to equal P; live or dead) or OPEN.  Seeds start FIXED.
+// Each round:
+//   fill:    monotone relaxation from +inf of W(n) = max(Z(n), min relay(c))
+//            over the OPEN cells, relay(c) = W(c) for OPEN and live FIXED
+//            cells.  Every relay the reference makes is available, so at
+//            convergence W <= P on the OPEN cells.  During the relaxation an
+//            OPEN cell with W(n) == Z(n) and a live FIXED neighbour c with
+//            P(c) <= Z(n) is fixed at once: P(n) <= max(Z(n), P(c)) = Z(n)
+//            and P(n) >= Z(n).  A cell fixed dead stops relaying.
+//   certify: after convergence an OPEN cell with a live FIXED neighbour c,
+//            P(c) <= W(n), is fixed: P(n) <= max(Z(n), P(c)) <= W(n) <= P(n).
+//   next:    OPEN cells restart at +inf.
+// A round fixes at least the first OPEN cell the reference reaches (every
+// relay path into it through OPEN cells has bottleneck >= its value), so a
+// round that fixes nothing leaves only unreached cells; they keep Z.
 //
 // Every comparison is on floats; the only arithmetic is the quantisation and
 // the round-trip test, both evaluated exactly as the reference does.
@@ -47,6 +48,32 @@ __device__ inline float min8(const float* s, int i) {
 
 __device__ inline float min_elev_of(const uint32_t* scal) {
     return ordered_to_float(scal[SC_MIN]);
+}
+
+// Loads relay (OPEN or live FIXED: W, else +inf) and crelay (live FIXED: W,
+// else +inf) for the tile and its 1-cell halo.
+__device__ inline void load_relays(float* relay, float* crelay, const float* __restrict__ w,
+                                   const uint8_t* __restrict__ st, int x0, int y0, int W,
+                                   int H) {
+    for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
+        int hy = k / HALO_W, hx = k - hy * HALO_W;
+        int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+        float r = PF_INF, cr = PF_INF;
+        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+            size_t gi = (size_t)gy * W + gx;
+            uint8_t s = st[gi];
+            if (s == PF_OPEN || s == PF_FIXED_LIVE) r = w[gi];
+            if (s == PF_FIXED_LIVE) cr = r;
+        }
+        relay[k] = r;
+        crelay[k] = cr;
+    }
+}
+
+// Block-wide sum of per-thread counts, added to *dst by one thread.
+__device__ inline void block_add(uint32_t v, uint32_t* dst) {
+    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
 __global__ __launch_bounds__(256) void k_pf_init(const float* __restrict__ zq,
@@ -77,30 +104,20 @@ __global__ __launch_bounds__(256) void k_pf_init(const float* __restrict__ zq,
     w[i] = v;
 }
 
-// (1) fill relaxation over the OPEN cells of the listed tiles.
-__global__ __launch_bounds__(256) void k_pf_fill(const int* __restrict__ work,
-                                                 const float* __restrict__ zq,
-                                                 float* __restrict__ w,
-                                                 const uint8_t* __restrict__ st, int W, int H,
-                                                 int ntx, int nty, int* __restrict__ work_out,
-                                                 uint32_t* count_out,
-                                                 uint32_t* __restrict__ mark, uint32_t epoch) {
+// Fill relaxation with immediate fixing of cells that sit at their own
+// elevation next to a live fixed neighbour no higher than them.
+__global__ __launch_bounds__(256) void k_pf_fill(
+    const int* __restrict__ work, const float* __restrict__ zq, float* __restrict__ w,
+    uint8_t* __restrict__ st, int W, int H, int ntx, int nty, int* __restrict__ work_out,
+    uint32_t* count_out, uint32_t* __restrict__ mark, uint32_t epoch,
+    uint32_t* __restrict__ topen, uint32_t* __restrict__ scal) {
     __shared__ float relay[HALO_W * HALO_W];
+    __shared__ float crelay[HALO_W * HALO_W];
     const int t = work[blockIdx.x];
     const int tx = t % ntx, ty = t / ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
-
-    for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
-        int hy = k / HALO_W, hx = k - hy * HALO_W;
-        int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-        float v = PF_INF;
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            size_t gi = (size_t)gy * W + gx;
-            uint8_t s = st[gi];
-            if (s == PF_OPEN || s == PF_FIXED_LIVE) v = w[gi];
-        }
-        relay[k] = v;
-    }
+    const float mn = min_elev_of(scal);
+    load_relays(relay, crelay, w, st, x0, y0, W, H);
 
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
@@ -122,7 +139,7 @@ __global__ __launch_bounds__(256) void k_pf_fill(const int* __restrict__ work,
     }
     __syncthreads();
 
-    uint32_t changed_cells = 0;
+    uint32_t lowered = 0, fixed = 0, fixed_live = 0;
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
         if (open) {
@@ -131,73 +148,84 @@ __global__ __launch_bounds__(256) void k_pf_fill(const int* __restrict__ work,
                 const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
                 if (!(open & (1u << k))) continue;
                 const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
-                float m = min8(relay, li);
-                float nw = fmaxf(zr[k], fminf(wr[k], m));
+                const float m = min8(relay, li);
+                const float nw = fmaxf(zr[k], fminf(wr[k], m));
                 if (nw < wr[k]) {
                     wr[k] = nw;
                     relay[li] = nw;
                     changed = 1;
-                    changed_cells |= 1u << k;
+                    lowered |= 1u << k;
+                }
+                if (wr[k] == zr[k] && min8(crelay, li) <= wr[k]) {
+                    open &= ~(1u << k);
+                    fixed |= 1u << k;
+                    changed = 1;
+                    if (pf_live(wr[k], mn)) {
+                        fixed_live |= 1u << k;
+                        crelay[li] = wr[k];
+                    } else {
+                        relay[li] = PF_INF;
+                    }
                 }
             }
         }
         if (!__syncthreads_or(changed)) break;
     }
 
-    uint32_t bits = 0;
+    uint32_t bits = 0, nfixed = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        if (changed_cells & (1u << k)) {
-            int ly = strip * STRIP + k;
-            w[(size_t)(y0 + ly) * W + gx] = wr[k];
-            bits |= border_bits(lx, ly);
+        const int ly = strip * STRIP + k;
+        const size_t gi = (size_t)(y0 + ly) * W + gx;
+        if (lowered & (1u << k)) w[gi] = wr[k];
+        if (fixed & (1u << k)) {
+            st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+            ++nfixed;
         }
+        if ((lowered | fixed) & (1u << k)) bits |= border_bits(lx, ly);
     }
+    block_add(nfixed, &scal[SC_CERTIFIED]);
+    const int any_open = __syncthreads_or(open != 0);
     bits = block_or_bits(bits);
-    if (threadIdx.x == 0 && bits)
-        activate_tiles(bits, tx, ty, ntx, nty, work_out, count_out, mark, epoch);
+    if (threadIdx.x == 0) {
+        topen[t] = any_open ? 1u : 0u;
+        if (bits) activate_tiles(bits, tx, ty, ntx, nty, work_out, count_out, mark, epoch);
+    }
 }
 
-// (2) certification of OPEN cells through live FIXED neighbours.
-__global__ __launch_bounds__(256) void k_pf_cert(const int* __restrict__ work,
-                                                 const float* __restrict__ w,
-                                                 uint8_t* __restrict__ st, int W, int H,
-                                                 int ntx, int nty, int* __restrict__ work_out,
-                                                 uint32_t* count_out,
-                                                 uint32_t* __restrict__ mark, uint32_t epoch,
-                                                 uint32_t* __restrict__ scal) {
-    __shared__ float wv[HALO_W * HALO_W];
-    __shared__ uint8_t sv[HALO_W * HALO_W];
+// Certification of the remaining OPEN cells after the fill has converged.
+__global__ __launch_bounds__(256) void k_pf_cert(
+    const int* __restrict__ work, const float* __restrict__ w, uint8_t* __restrict__ st, int W,
+    int H, int ntx, int nty, int* __restrict__ work_out, uint32_t* count_out,
+    uint32_t* __restrict__ mark, uint32_t epoch, uint32_t* __restrict__ topen,
+    uint32_t* __restrict__ scal) {
+    __shared__ float relay[HALO_W * HALO_W];
+    __shared__ float crelay[HALO_W * HALO_W];
     const int t = work[blockIdx.x];
     const int tx = t % ntx, ty = t / ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
     const float mn = min_elev_of(scal);
-
-    for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
-        int hy = k / HALO_W, hx = k - hy * HALO_W;
-        int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-        float v = PF_INF;
-        uint8_t s = PF_NODATA;
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            size_t gi = (size_t)gy * W + gx;
-            s = st[gi];
-            v = w[gi];
-        }
-        wv[k] = v;
-        sv[k] = s;
-    }
-    __syncthreads();
+    load_relays(relay, crelay, w, st, x0, y0, W, H);
 
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
     uint32_t open = 0;
+    float wr[STRIP];
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
-        if (sv[li] == PF_OPEN && wv[li] < PF_INF) open |= 1u << k;
+        const int gy = y0 + strip * STRIP + k;
+        wr[k] = PF_INF;
+        if (gx < W && gy < H) {
+            size_t gi = (size_t)gy * W + gx;
+            if (st[gi] == PF_OPEN) {
+                wr[k] = w[gi];
+                if (wr[k] < PF_INF) open |= 1u << k;
+            }
+        }
     }
+    __syncthreads();
 
-    uint32_t newly = 0;
+    uint32_t fixed = 0, fixed_live = 0;
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
         if (open) {
@@ -206,105 +234,96 @@ __global__ __launch_bounds__(256) void k_pf_cert(const int* __restrict__ work,
                 const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
                 if (!(open & (1u << k))) continue;
                 const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
-                const float v = wv[li];
-                bool ok = false;
-#pragma unroll
-                for (int d = 0; d < 8; ++d) {
-                    const int ni = li + TDY[d] * HALO_W + TDX[d];
-                    ok |= (sv[ni] == PF_FIXED_LIVE) && (wv[ni] <= v);
-                }
-                if (ok) {
-                    sv[li] = pf_live(v, mn) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+                if (min8(crelay, li) <= wr[k]) {
                     open &= ~(1u << k);
-                    newly |= 1u << k;
+                    fixed |= 1u << k;
                     changed = 1;
+                    if (pf_live(wr[k], mn)) {
+                        fixed_live |= 1u << k;
+                        crelay[li] = wr[k];
+                    }
                 }
             }
         }
         if (!__syncthreads_or(changed)) break;
     }
 
-    uint32_t bits = 0;
-    uint32_t cnt = 0;
+    uint32_t bits = 0, nfixed = 0, still_open = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        if (newly & (1u << k)) {
-            const int ly = strip * STRIP + k;
-            const int li = (ly + 1) * HALO_W + lx + 1;
-            const uint8_t s = sv[li];
-            st[(size_t)(y0 + ly) * W + gx] = s;
-            ++cnt;
-            if (s == PF_FIXED_LIVE) bits |= border_bits(lx, ly);
+        const int ly = strip * STRIP + k;
+        const int gy = y0 + ly;
+        if (gx >= W || gy >= H) continue;
+        const size_t gi = (size_t)gy * W + gx;
+        if (fixed & (1u << k)) {
+            st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+            ++nfixed;
+            if (fixed_live & (1u << k)) bits |= border_bits(lx, ly);
+        } else if (st[gi] == PF_OPEN) {
+            still_open = 1;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, off);
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&scal[SC_CERTIFIED], cnt);
+    block_add(nfixed, &scal[SC_CERTIFIED]);
+    const int any_open = __syncthreads_or(still_open);
     bits = block_or_bits(bits);
-    if (threadIdx.x == 0 && bits)
-        activate_tiles(bits, tx, ty, ntx, nty, work_out, count_out, mark, epoch);
+    if (threadIdx.x == 0) {
+        topen[t] = any_open ? 1u : 0u;
+        if (bits) activate_tiles(bits, tx, ty, ntx, nty, work_out, count_out, mark, epoch);
+    }
 }
 
-// Lists the tiles holding OPEN cells (all tiles, one workgroup each), counts
-// the OPEN cells and optionally resets them to +inf for the next round.
-__global__ __launch_bounds__(256) void k_pf_open_tiles(float* __restrict__ w,
+// Lists the tiles flagged as holding OPEN cells.
+__global__ __launch_bounds__(256) void k_list_flagged(const uint32_t* __restrict__ topen,
+                                                      int ntiles, int* __restrict__ work_out,
+                                                      uint32_t* count_out) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
+        if (topen[t]) work_out[atomicAdd(count_out, 1u)] = t;
+}
+
+// OPEN cells of the listed tiles restart from +inf.
+__global__ __launch_bounds__(256) void k_pf_reset_open(const int* __restrict__ work,
+                                                       float* __restrict__ w,
                                                        const uint8_t* __restrict__ st, int W,
-                                                       int H, int ntx, int reset,
-                                                       int* __restrict__ work_out,
-                                                       uint32_t* count_out,
-                                                       uint32_t* __restrict__ scal) {
-    const int t = blockIdx.x;
-    const int tx = t % ntx, ty = t / ntx;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
-    const int gx = tx * TILE + lx;
-    uint32_t cnt = 0;
-    if (gx < W) {
-#pragma unroll
-        for (int k = 0; k < STRIP; ++k) {
-            const int gy = ty * TILE + strip * STRIP + k;
-            if (gy >= H) break;
-            const size_t gi = (size_t)gy * W + gx;
-            if (st[gi] == PF_OPEN) {
-                ++cnt;
-                if (reset) w[gi] = PF_INF;
-            }
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, off);
-    __shared__ uint32_t s_cnt;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s_cnt, cnt);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_cnt) {
-        atomicAdd(&scal[SC_OPEN], s_cnt);
-        uint32_t slot = atomicAdd(count_out, 1u);
-        work_out[slot] = t;
+                                                       int H, int ntx) {
+    const int t = work[blockIdx.x];
+    const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
+    const int gx = x0 + (threadIdx.x & 63);
+    if (gx >= W) return;
+    for (int k = 0; k < STRIP; ++k) {
+        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
+        if (gy >= H) break;
+        const size_t gi = (size_t)gy * W + gx;
+        if (st[gi] == PF_OPEN) w[gi] = PF_INF;
     }
 }
 
-__global__ __launch_bounds__(256) void k_pf_finalize(float* __restrict__ w,
+// Cells never reached keep their quantized elevation.
+__global__ __launch_bounds__(256) void k_pf_finalize(const int* __restrict__ work,
+                                                     float* __restrict__ w,
                                                      const float* __restrict__ zq,
-                                                     const uint8_t* __restrict__ st, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        if (st[i] == PF_OPEN) w[i] = zq[i];
+                                                     const uint8_t* __restrict__ st, int W,
+                                                     int H, int ntx) {
+    const int t = work[blockIdx.x];
+    const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
+    const int gx = x0 + (threadIdx.x & 63);
+    if (gx >= W) return;
+    for (int k = 0; k < STRIP; ++k) {
+        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
+        if (gy >= H) break;
+        const size_t gi = (size_t)gy * W + gx;
+        if (st[gi] == PF_OPEN) w[gi] = zq[gi];
+    }
 }
 
 }  // namespace
 
-// Lists tiles with OPEN cells into d_work[buf]; returns their count and the
-// OPEN cell count through the scalar block.
-static int list_open_tiles(Context* c, float* w, int reset, int buf, uint32_t* tiles,
-                           uint32_t* open_cells) {
+static int list_open_tiles(Context* c, int buf, uint32_t* tiles) {
     CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_COUNT0 + buf, 0, sizeof(uint32_t), c->stream));
-    CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_OPEN, 0, sizeof(uint32_t), c->stream));
-    hipLaunchKernelGGL(k_pf_open_tiles, dim3(c->ntiles), dim3(TILE_THREADS), 0, c->stream, w,
-                       c->d_st, c->W, c->H, c->ntx, reset, c->d_work[buf],
-                       c->d_scal + SC_COUNT0 + buf, c->d_scal);
+    hipLaunchKernelGGL(k_list_flagged, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
+                       c->d_topen, c->ntiles, c->d_work[buf], c->d_scal + SC_COUNT0 + buf);
     CAGEO_HIP_TRY(hipGetLastError());
     CAGEO_TRY(read_scalars(c));
     *tiles = c->h_scal[SC_COUNT0 + buf];
-    *open_cells = c->h_scal[SC_OPEN];
     return CAGEO_OK;
 }
 
@@ -325,49 +344,44 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
                        zq, w, c->d_st, W, H, nodata, c->d_scal);
     CAGEO_HIP_TRY(hipGetLastError());
 
-    const int max_passes = 1 << 20;
-    uint32_t tiles = 0, open_cells = 0;
-    CAGEO_TRY(worklist_all_tiles(c, 0));
-    tiles = (uint32_t)c->ntiles;
-    for (int round = 0;; ++round) {
-        if (round > 0) {
-            CAGEO_TRY(list_open_tiles(c, w, 1, 0, &tiles, &open_cells));
-        }
-        int passes = 0;
-        CAGEO_TRY(run_tile_passes(
-            c, 0, tiles, max_passes,
-            [&](const int* win, uint32_t cnt, int* wout, uint32_t* slot, uint32_t epoch) {
-                hipLaunchKernelGGL(k_pf_fill, dim3(cnt), dim3(TILE_THREADS), 0, c->stream, win,
-                                   zq, w, c->d_st, W, H, c->ntx, c->nty, wout, slot, c->d_mark,
-                                   epoch);
-                CAGEO_HIP_TRY(hipGetLastError());
-                return CAGEO_OK;
-            },
-            &passes));
+    const int max_passes = 1 << 18;
+    auto fill = [&](const int* win, uint32_t cnt, int* wout, uint32_t* slot, uint32_t epoch) {
+        hipLaunchKernelGGL(k_pf_fill, dim3(cnt), dim3(TILE_THREADS), 0, c->stream, win, zq, w,
+                           c->d_st, W, H, c->ntx, c->nty, wout, slot, c->d_mark, epoch,
+                           c->d_topen, c->d_scal);
+        CAGEO_HIP_TRY(hipGetLastError());
+        return CAGEO_OK;
+    };
+    auto cert = [&](const int* win, uint32_t cnt, int* wout, uint32_t* slot, uint32_t epoch) {
+        hipLaunchKernelGGL(k_pf_cert, dim3(cnt), dim3(TILE_THREADS), 0, c->stream, win, w,
+                           c->d_st, W, H, c->ntx, c->nty, wout, slot, c->d_mark, epoch,
+                           c->d_topen, c->d_scal);
+        CAGEO_HIP_TRY(hipGetLastError());
+        return CAGEO_OK;
+    };
 
-        CAGEO_TRY(list_open_tiles(c, w, 0, 0, &tiles, &open_cells));
+    uint32_t tiles = (uint32_t)c->ntiles;
+    CAGEO_TRY(worklist_all_tiles(c, 0));
+    for (int round = 0;; ++round) {
+        if (round > (int)c->n + 1) return CAGEO_ERR_NOT_CONVERGED;
         CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_CERTIFIED, 0, sizeof(uint32_t), c->stream));
-        CAGEO_TRY(run_tile_passes(
-            c, 0, tiles, max_passes,
-            [&](const int* win, uint32_t cnt, int* wout, uint32_t* slot, uint32_t epoch) {
-                hipLaunchKernelGGL(k_pf_cert, dim3(cnt), dim3(TILE_THREADS), 0, c->stream, win,
-                                   w, c->d_st, W, H, c->ntx, c->nty, wout, slot, c->d_mark,
-                                   epoch, c->d_scal);
-                CAGEO_HIP_TRY(hipGetLastError());
-                return CAGEO_OK;
-            },
-            &passes));
-        CAGEO_TRY(read_scalars(c));
-        const uint32_t certified = c->h_scal[SC_CERTIFIED];
-        CAGEO_TRY(list_open_tiles(c, w, 0, 0, &tiles, &open_cells));
-        if (open_cells == 0) break;
-        if (certified == 0) {
-            hipLaunchKernelGGL(k_pf_finalize, dim3((unsigned)std::min<size_t>((c->n + 255) / 256, 65536)),
-                               dim3(256), 0, c->stream, w, zq, c->d_st, (int64_t)c->n);
+        if (round > 0) {
+            hipLaunchKernelGGL(k_pf_reset_open, dim3(tiles), dim3(TILE_THREADS), 0, c->stream,
+                               c->d_work[0], w, c->d_st, W, H, c->ntx);
+            CAGEO_HIP_TRY(hipGetLastError());
+        }
+        CAGEO_TRY(run_tile_passes(c, 0, tiles, max_passes, fill, nullptr));
+        CAGEO_TRY(list_open_tiles(c, 0, &tiles));
+        if (tiles == 0) break;
+        CAGEO_TRY(run_tile_passes(c, 0, tiles, max_passes, cert, nullptr));
+        CAGEO_TRY(list_open_tiles(c, 0, &tiles));
+        if (tiles == 0) break;
+        if (c->h_scal[SC_CERTIFIED] == 0) {
+            hipLaunchKernelGGL(k_pf_finalize, dim3(tiles), dim3(TILE_THREADS), 0, c->stream,
+                               c->d_work[0], w, zq, c->d_st, W, H, c->ntx);
             CAGEO_HIP_TRY(hipGetLastError());
             break;
         }
-        if (round > max_passes) return CAGEO_ERR_NOT_CONVERGED;
     }
     return CAGEO_OK;
 }

@@ -59,35 +59,79 @@ __global__ __launch_bounds__(256) void k_realistic(const float* __restrict__ dem
         field[i] = realistic_value(dem[i], field[i], nodata);
 }
 
-// Burned value of cell (x, y) given a source of unburned values `zr(n)`.
+// Burning works on LDS tiles of 64 x 16 cells with an R-cell halo holding
+// the unburned values and the stream flags.  For an in-grid neighbour the
+// reference's "already processed" test (raster index n < i) reduces to the
+// offset alone: dy < 0, or dy == 0 and dx < 0.
+constexpr int BT_X = 64, BT_Y = 16;
+constexpr int BH_MAX = CAGEO_MAX_BEVEL_RADIUS;
+constexpr int BL_W = BT_X + 2 * BH_MAX, BL_H = BT_Y + 2 * BH_MAX;
+
 template <class ZR>
-__device__ inline float burn_value(int x, int y, int W, int H, const uint8_t* __restrict__ sf,
-                                   const int* __restrict__ off, const float* __restrict__ fac,
-                                   int n_off, float nodata_dem, ZR zr) {
-    const int64_t idx = (int64_t)y * W + x;
-    if (sf[idx] & SF_BURN) return 0.0f;
-    for (int o = 0; o < n_off; ++o) {
-        int nx = x + off[2 * o], ny = y + off[2 * o + 1];
-        if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
-        int64_t ni = (int64_t)ny * W + nx;
-        if (!(sf[ni] & SF_BURN)) continue;
-        float seen = (ni < idx) ? 0.0f : zr(ni);
-        if (seen != nodata_dem) return zr(idx) * fac[o];
+__device__ inline void burn_tile(int W, int H, const uint8_t* __restrict__ sf,
+                                 const int* __restrict__ off, const float* __restrict__ fac,
+                                 int n_off, int R, float nodata_dem, ZR zr, float out[4],
+                                 int gx_out[4], int gy_out[4]) {
+    __shared__ float zt[BL_W * BL_H];
+    __shared__ uint8_t ft[BL_W * BL_H];
+    const int x0 = blockIdx.x * BT_X, y0 = blockIdx.y * BT_Y;
+    const int lw = BT_X + 2 * R, lh = BT_Y + 2 * R;
+    for (int k = threadIdx.x; k < lw * lh; k += 256) {
+        const int ly = k / lw, lx = k - ly * lw;
+        const int gx = x0 - R + lx, gy = y0 - R + ly;
+        float z = 0.0f;
+        uint8_t f = 0;
+        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+            const size_t gi = (size_t)gy * W + gx;
+            z = zr(gi);
+            f = sf[gi];
+        }
+        zt[ly * BL_W + lx] = z;
+        ft[ly * BL_W + lx] = f;
     }
-    return zr(idx);
+    __syncthreads();
+    const int lx = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int ly = (threadIdx.x >> 6) * 4 + j;
+        const int x = x0 + lx, y = y0 + ly;
+        gx_out[j] = x;
+        gy_out[j] = y;
+        const int c = (ly + R) * BL_W + lx + R;
+        float v = zt[c];
+        if (ft[c] & SF_BURN) {
+            v = 0.0f;
+        } else {
+            for (int o = 0; o < n_off; ++o) {
+                const int ox = off[2 * o], oy = off[2 * o + 1];
+                const int nx = x + ox, ny = y + oy;
+                if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
+                const int ni = c + oy * BL_W + ox;
+                if (!(ft[ni] & SF_BURN)) continue;
+                const float seen = (oy < 0 || (oy == 0 && ox < 0)) ? 0.0f : zt[ni];
+                if (seen != nodata_dem) {
+                    v = v * fac[o];
+                    break;
+                }
+            }
+        }
+        out[j] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_burn(const float* __restrict__ zin,
                                               float* __restrict__ zout,
                                               const uint8_t* __restrict__ sf,
                                               const int* __restrict__ off,
-                                              const float* __restrict__ fac, int n_off, int W,
-                                              int H, float nodata_dem) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    zout[(size_t)y * W + x] = burn_value(x, y, W, H, sf, off, fac, n_off, nodata_dem,
-                                         [&](int64_t i) { return zin[i]; });
+                                              const float* __restrict__ fac, int n_off, int R,
+                                              int W, int H, float nodata_dem) {
+    float v[4];
+    int gx[4], gy[4];
+    burn_tile(W, H, sf, off, fac, n_off, R, nodata_dem, [&](size_t i) { return zin[i]; }, v, gx,
+              gy);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (gx[j] < W && gy[j] < H) zout[(size_t)gy[j] * W + gx[j]] = v[j];
 }
 
 // Quantisation of PitFillingAlgorithmCPUOpenMP.h:84-106 for one value.
@@ -149,18 +193,22 @@ __global__ __launch_bounds__(256) void k_quantize_minmax(float* __restrict__ z, 
 // driver: one pass over dem, error field and stream flags.
 __global__ __launch_bounds__(256) void k_realistic_burn_quant(
     const float* __restrict__ dem, const float* __restrict__ e, const uint8_t* __restrict__ sf,
-    const int* __restrict__ off, const float* __restrict__ fac, int n_off, int W, int H,
+    const int* __restrict__ off, const float* __restrict__ fac, int n_off, int R, int W, int H,
     float nodata_dem, float* __restrict__ zq, uint32_t* __restrict__ scal) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    float v[4];
+    int gx[4], gy[4];
+    burn_tile(W, H, sf, off, fac, n_off, R, nodata_dem,
+              [&](size_t i) { return realistic_value(dem[i], e[i], nodata_dem); }, v, gx, gy);
     uint32_t mn = 0xffffffffu, mx = 0u, bad = 0u;
-    if (x < W && y < H) {
-        float zb = burn_value(x, y, W, H, sf, off, fac, n_off, nodata_dem, [&](int64_t i) {
-            return realistic_value(dem[i], e[i], nodata_dem);
-        });
-        float q = quantize_mm(zb, nodata_dem, &bad);
-        zq[(size_t)y * W + x] = q;
-        mn = mx = float_to_ordered(q);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (gx[j] < W && gy[j] < H) {
+            const float q = quantize_mm(v[j], nodata_dem, &bad);
+            zq[(size_t)gy[j] * W + gx[j]] = q;
+            const uint32_t o = float_to_ordered(q);
+            mn = min(mn, o);
+            mx = max(mx, o);
+        }
     }
     block_minmax(mn, mx, bad, scal);
 }
@@ -228,9 +276,9 @@ int launch_realistic(Context* c, const float* dem, float* field, float nodata) {
 
 int launch_burn(Context* c, const float* zin, float* zout, const uint8_t* sflags,
                 float nodata_dem) {
-    dim3 grid((c->W + 63) / 64, (c->H + 3) / 4);
+    dim3 grid((c->W + BT_X - 1) / BT_X, (c->H + BT_Y - 1) / BT_Y);
     hipLaunchKernelGGL(k_burn, grid, dim3(256), 0, c->stream, zin, zout, sflags, c->d_off,
-                       c->d_factor, c->n_offsets, c->W, c->H, nodata_dem);
+                       c->d_factor, c->n_offsets, c->off_radius, c->W, c->H, nodata_dem);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
@@ -243,9 +291,9 @@ int launch_quantize(Context* c, float* dem, float nodata) {
 }
 
 int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq) {
-    dim3 grid((c->W + 63) / 64, (c->H + 3) / 4);
+    dim3 grid((c->W + BT_X - 1) / BT_X, (c->H + BT_Y - 1) / BT_Y);
     hipLaunchKernelGGL(k_realistic_burn_quant, grid, dim3(256), 0, c->stream, dem, e,
-                       c->d_sflags, c->d_off, c->d_factor, c->n_offsets, c->W, c->H,
+                       c->d_sflags, c->d_off, c->d_factor, c->n_offsets, c->off_radius, c->W, c->H,
                        c->nodata_dem, zq, c->d_scal);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
