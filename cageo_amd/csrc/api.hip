@@ -39,7 +39,7 @@ int alloc(T** p, size_t count) {
 }
 
 void free_all(Context* c) {
-    void* ptrs[] = {c->d_work[0], c->d_work[1], c->d_mark, c->d_topen, c->d_scal, c->d_filter,
+    void* ptrs[] = {c->d_work[0], c->d_work[1], c->d_mark, c->d_topen, c->d_mark_pf, c->d_work3, c->d_ctrl, c->d_scal, c->d_filter,
                     c->d_sflags, c->d_slabel, c->d_off, c->d_factor, c->d_a, c->d_b, c->d_z,
                     c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr, c->d_ptr2};
     for (void* p : ptrs)
@@ -86,6 +86,21 @@ int read_scalars(Context* c) {
                                  hipMemcpyDeviceToHost, c->stream));
     CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
     return CAGEO_OK;
+}
+
+int clear_status(Context* c) {
+    CAGEO_HIP_TRY(hipMemsetAsync(c->d_ctrl + CTL_STATUS, 0, sizeof(uint32_t), c->stream));
+    return CAGEO_OK;
+}
+
+// Synchronises and returns the sticky device status of the persistent kernels.
+int check_status(Context* c) {
+    uint32_t st = 0;
+    CAGEO_HIP_TRY(hipMemcpyAsync(&c->h_scal[SC_NSLOTS - 1], c->d_ctrl + CTL_STATUS,
+                                 sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    st = c->h_scal[SC_NSLOTS - 1];
+    return st ? -(int)st : CAGEO_OK;
 }
 
 uint32_t next_epoch(Context* c) {
@@ -141,6 +156,9 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
     A(cageo::alloc(&c->d_work[1], c->ntiles));
     A(cageo::alloc(&c->d_mark, c->ntiles));
     A(cageo::alloc(&c->d_topen, c->ntiles));
+    A(cageo::alloc(&c->d_mark_pf, c->ntiles));
+    A(cageo::alloc(&c->d_work3, 3 * (size_t)c->ntiles));
+    A(cageo::alloc(&c->d_ctrl, cageo::CTL_WORDS));
     A(cageo::alloc(&c->d_scal, cageo::SC_NSLOTS));
     A(cageo::alloc(&c->d_sflags, n));
     A(cageo::alloc(&c->d_slabel, n));
@@ -163,6 +181,8 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
         st = CAGEO_ERR_NOMEM;
     if (st == CAGEO_OK &&
         (hipMemsetAsync(c->d_mark, 0, (size_t)c->ntiles * sizeof(uint32_t), c->stream) != hipSuccess ||
+         hipMemsetAsync(c->d_mark_pf, 0, (size_t)c->ntiles * sizeof(uint32_t), c->stream) != hipSuccess ||
+         hipMemsetAsync(c->d_ctrl, 0, cageo::CTL_WORDS * sizeof(uint32_t), c->stream) != hipSuccess ||
          hipMemsetAsync(c->d_scal, 0, cageo::SC_NSLOTS * sizeof(uint32_t), c->stream) != hipSuccess ||
          hipStreamSynchronize(c->stream) != hipSuccess))
         st = CAGEO_ERR_HIP;
@@ -258,10 +278,11 @@ int cageo_stream_burn(cageo_context* ctx, float* d_z, const float* d_stream, int
 int cageo_pit_fill(cageo_context* ctx, float* d_filled, float* d_dem, float nodata) {
     if (!ctx || !d_filled || !d_dem || d_filled == d_dem) return CAGEO_ERR_ARG;
     Context* c = &ctx->c;
+    CAGEO_TRY(cageo::clear_status(c));
     CAGEO_TRY(cageo::reset_minmax(c));
     CAGEO_TRY(cageo::launch_quantize(c, d_dem, nodata));
     CAGEO_TRY(cageo::pit_fill_quantized(c, d_dem, d_filled, nodata, true));
-    return cageo::sync(c);
+    return cageo::check_status(c);
 }
 
 int cageo_flow_routing(cageo_context* ctx, int8_t* d_flowdir, const float* d_filled,
@@ -314,6 +335,7 @@ int cageo_run_realizations(cageo_context* ctx, float* d_mc, int n, uint64_t seed
     if (!c->mc_ready || c->model < 0) return CAGEO_ERR_STATE;
     const bool timed = c->timing;
     if (timed) std::fill(c->stage_ms, c->stage_ms + 7, 0.0f);
+    CAGEO_TRY(cageo::clear_status(c));
     for (int r = 0; r < n; ++r) {
         if (timed) CAGEO_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
         // RandomFieldAlgorithm::execute(rand, seed), main.cpp:189-192
@@ -330,7 +352,7 @@ int cageo_run_realizations(cageo_context* ctx, float* d_mc, int n, uint64_t seed
             }
         }
     }
-    return cageo::sync(c);
+    return cageo::check_status(c);
 }
 
 int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc,
@@ -341,7 +363,9 @@ int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc
     if (!c->mc_ready || c->model < 0) return CAGEO_ERR_STATE;
     CAGEO_HIP_TRY(hipMemcpyAsync(c->d_a, d_noise, c->n * sizeof(float), hipMemcpyDeviceToDevice,
                                  c->stream));
+    CAGEO_TRY(cageo::clear_status(c));
     CAGEO_TRY(cageo::realization_body(c, d_mc, false, 0));
+    CAGEO_TRY(cageo::check_status(c));
     if (d_filled_out)
         CAGEO_HIP_TRY(hipMemcpyAsync(d_filled_out, c->d_w, c->n * sizeof(float),
                                      hipMemcpyDeviceToDevice, c->stream));

@@ -52,6 +52,10 @@ struct Context {
     int* d_work[2] = {nullptr, nullptr};
     uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
     uint32_t* d_topen = nullptr; // per-tile flag: tile holds OPEN pit-fill cells
+    uint32_t* d_mark_pf = nullptr;  // epoch stamps of the pit-fill kernel
+    int* d_work3 = nullptr;         // 3 worklists for the persistent kernels
+    uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
+    int pf_grid = 0, fr_grid = 0, tr_grid = 0;
     uint32_t epoch = 0;
 
     // scalars
@@ -134,6 +138,19 @@ __host__ __device__ inline bool pf_live(float v, float min_elev) {
     return pf_to_elev(pf_to_index(v, min_elev), min_elev) == v;
 }
 
+// Words of the control block (d_ctrl), shared by the persistent kernels.
+enum : int {
+    CTL_COUNT = 0,      // 3 worklist counts
+    CTL_CERT = 3,       // 3 rotating per-round fixed-cell counters (pit fill)
+    CTL_STATUS = 6,     // sticky error: -CAGEO_ERR_* of the first failing stage
+    CTL_EPOCH = 7,      // epoch counter of the pit-fill worklists
+    CTL_ROUNDS = 8,
+    CTL_PASSES = 9,
+    CTL_FLAG = 10,      // 3 rotating "pending" flags (tracing)
+    CTL_EPOCH_FR = 13,  // epoch counter of the flat-resolution worklists
+    CTL_WORDS = 16
+};
+
 // Worklist helpers --------------------------------------------------------
 int worklist_all_tiles(Context* c, int buf);
 int read_scalars(Context* c);
@@ -152,6 +169,8 @@ int launch_stream_flags(Context* c, const float* stream, float nodata_stream, ui
                         int32_t* slabel);
 int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq);
 int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool stats_ready);
+int check_status(Context* c);
+int clear_status(Context* c);
 int launch_quantize(Context* c, float* dem, float nodata);
 int flow_routing(Context* c, const float* z, int8_t* fd);
 int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,

@@ -18,9 +18,9 @@
 //            over the OPEN cells, relay(c) = W(c) for OPEN and live FIXED
 //            cells.  Every relay the reference makes is available, so at
 //            convergence W <= P on the OPEN cells.  During the relaxation an
-//            OPEN cell with W(n) == Z(n) and a live FIXED neighbour c with
-//            P(c) <= Z(n) is fixed at once: P(n) <= max(Z(n), P(c)) = Z(n)
-//            and P(n) >= Z(n).  A cell fixed dead stops relaying.
+//            OPEN cell with a live FIXED neighbour c, P(c) <= Z(n), is fixed
+//            at Z(n) at once: P(n) <= max(Z(n), P(c)) = Z(n) <= P(n).  A cell
+//            fixed dead stops relaying.
 //   certify: after convergence an OPEN cell with a live FIXED neighbour c,
 //            P(c) <= W(n), is fixed: P(n) <= max(Z(n), P(c)) <= W(n) <= P(n).
 //   next:    OPEN cells restart at +inf.
@@ -28,8 +28,16 @@
 // relay path into it through OPEN cells has bottleneck >= its value), so a
 // round that fixes nothing leaves only unreached cells; they keep Z.
 //
+// Execution: one persistent cooperative kernel runs every round.  Passes
+// over a de-duplicated worklist of 64x64 tiles are separated by grid
+// barriers; inside a tile the relaxation sweeps only cells whose
+// neighbourhood changed (dirty bits in LDS).  Every decision that ends a
+// loop is read by all workgroups after a barrier, so they leave together.
+//
 // Every comparison is on floats; the only arithmetic is the quantisation and
 // the round-trip test, both evaluated exactly as the reference does.
+#include <hip/hip_cooperative_groups.h>
+
 #include "tiles.hpp"
 
 namespace cageo {
@@ -38,102 +46,128 @@ namespace {
 
 constexpr float PF_INF = __builtin_huge_valf();
 
-__device__ inline float min8(const float* s, int i) {
-    float a = fminf(s[i - HALO_W - 1], s[i - HALO_W]);
-    float b = fminf(s[i - HALO_W + 1], s[i - 1]);
-    float c = fminf(s[i + 1], s[i + HALO_W - 1]);
-    float d = fminf(s[i + HALO_W], s[i + HALO_W + 1]);
+struct PfShared {
+    float2 rel[HALO_W * HALO_W];  // x: relay (OPEN or live FIXED), y: live FIXED only
+    unsigned long long dirty[2][TILE];
+    uint32_t bits;
+    uint32_t any;
+};
+
+__device__ inline float min8x(const float2* s, int i) {
+    float a = fminf(s[i - HALO_W - 1].x, s[i - HALO_W].x);
+    float b = fminf(s[i - HALO_W + 1].x, s[i - 1].x);
+    float c = fminf(s[i + 1].x, s[i + HALO_W - 1].x);
+    float d = fminf(s[i + HALO_W].x, s[i + HALO_W + 1].x);
+    return fminf(fminf(a, b), fminf(c, d));
+}
+__device__ inline float min8y(const float2* s, int i) {
+    float a = fminf(s[i - HALO_W - 1].y, s[i - HALO_W].y);
+    float b = fminf(s[i - HALO_W + 1].y, s[i - 1].y);
+    float c = fminf(s[i + 1].y, s[i + HALO_W - 1].y);
+    float d = fminf(s[i + HALO_W].y, s[i + HALO_W + 1].y);
     return fminf(fminf(a, b), fminf(c, d));
 }
 
-__device__ inline float min_elev_of(const uint32_t* scal) {
-    return ordered_to_float(scal[SC_MIN]);
+// Marks the in-tile 8-neighbours of (lx, ly) dirty in `d`.
+__device__ inline void mark_dirty(unsigned long long* d, int lx, int ly) {
+    const unsigned long long m =
+        (lx == 0) ? 3ull : (lx == TILE - 1 ? (3ull << (TILE - 2)) : (7ull << (lx - 1)));
+    if (ly > 0) atomicOr(&d[ly - 1], m);
+    atomicOr(&d[ly], m & ~(1ull << lx));
+    if (ly < TILE - 1) atomicOr(&d[ly + 1], m);
 }
 
-// Loads relay (OPEN or live FIXED: W, else +inf) and crelay (live FIXED: W,
-// else +inf) for the tile and its 1-cell halo.
-__device__ inline void load_relays(float* relay, float* crelay, const float* __restrict__ w,
-                                   const uint8_t* __restrict__ st, int x0, int y0, int W,
-                                   int H) {
+__device__ inline void grid_barrier() {
+    __threadfence();
+    cooperative_groups::this_grid().sync();
+    __threadfence();
+}
+
+__device__ inline uint32_t ld(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct PfArgs {
+    const float* zq;
+    float* w;
+    uint8_t* st;
+    int W, H, ntx, nty, ntiles;
+    float nodata;
+    int* work;            // 3 lists of ntiles entries
+    uint32_t* mark;       // per-tile epoch stamps
+    uint32_t* topen;      // per-tile: tile holds OPEN cells
+    uint32_t* ctrl;       // control block (CTL_*)
+    const uint32_t* scal; // min/max/bad of the quantized raster
+    int max_passes;
+};
+
+__device__ inline void load_tile(PfShared& S, const PfArgs& a, int x0, int y0) {
     for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
         int hy = k / HALO_W, hx = k - hy * HALO_W;
         int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
         float r = PF_INF, cr = PF_INF;
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            size_t gi = (size_t)gy * W + gx;
-            uint8_t s = st[gi];
-            if (s == PF_OPEN || s == PF_FIXED_LIVE) r = w[gi];
+        if (gx >= 0 && gy >= 0 && gx < a.W && gy < a.H) {
+            size_t gi = (size_t)gy * a.W + gx;
+            uint8_t s = a.st[gi];
+            if (s == PF_OPEN || s == PF_FIXED_LIVE) r = a.w[gi];
             if (s == PF_FIXED_LIVE) cr = r;
         }
-        relay[k] = r;
-        crelay[k] = cr;
+        S.rel[k] = make_float2(r, cr);
+    }
+    if (threadIdx.x < TILE) {
+        S.dirty[0][threadIdx.x] = ~0ull;
+        S.dirty[1][threadIdx.x] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        S.bits = 0;
+        S.any = 0;
     }
 }
 
-// Block-wide sum of per-thread counts, added to *dst by one thread.
-__device__ inline void block_add(uint32_t v, uint32_t* dst) {
-    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
-}
-
-__global__ __launch_bounds__(256) void k_pf_init(const float* __restrict__ zq,
-                                                 float* __restrict__ w,
-                                                 uint8_t* __restrict__ st, int W, int H,
-                                                 float nodata,
-                                                 const uint32_t* __restrict__ scal) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const size_t i = (size_t)y * W + x;
-    const float z = zq[i];
-    const float mn = min_elev_of(scal);
-    uint8_t s;
-    float v;
-    if (z == nodata) {
-        s = PF_NODATA;
-        v = z;
-    } else if (x == 0 || y == 0 || x == W - 1 || y == H - 1 || z == 0.0f) {
-        // seeds, PitFillingAlgorithmCPUOpenMP.h:121-144
-        s = pf_live(z, mn) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
-        v = z;
-    } else {
-        s = PF_OPEN;
-        v = PF_INF;
+// Ends a tile: OR of border bits, tile-open flag, neighbour activation.
+__device__ inline void finish_tile(PfShared& S, const PfArgs& a, int t, uint32_t bits,
+                                   uint32_t open_left, uint32_t nfixed, uint32_t* cert_ctr,
+                                   int* work_out, uint32_t* count_out, uint32_t epoch) {
+    for (int off = 32; off > 0; off >>= 1) {
+        bits |= (uint32_t)__shfl_xor((int)bits, off);
+        open_left |= (uint32_t)__shfl_xor((int)open_left, off);
+        nfixed += (uint32_t)__shfl_xor((int)nfixed, off);
     }
-    st[i] = s;
-    w[i] = v;
+    if ((threadIdx.x & 63) == 0) {
+        if (bits) atomicOr(&S.bits, bits);
+        if (open_left) atomicOr(&S.any, 1u);
+        if (nfixed) atomicAdd(cert_ctr, nfixed);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.topen[t] = S.any;
+        if (S.bits)
+            activate_tiles(S.bits, t % a.ntx, t / a.ntx, a.ntx, a.nty, work_out, count_out, a.mark,
+                           epoch);
+    }
+    __syncthreads();
 }
 
-// Fill relaxation with immediate fixing of cells that sit at their own
-// elevation next to a live fixed neighbour no higher than them.
-__global__ __launch_bounds__(256) void k_pf_fill(
-    const int* __restrict__ work, const float* __restrict__ zq, float* __restrict__ w,
-    uint8_t* __restrict__ st, int W, int H, int ntx, int nty, int* __restrict__ work_out,
-    uint32_t* count_out, uint32_t* __restrict__ mark, uint32_t epoch,
-    uint32_t* __restrict__ topen, uint32_t* __restrict__ scal) {
-    __shared__ float relay[HALO_W * HALO_W];
-    __shared__ float crelay[HALO_W * HALO_W];
-    const int t = work[blockIdx.x];
-    const int tx = t % ntx, ty = t / ntx;
-    const int x0 = tx * TILE, y0 = ty * TILE;
-    const float mn = min_elev_of(scal);
-    load_relays(relay, crelay, w, st, x0, y0, W, H);
-
+// Fill relaxation of one tile (with immediate fixing at Z).
+__device__ void fill_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_t* cert_ctr,
+                          int* work_out, uint32_t* count_out, uint32_t epoch) {
+    const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+    load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
     float zr[STRIP], wr[STRIP];
     uint32_t open = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        int gy = y0 + strip * STRIP + k;
+        const int gy = y0 + strip * STRIP + k;
         zr[k] = 0.0f;
         wr[k] = PF_INF;
-        if (gx < W && gy < H) {
-            size_t gi = (size_t)gy * W + gx;
-            if (st[gi] == PF_OPEN) {
+        if (gx < a.W && gy < a.H) {
+            const size_t gi = (size_t)gy * a.W + gx;
+            if (a.st[gi] == PF_OPEN) {
                 open |= 1u << k;
-                zr[k] = zq[gi];
-                wr[k] = w[gi];
+                zr[k] = a.zq[gi];
+                wr[k] = a.w[gi];
             }
         }
     }
@@ -141,32 +175,44 @@ __global__ __launch_bounds__(256) void k_pf_fill(
 
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
     for (int sweep = 0;; ++sweep) {
+        unsigned long long* cur = S.dirty[sweep & 1];
+        unsigned long long* nxt = S.dirty[(sweep + 1) & 1];
         int changed = 0;
-        if (open) {
 #pragma unroll
-            for (int kk = 0; kk < STRIP; ++kk) {
-                const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
-                if (!(open & (1u << k))) continue;
-                const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
-                const float m = min8(relay, li);
-                const float nw = fmaxf(zr[k], fminf(wr[k], m));
+        for (int kk = 0; kk < STRIP; ++kk) {
+            const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
+            const int ly = strip * STRIP + k;
+            const unsigned long long bit = 1ull << lx;
+            if (!(cur[ly] & bit)) continue;
+            atomicAnd(&cur[ly], ~bit);
+            if (!(open & (1u << k))) continue;
+            const int li = (ly + 1) * HALO_W + lx + 1;
+            bool ch = false;
+            if (min8y(S.rel, li) <= zr[k]) {
+                // a live fixed neighbour no higher than Z: P = Z
+                open &= ~(1u << k);
+                fixed |= 1u << k;
+                if (wr[k] != zr[k]) lowered |= 1u << k;
+                wr[k] = zr[k];
+                if (pf_live(zr[k], mn)) {
+                    fixed_live |= 1u << k;
+                    S.rel[li] = make_float2(zr[k], zr[k]);
+                } else {
+                    S.rel[li] = make_float2(PF_INF, PF_INF);
+                }
+                ch = true;
+            } else {
+                const float nw = fmaxf(zr[k], fminf(wr[k], min8x(S.rel, li)));
                 if (nw < wr[k]) {
                     wr[k] = nw;
-                    relay[li] = nw;
-                    changed = 1;
+                    S.rel[li].x = nw;
                     lowered |= 1u << k;
+                    ch = true;
                 }
-                if (wr[k] == zr[k] && min8(crelay, li) <= wr[k]) {
-                    open &= ~(1u << k);
-                    fixed |= 1u << k;
-                    changed = 1;
-                    if (pf_live(wr[k], mn)) {
-                        fixed_live |= 1u << k;
-                        crelay[li] = wr[k];
-                    } else {
-                        relay[li] = PF_INF;
-                    }
-                }
+            }
+            if (ch) {
+                changed = 1;
+                mark_dirty(nxt, lx, ly);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -176,50 +222,36 @@ __global__ __launch_bounds__(256) void k_pf_fill(
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         const int ly = strip * STRIP + k;
-        const size_t gi = (size_t)(y0 + ly) * W + gx;
-        if (lowered & (1u << k)) w[gi] = wr[k];
+        const size_t gi = (size_t)(y0 + ly) * a.W + gx;
+        if (lowered & (1u << k)) a.w[gi] = wr[k];
         if (fixed & (1u << k)) {
-            st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+            a.st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
             ++nfixed;
         }
         if ((lowered | fixed) & (1u << k)) bits |= border_bits(lx, ly);
     }
-    block_add(nfixed, &scal[SC_CERTIFIED]);
-    const int any_open = __syncthreads_or(open != 0);
-    bits = block_or_bits(bits);
-    if (threadIdx.x == 0) {
-        topen[t] = any_open ? 1u : 0u;
-        if (bits) activate_tiles(bits, tx, ty, ntx, nty, work_out, count_out, mark, epoch);
-    }
+    finish_tile(S, a, t, bits, open, nfixed, cert_ctr, work_out, count_out, epoch);
 }
 
-// Certification of the remaining OPEN cells after the fill has converged.
-__global__ __launch_bounds__(256) void k_pf_cert(
-    const int* __restrict__ work, const float* __restrict__ w, uint8_t* __restrict__ st, int W,
-    int H, int ntx, int nty, int* __restrict__ work_out, uint32_t* count_out,
-    uint32_t* __restrict__ mark, uint32_t epoch, uint32_t* __restrict__ topen,
-    uint32_t* __restrict__ scal) {
-    __shared__ float relay[HALO_W * HALO_W];
-    __shared__ float crelay[HALO_W * HALO_W];
-    const int t = work[blockIdx.x];
-    const int tx = t % ntx, ty = t / ntx;
-    const int x0 = tx * TILE, y0 = ty * TILE;
-    const float mn = min_elev_of(scal);
-    load_relays(relay, crelay, w, st, x0, y0, W, H);
-
+// Certification of one tile after the fill has converged.
+__device__ void cert_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_t* cert_ctr,
+                          int* work_out, uint32_t* count_out, uint32_t epoch) {
+    const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+    load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
-    uint32_t open = 0;
+    uint32_t open = 0, open_inf = 0;
     float wr[STRIP];
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         const int gy = y0 + strip * STRIP + k;
         wr[k] = PF_INF;
-        if (gx < W && gy < H) {
-            size_t gi = (size_t)gy * W + gx;
-            if (st[gi] == PF_OPEN) {
-                wr[k] = w[gi];
+        if (gx < a.W && gy < a.H) {
+            const size_t gi = (size_t)gy * a.W + gx;
+            if (a.st[gi] == PF_OPEN) {
+                wr[k] = a.w[gi];
                 if (wr[k] < PF_INF) open |= 1u << k;
+                else open_inf |= 1u << k;
             }
         }
     }
@@ -227,162 +259,246 @@ __global__ __launch_bounds__(256) void k_pf_cert(
 
     uint32_t fixed = 0, fixed_live = 0;
     for (int sweep = 0;; ++sweep) {
+        unsigned long long* cur = S.dirty[sweep & 1];
+        unsigned long long* nxt = S.dirty[(sweep + 1) & 1];
         int changed = 0;
-        if (open) {
 #pragma unroll
-            for (int kk = 0; kk < STRIP; ++kk) {
-                const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
-                if (!(open & (1u << k))) continue;
-                const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
-                if (min8(crelay, li) <= wr[k]) {
-                    open &= ~(1u << k);
-                    fixed |= 1u << k;
+        for (int kk = 0; kk < STRIP; ++kk) {
+            const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
+            const int ly = strip * STRIP + k;
+            const unsigned long long bit = 1ull << lx;
+            if (!(cur[ly] & bit)) continue;
+            atomicAnd(&cur[ly], ~bit);
+            if (!(open & (1u << k))) continue;
+            const int li = (ly + 1) * HALO_W + lx + 1;
+            if (min8y(S.rel, li) <= wr[k]) {
+                open &= ~(1u << k);
+                fixed |= 1u << k;
+                if (pf_live(wr[k], mn)) {
+                    fixed_live |= 1u << k;
+                    S.rel[li].y = wr[k];
                     changed = 1;
-                    if (pf_live(wr[k], mn)) {
-                        fixed_live |= 1u << k;
-                        crelay[li] = wr[k];
-                    }
+                    mark_dirty(nxt, lx, ly);
                 }
             }
         }
         if (!__syncthreads_or(changed)) break;
     }
 
-    uint32_t bits = 0, nfixed = 0, still_open = 0;
+    uint32_t bits = 0, nfixed = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        const int ly = strip * STRIP + k;
-        const int gy = y0 + ly;
-        if (gx >= W || gy >= H) continue;
-        const size_t gi = (size_t)gy * W + gx;
         if (fixed & (1u << k)) {
-            st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+            const int ly = strip * STRIP + k;
+            a.st[(size_t)(y0 + ly) * a.W + gx] =
+                (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
             ++nfixed;
             if (fixed_live & (1u << k)) bits |= border_bits(lx, ly);
-        } else if (st[gi] == PF_OPEN) {
-            still_open = 1;
         }
     }
-    block_add(nfixed, &scal[SC_CERTIFIED]);
-    const int any_open = __syncthreads_or(still_open);
-    bits = block_or_bits(bits);
-    if (threadIdx.x == 0) {
-        topen[t] = any_open ? 1u : 0u;
-        if (bits) activate_tiles(bits, tx, ty, ntx, nty, work_out, count_out, mark, epoch);
+    finish_tile(S, a, t, bits, open | open_inf, nfixed, cert_ctr, work_out, count_out, epoch);
+}
+
+// Runs passes of `kind` (0 fill, 1 certify) from worklist `li` until a pass
+// lists nothing; returns the index of the (empty) list it stopped at, or -1
+// when the pass cap is hit.
+template <int KIND>
+__device__ int run_passes(PfShared& S, const PfArgs& a, int li, float mn, uint32_t* cert_ctr,
+                          uint32_t& epoch) {
+    for (int pass = 0; pass < a.max_passes; ++pass) {
+        const int cin = li, cout = (li + 1) % 3, cclr = (li + 2) % 3;
+        const uint32_t count = ld(&a.ctrl[CTL_COUNT + cin]);
+        if (count == 0) return cin;
+        ++epoch;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.ctrl[CTL_COUNT + cclr] = 0;
+            atomicAdd(&a.ctrl[CTL_PASSES], 1u);
+        }
+        const int* win = a.work + (size_t)cin * a.ntiles;
+        int* wout = a.work + (size_t)cout * a.ntiles;
+        for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+            const int t = win[i];
+            if (KIND == 0)
+                fill_tile(S, a, t, mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
+            else
+                cert_tile(S, a, t, mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
+        }
+        grid_barrier();
+        li = cout;
     }
+    return -1;
 }
 
-// Lists the tiles flagged as holding OPEN cells.
-__global__ __launch_bounds__(256) void k_list_flagged(const uint32_t* __restrict__ topen,
-                                                      int ntiles, int* __restrict__ work_out,
-                                                      uint32_t* count_out) {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
-        if (topen[t]) work_out[atomicAdd(count_out, 1u)] = t;
+// Lists the tiles flagged open into list (b+1) % 3 and returns its index.
+// `b` is the list run_passes stopped at: some workgroups may still be reading
+// its (zero) count, so it is left untouched; (b+1) % 3 was cleared during the
+// last pass and nobody reads it; (b+2) % 3 was last read before the last
+// barrier and is cleared here for the first pass of the next phase.  The
+// caller ends with a grid barrier.
+__device__ inline int build_open_list(const PfArgs& a, int b) {
+    const int l = (b + 1) % 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_COUNT + (b + 2) % 3] = 0;
+    int* out = a.work + (size_t)l * a.ntiles;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.ntiles; t += gridDim.x * blockDim.x)
+        if (a.topen[t]) out[atomicAdd(&a.ctrl[CTL_COUNT + l], 1u)] = t;
+    return l;
 }
 
-// OPEN cells of the listed tiles restart from +inf.
-__global__ __launch_bounds__(256) void k_pf_reset_open(const int* __restrict__ work,
-                                                       float* __restrict__ w,
-                                                       const uint8_t* __restrict__ st, int W,
-                                                       int H, int ntx) {
-    const int t = work[blockIdx.x];
-    const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
-    const int gx = x0 + (threadIdx.x & 63);
-    if (gx >= W) return;
-    for (int k = 0; k < STRIP; ++k) {
-        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
-        if (gy >= H) break;
-        const size_t gi = (size_t)gy * W + gx;
-        if (st[gi] == PF_OPEN) w[gi] = PF_INF;
+__global__ __launch_bounds__(256) void k_pf_persistent(PfArgs a) {
+    __shared__ PfShared S;
+    const uint32_t* scal = a.scal;
+    const uint32_t bad = ld(&scal[SC_BAD]);
+    const float mn = ordered_to_float(ld(&scal[SC_MIN]));
+    const float mx = ordered_to_float(ld(&scal[SC_MAX]));
+    // Domain of the reference's arithmetic: non-negative elevations and a
+    // bucket array fl(fl(max-min)*1000)+1 that float arithmetic indexes
+    // exactly (beyond 2^24 the reference reads past it).
+    const float span = (mx - mn) * 1000.0f;
+    if (bad || !(span < 16777216.0f)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_STATUS] = (uint32_t)(-CAGEO_ERR_RANGE);
+        return;  // uniform: every workgroup takes this branch
     }
-}
 
-// Cells never reached keep their quantized elevation.
-__global__ __launch_bounds__(256) void k_pf_finalize(const int* __restrict__ work,
-                                                     float* __restrict__ w,
-                                                     const float* __restrict__ zq,
-                                                     const uint8_t* __restrict__ st, int W,
-                                                     int H, int ntx) {
-    const int t = work[blockIdx.x];
-    const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
-    const int gx = x0 + (threadIdx.x & 63);
-    if (gx >= W) return;
-    for (int k = 0; k < STRIP; ++k) {
-        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
-        if (gy >= H) break;
-        const size_t gi = (size_t)gy * W + gx;
-        if (st[gi] == PF_OPEN) w[gi] = zq[gi];
+    // initial states (PitFillingAlgorithmCPUOpenMP.h:121-144 seeds) and the
+    // all-tiles worklist
+    const size_t n = (size_t)a.W * a.H;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % a.W), y = (int)(i / a.W);
+        const float z = a.zq[i];
+        uint8_t s;
+        float v;
+        if (z == a.nodata) {
+            s = PF_NODATA;
+            v = z;
+        } else if (x == 0 || y == 0 || x == a.W - 1 || y == a.H - 1 || z == 0.0f) {
+            s = pf_live(z, mn) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+            v = z;
+        } else {
+            s = PF_OPEN;
+            v = PF_INF;
+        }
+        a.st[i] = s;
+        a.w[i] = v;
+    }
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.ntiles; t += gridDim.x * blockDim.x)
+        a.work[t] = t;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl[CTL_COUNT + 0] = (uint32_t)a.ntiles;
+        a.ctrl[CTL_COUNT + 1] = 0;
+        a.ctrl[CTL_COUNT + 2] = 0;
+        a.ctrl[CTL_CERT + 0] = 0;
+        a.ctrl[CTL_CERT + 1] = 0;
+        a.ctrl[CTL_CERT + 2] = 0;
+    }
+    uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
+    grid_barrier();
+
+    int li = 0;
+    int round = 0;
+    for (;; ++round) {
+        // Round r counts into counter r % 3 and clears counter (r+1) % 3, whose
+        // last read (end of round r-2) is separated from here by barriers.
+        uint32_t* cert_ctr = &a.ctrl[CTL_CERT + round % 3];
+        if (round > 0) {
+            // OPEN cells of the listed tiles restart from +inf
+            const uint32_t count = ld(&a.ctrl[CTL_COUNT + li]);
+            const int* lst = a.work + (size_t)li * a.ntiles;
+            for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+                const int t = lst[i];
+                const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+                const int gx = x0 + (threadIdx.x & 63);
+                if (gx < a.W)
+                    for (int k = 0; k < STRIP; ++k) {
+                        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
+                        if (gy >= a.H) break;
+                        const size_t gi = (size_t)gy * a.W + gx;
+                        if (a.st[gi] == PF_OPEN) a.w[gi] = PF_INF;
+                    }
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_CERT + (round + 1) % 3] = 0;
+            grid_barrier();
+        }
+        li = run_passes<0>(S, a, li, mn, cert_ctr, epoch);
+        if (li < 0) break;
+        li = build_open_list(a, li);
+        grid_barrier();
+        if (ld(&a.ctrl[CTL_COUNT + li]) == 0) break;
+        li = run_passes<1>(S, a, li, mn, cert_ctr, epoch);
+        if (li < 0) break;
+        li = build_open_list(a, li);
+        grid_barrier();
+        const uint32_t open_tiles = ld(&a.ctrl[CTL_COUNT + li]);
+        if (open_tiles == 0) break;
+        if (ld(cert_ctr) == 0) {
+            // nothing fixed this round: the OPEN cells are never reached and
+            // keep their quantized elevation
+            const int* lst = a.work + (size_t)li * a.ntiles;
+            for (uint32_t i = blockIdx.x; i < open_tiles; i += gridDim.x) {
+                const int t = lst[i];
+                const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+                const int gx = x0 + (threadIdx.x & 63);
+                if (gx < a.W)
+                    for (int k = 0; k < STRIP; ++k) {
+                        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
+                        if (gy >= a.H) break;
+                        const size_t gi = (size_t)gy * a.W + gx;
+                        if (a.st[gi] == PF_OPEN) a.w[gi] = a.zq[gi];
+                    }
+            }
+            break;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl[CTL_EPOCH] = epoch;
+        a.ctrl[CTL_ROUNDS] = (uint32_t)round + 1;
+        if (li < 0) a.ctrl[CTL_STATUS] = (uint32_t)(-CAGEO_ERR_NOT_CONVERGED);
     }
 }
 
 }  // namespace
 
-static int list_open_tiles(Context* c, int buf, uint32_t* tiles) {
-    CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_COUNT0 + buf, 0, sizeof(uint32_t), c->stream));
-    hipLaunchKernelGGL(k_list_flagged, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
-                       c->d_topen, c->ntiles, c->d_work[buf], c->d_scal + SC_COUNT0 + buf);
-    CAGEO_HIP_TRY(hipGetLastError());
-    CAGEO_TRY(read_scalars(c));
-    *tiles = c->h_scal[SC_COUNT0 + buf];
+int pit_fill_persistent_grid(Context* c, int* grid) {
+    if (c->pf_grid > 0) {
+        *grid = c->pf_grid;
+        return CAGEO_OK;
+    }
+    int per_cu = 0;
+    CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_pf_persistent), TILE_THREADS, 0));
+    hipDeviceProp_t prop;
+    CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+    if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+    c->pf_grid = per_cu * prop.multiProcessorCount;
+    *grid = c->pf_grid;
     return CAGEO_OK;
 }
 
 // Pit fill of an already quantized raster whose min/max/bad scalars are in
-// the scalar block (written by the quantisation kernel).
+// the scalar block (written by the quantisation kernel).  Enqueues one
+// cooperative launch; the status word is checked by check_status().
 int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*stats_ready*/) {
-    CAGEO_TRY(read_scalars(c));
-    if (c->h_scal[SC_BAD]) return CAGEO_ERR_RANGE;
-    const float mn = ordered_to_float(c->h_scal[SC_MIN]);
-    const float mx = ordered_to_float(c->h_scal[SC_MAX]);
-    // The reference's bucket array holds fl(fl(max-min)*1000)+1 buckets; past
-    // 2^24 the float arithmetic indexes beyond it (undefined there).
-    const float span = (mx - mn) * 1000.0f;
-    if (!(span < 16777216.0f)) return CAGEO_ERR_RANGE;
-
-    const int W = c->W, H = c->H;
-    hipLaunchKernelGGL(k_pf_init, dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, c->stream,
-                       zq, w, c->d_st, W, H, nodata, c->d_scal);
-    CAGEO_HIP_TRY(hipGetLastError());
-
-    const int max_passes = 1 << 18;
-    auto fill = [&](const int* win, uint32_t cnt, int* wout, uint32_t* slot, uint32_t epoch) {
-        hipLaunchKernelGGL(k_pf_fill, dim3(cnt), dim3(TILE_THREADS), 0, c->stream, win, zq, w,
-                           c->d_st, W, H, c->ntx, c->nty, wout, slot, c->d_mark, epoch,
-                           c->d_topen, c->d_scal);
-        CAGEO_HIP_TRY(hipGetLastError());
-        return CAGEO_OK;
-    };
-    auto cert = [&](const int* win, uint32_t cnt, int* wout, uint32_t* slot, uint32_t epoch) {
-        hipLaunchKernelGGL(k_pf_cert, dim3(cnt), dim3(TILE_THREADS), 0, c->stream, win, w,
-                           c->d_st, W, H, c->ntx, c->nty, wout, slot, c->d_mark, epoch,
-                           c->d_topen, c->d_scal);
-        CAGEO_HIP_TRY(hipGetLastError());
-        return CAGEO_OK;
-    };
-
-    uint32_t tiles = (uint32_t)c->ntiles;
-    CAGEO_TRY(worklist_all_tiles(c, 0));
-    for (int round = 0;; ++round) {
-        if (round > (int)c->n + 1) return CAGEO_ERR_NOT_CONVERGED;
-        CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_CERTIFIED, 0, sizeof(uint32_t), c->stream));
-        if (round > 0) {
-            hipLaunchKernelGGL(k_pf_reset_open, dim3(tiles), dim3(TILE_THREADS), 0, c->stream,
-                               c->d_work[0], w, c->d_st, W, H, c->ntx);
-            CAGEO_HIP_TRY(hipGetLastError());
-        }
-        CAGEO_TRY(run_tile_passes(c, 0, tiles, max_passes, fill, nullptr));
-        CAGEO_TRY(list_open_tiles(c, 0, &tiles));
-        if (tiles == 0) break;
-        CAGEO_TRY(run_tile_passes(c, 0, tiles, max_passes, cert, nullptr));
-        CAGEO_TRY(list_open_tiles(c, 0, &tiles));
-        if (tiles == 0) break;
-        if (c->h_scal[SC_CERTIFIED] == 0) {
-            hipLaunchKernelGGL(k_pf_finalize, dim3(tiles), dim3(TILE_THREADS), 0, c->stream,
-                               c->d_work[0], w, zq, c->d_st, W, H, c->ntx);
-            CAGEO_HIP_TRY(hipGetLastError());
-            break;
-        }
-    }
+    int grid = 0;
+    CAGEO_TRY(pit_fill_persistent_grid(c, &grid));
+    PfArgs a;
+    a.zq = zq;
+    a.w = w;
+    a.st = c->d_st;
+    a.W = c->W;
+    a.H = c->H;
+    a.ntx = c->ntx;
+    a.nty = c->nty;
+    a.ntiles = c->ntiles;
+    a.nodata = nodata;
+    a.work = c->d_work3;
+    a.mark = c->d_mark_pf;
+    a.topen = c->d_topen;
+    a.ctrl = c->d_ctrl;
+    a.scal = c->d_scal;
+    a.max_passes = 1 << 20;
+    void* args[] = {&a};
+    CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pf_persistent),
+                                             dim3(grid), dim3(TILE_THREADS), args, 0, c->stream));
     return CAGEO_OK;
 }
 
