@@ -289,8 +289,9 @@ int cageo_flow_routing(cageo_context* ctx, int8_t* d_flowdir, const float* d_fil
                        float /*nodata_dem*/) {
     if (!ctx || !d_flowdir || !d_filled) return CAGEO_ERR_ARG;
     Context* c = &ctx->c;
+    CAGEO_TRY(cageo::clear_status(c));
     CAGEO_TRY(cageo::flow_routing(c, d_filled, d_flowdir));
-    return cageo::sync(c);
+    return cageo::check_status(c);
 }
 
 int cageo_flow_tracing(cageo_context* ctx, int32_t* d_labels, const int8_t* d_flowdir,
@@ -299,8 +300,9 @@ int cageo_flow_tracing(cageo_context* ctx, int32_t* d_labels, const int8_t* d_fl
     Context* c = &ctx->c;
     // The trace rule tests stream >= 0 only; the nodata value is irrelevant.
     CAGEO_TRY(cageo::launch_stream_flags(c, d_stream, -__builtin_huge_valf(), c->d_st, c->d_dist));
+    CAGEO_TRY(cageo::clear_status(c));
     CAGEO_TRY(cageo::flow_tracing(c, d_flowdir, c->d_st, c->d_dist, d_labels, long_paths));
-    return cageo::sync(c);
+    return cageo::check_status(c);
 }
 
 int cageo_catchment_border(cageo_context* ctx, float* d_mc, const int32_t* d_labels) {

@@ -14,7 +14,11 @@
 //   d(c) = 1 + min { d(n) : n an 8-neighbour, z_n <= z_c,
 //                            n has a direction (d = 0) or is a flat },
 // followed by direction = last n in order with d(n) = d(c) - 1 and
-// z_n <= z_c.  d is computed here by monotone min-relaxation on LDS tiles.
+// z_n <= z_c.  d is computed by monotone min-relaxation over 64x64 LDS
+// tiles (dirty-bit sweeps) inside one persistent cooperative kernel, with the
+// same worklist protocol as the pit fill.
+#include <hip/hip_cooperative_groups.h>
+
 #include "tiles.hpp"
 
 namespace cageo {
@@ -28,12 +32,46 @@ constexpr int D_OUT = -3;
 __device__ constexpr int FDX[8] = {0, 1, 1, 1, 0, -1, -1, -1};
 __device__ constexpr int FDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 
-__global__ __launch_bounds__(256) void k_d8(const float* __restrict__ z,
-                                            int8_t* __restrict__ fd,
-                                            int32_t* __restrict__ dist, int W, int H) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
+struct FrShared {
+    int2 dz[HALO_W * HALO_W];  // x: distance, y: bits of z
+    unsigned long long dirty[2][TILE];
+    uint32_t bits;
+    uint32_t any;
+};
+
+struct FrArgs {
+    const float* z;
+    int8_t* fd;
+    int32_t* dist;
+    int W, H, ntx, nty, ntiles;
+    int* work;
+    uint32_t* mark;
+    uint32_t* ctrl;
+    int max_passes;
+};
+
+__device__ inline void grid_barrier() {
+    __threadfence();
+    cooperative_groups::this_grid().sync();
+    __threadfence();
+}
+
+__device__ inline uint32_t ld(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void mark_dirty(unsigned long long* d, int lx, int ly) {
+    const unsigned long long m =
+        (lx == 0) ? 3ull : (lx == TILE - 1 ? (3ull << (TILE - 2)) : (7ull << (lx - 1)));
+    if (ly > 0) atomicOr(&d[ly - 1], m);
+    atomicOr(&d[ly], m & ~(1ull << lx));
+    if (ly < TILE - 1) atomicOr(&d[ly + 1], m);
+}
+
+// D8 of one cell (FlowRoutingAlgorithmCPUOpenMP.h:60-150); returns the
+// initial distance: 0 has a direction, D_STREAM, or D_INF (flat).
+__device__ inline int32_t d8_cell(const float* __restrict__ z, int8_t* __restrict__ fd, int x,
+                                  int y, int W, int H) {
     const size_t i = (size_t)y * W + x;
     int dxi = 0, dyi = 0;
     if (x == 0) dxi = -1;
@@ -74,65 +112,64 @@ __global__ __launch_bounds__(256) void k_d8(const float* __restrict__ z,
         }
     }
     reinterpret_cast<char2*>(fd)[i] = make_char2(ox, oy);
-    dist[i] = d;
+    return d;
 }
 
-__global__ __launch_bounds__(256) void k_flat_bfs(const int* __restrict__ work,
-                                                  const float* __restrict__ z,
-                                                  int32_t* __restrict__ dist, int W, int H,
-                                                  int ntx, int nty, int* __restrict__ work_out,
-                                                  uint32_t* count_out,
-                                                  uint32_t* __restrict__ mark, uint32_t epoch) {
-    __shared__ int32_t dv[HALO_W * HALO_W];
-    __shared__ float zv[HALO_W * HALO_W];
-    const int t = work[blockIdx.x];
-    const int tx = t % ntx, ty = t / ntx;
+__device__ void bfs_tile(FrShared& S, const FrArgs& a, int t, int* work_out, uint32_t* count_out,
+                         uint32_t epoch) {
+    const int tx = t % a.ntx, ty = t / a.ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
     for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
-        int hy = k / HALO_W, hx = k - hy * HALO_W;
-        int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-        int32_t d = D_OUT;
-        float zz = 0.0f;
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            size_t gi = (size_t)gy * W + gx;
-            d = dist[gi];
-            zz = z[gi];
+        const int hy = k / HALO_W, hx = k - hy * HALO_W;
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+        int2 v = make_int2(D_OUT, 0);
+        if (gx >= 0 && gy >= 0 && gx < a.W && gy < a.H) {
+            const size_t gi = (size_t)gy * a.W + gx;
+            v = make_int2(a.dist[gi], __float_as_int(a.z[gi]));
         }
-        dv[k] = d;
-        zv[k] = zz;
+        S.dz[k] = v;
     }
+    if (threadIdx.x < TILE) {
+        S.dirty[0][threadIdx.x] = ~0ull;
+        S.dirty[1][threadIdx.x] = 0ull;
+    }
+    if (threadIdx.x == 0) S.bits = 0;
     __syncthreads();
 
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
-    const int gx = x0 + lx;
     uint32_t flat = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
-        if (dv[li] >= 1) flat |= 1u << k;
+        if (S.dz[li].x >= 1) flat |= 1u << k;
     }
     uint32_t changed_cells = 0;
     for (int sweep = 0;; ++sweep) {
+        unsigned long long* cur = S.dirty[sweep & 1];
+        unsigned long long* nxt = S.dirty[(sweep + 1) & 1];
         int changed = 0;
-        if (flat) {
 #pragma unroll
-            for (int kk = 0; kk < STRIP; ++kk) {
-                const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
-                if (!(flat & (1u << k))) continue;
-                const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
-                const float zc = zv[li];
-                int best = dv[li];
+        for (int kk = 0; kk < STRIP; ++kk) {
+            const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
+            const int ly = strip * STRIP + k;
+            const unsigned long long bit = 1ull << lx;
+            if (!(cur[ly] & bit)) continue;
+            atomicAnd(&cur[ly], ~bit);
+            if (!(flat & (1u << k))) continue;
+            const int li = (ly + 1) * HALO_W + lx + 1;
+            const int2 c = S.dz[li];
+            const float zc = __int_as_float(c.y);
+            int best = c.x;
 #pragma unroll
-                for (int d = 0; d < 8; ++d) {
-                    const int ni = li + FDY[d] * HALO_W + FDX[d];
-                    const int dn = dv[ni];
-                    if (dn >= 0 && zv[ni] <= zc && dn + 1 < best) best = dn + 1;
-                }
-                if (best < dv[li]) {
-                    dv[li] = best;
-                    changed = 1;
-                    changed_cells |= 1u << k;
-                }
+            for (int d = 0; d < 8; ++d) {
+                const int2 nv = S.dz[li + FDY[d] * HALO_W + FDX[d]];
+                if (nv.x >= 0 && __int_as_float(nv.y) <= zc && nv.x + 1 < best) best = nv.x + 1;
+            }
+            if (best < c.x) {
+                S.dz[li].x = best;
+                changed = 1;
+                changed_cells |= 1u << k;
+                mark_dirty(nxt, lx, ly);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -142,56 +179,123 @@ __global__ __launch_bounds__(256) void k_flat_bfs(const int* __restrict__ work,
     for (int k = 0; k < STRIP; ++k) {
         if (changed_cells & (1u << k)) {
             const int ly = strip * STRIP + k;
-            dist[(size_t)(y0 + ly) * W + gx] = dv[(ly + 1) * HALO_W + lx + 1];
+            a.dist[(size_t)(y0 + ly) * a.W + x0 + lx] = S.dz[(ly + 1) * HALO_W + lx + 1].x;
             bits |= border_bits(lx, ly);
         }
     }
-    bits = block_or_bits(bits);
-    if (threadIdx.x == 0 && bits)
-        activate_tiles(bits, tx, ty, ntx, nty, work_out, count_out, mark, epoch);
+    for (int off = 32; off > 0; off >>= 1) bits |= (uint32_t)__shfl_xor((int)bits, off);
+    if ((threadIdx.x & 63) == 0 && bits) atomicOr(&S.bits, bits);
+    __syncthreads();
+    if (threadIdx.x == 0 && S.bits)
+        activate_tiles(S.bits, tx, ty, a.ntx, a.nty, work_out, count_out, a.mark, epoch);
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_flat_dir(const float* __restrict__ z,
-                                                  const int32_t* __restrict__ dist,
-                                                  int8_t* __restrict__ fd, int W, int H) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const size_t i = (size_t)y * W + x;
-    const int32_t d = dist[i];
-    if (d < 1 || d >= D_INF) return;
-    const float zc = z[i];
-    int sk = -1;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int nx = x + FDX[k], ny = y + FDY[k];
-        if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
-        const size_t ni = (size_t)ny * W + nx;
-        if (dist[ni] == d - 1 && z[ni] <= zc) sk = k;
+__global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
+    __shared__ FrShared S;
+    uint32_t epoch = ld(&a.ctrl[CTL_EPOCH_FR]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl[CTL_COUNT + 0] = 0;
+        a.ctrl[CTL_COUNT + 1] = 0;
+        a.ctrl[CTL_COUNT + 2] = 0;
     }
-    if (sk >= 0) reinterpret_cast<char2*>(fd)[i] = make_char2((int8_t)FDX[sk], (int8_t)FDY[sk]);
+    grid_barrier();
+
+    // D8 tile by tile; tiles holding flats open list 0.
+    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+        const int x = x0 + (threadIdx.x & 63);
+        int has_flat = 0;
+        if (x < a.W)
+            for (int k = 0; k < STRIP; ++k) {
+                const int y = y0 + (threadIdx.x >> 6) * STRIP + k;
+                if (y >= a.H) break;
+                const int32_t d = d8_cell(a.z, a.fd, x, y, a.W, a.H);
+                a.dist[(size_t)y * a.W + x] = d;
+                has_flat |= d == D_INF;
+            }
+        if (__syncthreads_or(has_flat) && threadIdx.x == 0)
+            a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = t;
+    }
+    grid_barrier();
+
+    int li = 0;
+    bool converged = false;
+    for (int pass = 0; pass < a.max_passes; ++pass) {
+        const int cin = li, cout = (li + 1) % 3, cclr = (li + 2) % 3;
+        const uint32_t count = ld(&a.ctrl[CTL_COUNT + cin]);
+        if (count == 0) {
+            converged = true;
+            break;
+        }
+        ++epoch;
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_COUNT + cclr] = 0;
+        const int* win = a.work + (size_t)cin * a.ntiles;
+        int* wout = a.work + (size_t)cout * a.ntiles;
+        for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+            bfs_tile(S, a, win[i], wout, &a.ctrl[CTL_COUNT + cout], epoch);
+        grid_barrier();
+        li = cout;
+    }
+    // Nothing below writes a control word that the pass loop reads, so no
+    // barrier is needed between the last count read and the direction pick.
+    if (!converged) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            a.ctrl[CTL_STATUS] = (uint32_t)(-CAGEO_ERR_NOT_CONVERGED);
+        return;
+    }
+
+    // Directions of the resolved flats.
+    const size_t n = (size_t)a.W * a.H;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int32_t d = a.dist[i];
+        if (d < 1 || d >= D_INF) continue;
+        const int x = (int)(i % a.W), y = (int)(i / a.W);
+        const float zc = a.z[i];
+        int sk = -1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int nx = x + FDX[k], ny = y + FDY[k];
+            if (nx < 0 || ny < 0 || nx >= a.W || ny >= a.H) continue;
+            const size_t ni = (size_t)ny * a.W + nx;
+            if (a.dist[ni] == d - 1 && a.z[ni] <= zc) sk = k;
+        }
+        if (sk >= 0)
+            reinterpret_cast<char2*>(a.fd)[i] = make_char2((int8_t)FDX[sk], (int8_t)FDY[sk]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_EPOCH_FR] = epoch;
 }
 
 }  // namespace
 
 int flow_routing(Context* c, const float* z, int8_t* fd) {
-    const int W = c->W, H = c->H;
-    dim3 grid((W + 63) / 64, (H + 3) / 4);
-    hipLaunchKernelGGL(k_d8, grid, dim3(256), 0, c->stream, z, fd, c->d_dist, W, H);
-    CAGEO_HIP_TRY(hipGetLastError());
-    CAGEO_TRY(worklist_all_tiles(c, 0));
-    int passes = 0;
-    CAGEO_TRY(run_tile_passes(
-        c, 0, (uint32_t)c->ntiles, 1 << 20,
-        [&](const int* win, uint32_t cnt, int* wout, uint32_t* slot, uint32_t epoch) {
-            hipLaunchKernelGGL(k_flat_bfs, dim3(cnt), dim3(TILE_THREADS), 0, c->stream, win, z,
-                               c->d_dist, W, H, c->ntx, c->nty, wout, slot, c->d_mark, epoch);
-            CAGEO_HIP_TRY(hipGetLastError());
-            return CAGEO_OK;
-        },
-        &passes));
-    hipLaunchKernelGGL(k_flat_dir, grid, dim3(256), 0, c->stream, z, c->d_dist, fd, W, H);
-    CAGEO_HIP_TRY(hipGetLastError());
+    if (c->fr_grid <= 0) {
+        int per_cu = 0;
+        CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(k_fr_persistent), TILE_THREADS, 0));
+        hipDeviceProp_t prop;
+        CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+        if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+        c->fr_grid = per_cu * prop.multiProcessorCount;
+    }
+    FrArgs a;
+    a.z = z;
+    a.fd = fd;
+    a.dist = c->d_dist;
+    a.W = c->W;
+    a.H = c->H;
+    a.ntx = c->ntx;
+    a.nty = c->nty;
+    a.ntiles = c->ntiles;
+    a.work = c->d_work3;
+    a.mark = c->d_mark;
+    a.ctrl = c->d_ctrl;
+    a.max_passes = 1 << 20;
+    void* args[] = {&a};
+    CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_fr_persistent),
+                                             dim3(c->fr_grid), dim3(TILE_THREADS), args, 0,
+                                             c->stream));
     return CAGEO_OK;
 }
 

@@ -19,6 +19,8 @@
 // than 100 001 steps away (where the reference's result depends on its
 // raster order) are counted.  performConnectFlowPath (:115-134) then maps a
 // label >= 0 to label[label] when that is negative.
+#include <hip/hip_cooperative_groups.h>
+
 #include "common.hpp"
 
 namespace cageo {
@@ -45,13 +47,9 @@ __device__ inline uint64_t hop(uint64_t e, uint64_t f) {
     return pack(k, s, value_of(f));
 }
 
-__global__ __launch_bounds__(256) void k_trace_tile(const int8_t* __restrict__ fd,
-                                                    const uint8_t* __restrict__ sf,
-                                                    const int32_t* __restrict__ slabel,
-                                                    uint64_t* __restrict__ ent, int W, int H,
-                                                    int ntx) {
-    __shared__ uint64_t L[TILE * TILE];
-    const int t = blockIdx.x;
+__device__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
+                           const uint8_t* __restrict__ sf, const int32_t* __restrict__ slabel,
+                           uint64_t* __restrict__ ent, int W, int H, int ntx, int t) {
     const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
@@ -124,86 +122,130 @@ __global__ __launch_bounds__(256) void k_trace_tile(const int8_t* __restrict__ f
             ent[(size_t)gy * W + gx] = e;
         }
     }
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_trace_exit_jump(const uint64_t* __restrict__ src,
-                                                         uint64_t* __restrict__ dst, int64_t n,
-                                                         uint32_t* __restrict__ scal) {
-    uint32_t pending = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t e = src[i];
-        if (kind_of(e) == K_EXIT) {
-            e = hop(e, src[value_of(e)]);
-            pending |= kind_of(e) == K_EXIT;
-        }
-        dst[i] = e;
+__device__ inline void grid_barrier() {
+    __threadfence();
+    cooperative_groups::this_grid().sync();
+    __threadfence();
+}
+
+__device__ inline uint32_t ld(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct TrArgs {
+    const int8_t* fd;
+    const uint8_t* sf;
+    const int32_t* slabel;
+    uint64_t* ent[2];
+    int32_t* lab;
+    int W, H, ntx, ntiles;
+    int max_rounds;
+    uint32_t* ctrl;
+    uint32_t* scal;
+};
+
+// Tile pass, exit jumping, labels and the connect step in one cooperative
+// launch.  Round r of the exit jumping raises flag r % 3 while exits remain
+// and clears flag (r+1) % 3, whose last read (end of round r-2) is separated
+// from the clear by barriers.
+__global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
+    __shared__ uint64_t L[TILE * TILE];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl[CTL_FLAG + 0] = 0;
+        a.ctrl[CTL_FLAG + 1] = 0;
+        a.ctrl[CTL_FLAG + 2] = 0;
+        a.scal[SC_LONGPATH] = 0;
     }
-    if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&scal[SC_CHANGED], 1u);
-}
+    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x)
+        trace_tile(L, a.fd, a.sf, a.slabel, a.ent[0], a.W, a.H, a.ntx, t);
+    grid_barrier();
 
-__global__ __launch_bounds__(256) void k_trace_final(const uint64_t* __restrict__ ent,
-                                                     int32_t* __restrict__ lab, int64_t n,
-                                                     uint32_t* __restrict__ scal) {
+    const size_t n = (size_t)a.W * a.H;
+    int cur = 0;
+    for (int r = 0; r < a.max_rounds; ++r) {
+        const uint64_t* src = a.ent[cur];
+        uint64_t* dst = a.ent[cur ^ 1];
+        uint32_t pending = 0;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (size_t)gridDim.x * blockDim.x) {
+            uint64_t e = src[i];
+            if (kind_of(e) == K_EXIT) {
+                e = hop(e, src[value_of(e)]);
+                pending |= kind_of(e) == K_EXIT;
+            }
+            dst[i] = e;
+        }
+        if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&a.ctrl[CTL_FLAG + r % 3], 1u);
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_FLAG + (r + 1) % 3] = 0;
+        grid_barrier();
+        cur ^= 1;
+        if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
+    }
+
     uint32_t longp = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t e = ent[i];
+    const uint64_t* fin = a.ent[cur];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t e = fin[i];
         int32_t v = 0;
         if (kind_of(e) == K_RESOLVED) {
             v = value_of(e);
             if (steps_of(e) > 100001u) ++longp;
         }
-        lab[i] = v;
+        a.lab[i] = v;
     }
-    if (longp) atomicAdd(&scal[SC_LONGPATH], longp);
-}
-
-__global__ __launch_bounds__(256) void k_trace_connect(int32_t* __restrict__ lab, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t v = lab[i];
-        if (v >= 0 && (int64_t)v < n) {
-            const int32_t t = lab[v];
-            if (t < 0) lab[i] = t;
+    for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
+    if ((threadIdx.x & 63) == 0 && longp) atomicAdd(&a.scal[SC_LONGPATH], longp);
+    grid_barrier();
+    // performConnectFlowPath: label[0] is final and never changes here.
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int32_t v = a.lab[i];
+        if (v >= 0 && (size_t)v < n) {
+            const int32_t t = a.lab[v];
+            if (t < 0) a.lab[i] = t;
         }
     }
 }
 
-inline unsigned grid_1d(size_t n) {
-    size_t g = (n + 255) / 256;
-    if (g > 32768) g = 32768;
-    return (unsigned)(g ? g : 1);
-}
-
 }  // namespace
 
-// Needs two 64-bit rasters: c->d_ptr and the scratch `alt` (same size).
+// Needs two 64-bit rasters: c->d_ptr and c->d_ptr2.  The long-path count is
+// left in the scalar block.
 int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,
                  int32_t* lab, int64_t* long_paths) {
-    hipLaunchKernelGGL(k_trace_tile, dim3(c->ntiles), dim3(TILE_THREADS), 0, c->stream, fd,
-                       sflags, slabel, c->d_ptr, c->W, c->H, c->ntx);
-    CAGEO_HIP_TRY(hipGetLastError());
-    uint64_t* cur = c->d_ptr;
-    uint64_t* alt = c->d_ptr2;
+    if (c->tr_grid <= 0) {
+        int per_cu = 0;
+        CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(k_tr_persistent), TILE_THREADS, 0));
+        hipDeviceProp_t prop;
+        CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+        if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+        c->tr_grid = per_cu * prop.multiProcessorCount;
+    }
+    TrArgs a;
+    a.fd = fd;
+    a.sf = sflags;
+    a.slabel = slabel;
+    a.ent[0] = c->d_ptr;
+    a.ent[1] = c->d_ptr2;
+    a.lab = lab;
+    a.W = c->W;
+    a.H = c->H;
+    a.ntx = c->ntx;
+    a.ntiles = c->ntiles;
     int max_rounds = 2;
     for (size_t m = 1; m < c->n; m <<= 1) ++max_rounds;
-    for (int r = 0; r < max_rounds; ++r) {
-        CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_CHANGED, 0, sizeof(uint32_t), c->stream));
-        hipLaunchKernelGGL(k_trace_exit_jump, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, cur,
-                           alt, (int64_t)c->n, c->d_scal);
-        CAGEO_HIP_TRY(hipGetLastError());
-        std::swap(cur, alt);
-        CAGEO_TRY(read_scalars(c));
-        if (!c->h_scal[SC_CHANGED]) break;
-    }
-    CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_LONGPATH, 0, sizeof(uint32_t), c->stream));
-    hipLaunchKernelGGL(k_trace_final, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, cur, lab,
-                       (int64_t)c->n, c->d_scal);
-    CAGEO_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_trace_connect, dim3(grid_1d(c->n)), dim3(256), 0, c->stream, lab,
-                       (int64_t)c->n);
-    CAGEO_HIP_TRY(hipGetLastError());
+    a.max_rounds = max_rounds;
+    a.ctrl = c->d_ctrl;
+    a.scal = c->d_scal;
+    void* args[] = {&a};
+    CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tr_persistent),
+                                             dim3(c->tr_grid), dim3(TILE_THREADS), args, 0,
+                                             c->stream));
     if (long_paths) {
         CAGEO_TRY(read_scalars(c));
         *long_paths = (int64_t)c->h_scal[SC_LONGPATH];

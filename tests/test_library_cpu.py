@@ -1,0 +1,70 @@
+"""The HIP library on the CPU side: it loads, exports every function the C
+header declares, and its host-only entry points (filter construction, status
+strings, synthetic inputs) behave.  No device calls are made here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cageo_amd as ca
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cageo_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cageo_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_lists_entry_points():
+    names = declared_functions()
+    for must in ["cageo_context_create", "cageo_pit_fill", "cageo_flow_routing",
+                 "cageo_flow_tracing", "cageo_catchment_border", "cageo_run_realizations"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ca.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_status_strings_and_version():
+    lib = ca.lib()
+    assert lib.cageo_status_string(0) == b"ok"
+    assert lib.cageo_status_string(-6) == b"input outside the supported domain"
+    assert b"gfx950" in lib.cageo_version()
+
+
+def test_create_filter_argument_checks():
+    lib = ca.lib()
+    out = np.zeros(31, np.float32)
+    assert lib.cageo_create_filter(7, 60.0, 10.0, 15, out.ctypes.data) == -1
+    assert lib.cageo_create_filter(1, 60.0, 0.0, 15, out.ctypes.data) == -1
+    assert lib.cageo_filter_radius(60.0, 0.0) == -1
+
+
+def test_synthetic_inputs():
+    dem, st = ca.synth_dem(256, 192, 5, 10.0, 400, -9999.0)
+    dem2, st2 = ca.synth_dem(256, 192, 5, 10.0, 400, -9999.0)
+    assert np.array_equal(dem, dem2) and np.array_equal(st, st2)
+    assert dem.shape == (192, 256) and dem.dtype == np.float32
+    assert 90.0 < dem.min() and dem.max() < 100.0 + 0.05 * 256 + 40 * 60 + 5
+    ids = st[st != -9999.0]
+    assert ids.size > 0 and ids.min() >= 1.0 and np.all(ids == np.round(ids))
+    # tilt of 0.05 m per cell along +x (dominates the hills at 1024 cells)
+    big, _ = ca.synth_dem(1024, 1024, 5, 10.0, 2000, -9999.0)
+    assert big[:, 512:].mean() - big[:, :512].mean() > 10.0
+    dem3, _ = ca.synth_dem(256, 192, 6, 10.0, 400, -9999.0)
+    assert not np.array_equal(dem, dem3)
+
+
+def test_context_requires_device_or_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("device present; covered by the GPU tests")
+    with pytest.raises(ca.CageoError):
+        ca.Context(16, 16)
