@@ -85,6 +85,11 @@ def lib():
         "cageo_realization_from_noise": ([vp, vp, vp, vp, vp, vp], i),
         "cageo_stage_times": ([vp, vp, i], i),
         "cageo_synth_dem": ([i, i, u64, f, i, f, vp, vp], i),
+        "cageo_to_byte_array": ([vp, vp, vp], i),
+        "cageo_read_header": ([ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(i),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(f), ctypes.POINTER(f)], i),
+        "cageo_write_jpeg_gray": ([ctypes.c_char_p, vp, i, i, i], i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -133,6 +138,26 @@ def synth_dem(width: int, height: int, seed: int, cell_size: float = 10.0,
     _check(lib().cageo_synth_dem(width, height, seed, cell_size, acc_threshold, nodata_stream,
                                  dem.ctypes.data, stream.ctypes.data), "synth_dem")
     return dem, stream
+
+
+def read_header(path: str) -> dict:
+    """The .hdr raster header (utils.cpp:58-112 loadHeaderData)."""
+    c, r = ctypes.c_int(), ctypes.c_int()
+    x, y = ctypes.c_double(), ctypes.c_double()
+    cs, nd = ctypes.c_float(), ctypes.c_float()
+    _check(lib().cageo_read_header(os.fsencode(path), ctypes.byref(c), ctypes.byref(r),
+                                   ctypes.byref(x), ctypes.byref(y), ctypes.byref(cs),
+                                   ctypes.byref(nd)), f"read_header({path})")
+    return {"ncols": c.value, "nrows": r.value, "xllcorner": x.value, "yllcorner": y.value,
+            "cellsize": cs.value, "NODATA_value": nd.value}
+
+
+def write_jpeg_gray(path: str, pixels: np.ndarray, quality: int = 90) -> None:
+    """Baseline grayscale JPEG of a uint8 H x W image (the CLI's .jpg output)."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint8)
+    h, w = px.shape
+    _check(lib().cageo_write_jpeg_gray(os.fsencode(path), px.ctypes.data, w, h, quality),
+           "write_jpeg_gray")
 
 
 class Context:
@@ -213,6 +238,9 @@ class Context:
         _check(lib().cageo_realization_from_noise(self._h, _ptr(noise), _ptr(mc), _ptr(filled),
                                                   _ptr(flowdir), _ptr(labels)),
                "realization_from_noise")
+
+    def to_byte_array(self, out_bytes, values):
+        _check(lib().cageo_to_byte_array(self._h, _ptr(out_bytes), _ptr(values)), "to_byte_array")
 
     def stage_times(self, enable=True):
         out = np.zeros(7, np.float32)

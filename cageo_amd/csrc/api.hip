@@ -312,6 +312,13 @@ int cageo_catchment_border(cageo_context* ctx, float* d_mc, const int32_t* d_lab
     return cageo::sync(c);
 }
 
+int cageo_to_byte_array(cageo_context* ctx, uint8_t* d_bytes, const float* d_values) {
+    if (!ctx || !d_bytes || !d_values) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::launch_to_bytes(c, d_bytes, d_values));
+    return cageo::sync(c);
+}
+
 int cageo_mc_setup(cageo_context* ctx, const float* d_dem, const float* d_stream,
                    float nodata_dem, float nodata_stream, int bevel_radius, float mean,
                    float std_dev) {

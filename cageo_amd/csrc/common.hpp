@@ -176,6 +176,7 @@ int flow_routing(Context* c, const float* z, int8_t* fd);
 int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,
                  int32_t* lab, int64_t* long_paths);
 int launch_border(Context* c, float* mc, const int32_t* lab);
+int launch_to_bytes(Context* c, uint8_t* out, const float* in);
 
 }  // namespace cageo
 

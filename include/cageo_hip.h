@@ -126,6 +126,12 @@ int cageo_flow_tracing(cageo_context* ctx, int32_t* d_labels, const int8_t* d_fl
  * mc != -1. */
 int cageo_catchment_border(cageo_context* ctx, float* d_mc, const int32_t* d_labels);
 
+/* ToByteArrayAlgorithm<float>::execute(bytes, values) (GPU
+ * ToByteArrayAlgorithmGPUCUDA.cu:22-84, CPU ToByteArrayAlgorithmCPUOpenMP.h:
+ * 32-52): bytes = (uint8) rintf(255 * ((v - min) / max)) with min and max over
+ * all cells; out-of-range values wrap modulo 256 and 0/0 gives 0. */
+int cageo_to_byte_array(cageo_context* ctx, uint8_t* d_bytes, const float* d_values);
+
 /* ---- fused Monte Carlo driver (main.cpp:180-283) -------------------------
  * cageo_mc_setup binds the resident inputs and the error model; the filter
  * comes from cageo_set_filter.  cageo_run_realizations runs n realizations
