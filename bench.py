@@ -11,18 +11,27 @@ convolution, realistic DEM, stream burning, pit filling, flow routing, flow
 tracing and border accumulation, with inputs resident in HBM.
 
 Prints one JSON line.  `value` is realizations/s of the Gaussian workload on
-one GPU.  `roofline` covers the dominant kernel (most device time per
-realization), measured with HIP events inside the library on its stream.
+one GPU.  Per-stage device times come from HIP events that the library
+records on its stream between the stages of every timed realization (no host
+synchronisation inside the timed region).  `roofline` covers the dominant
+kernel: algorithmic bytes per launch / its average launch time.  `traffic`
+is the HBM bytes per launch of that kernel from two rocprofv3 counter passes
+(FETCH_SIZE, then WRITE_SIZE) over a short probe run, scaled by what the same
+counters read for an elementwise kernel that reads and writes 512 MiB.
 `cpu_baseline` times the CPU oracle (a restatement of the reference's OpenMP
 CPU path) on a bounded sample, in a subprocess pinned to one thread.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -30,18 +39,26 @@ sys.path.insert(0, ROOT)
 
 METRIC = "realizations/s + Mcells/s at 4096² DEM; surrogate vs MC time @ L1≤0.02"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 NODATA = -9999.0
+CALIB_BYTES = 512 << 20  # the calibration kernel reads and writes this much
 
-# Algorithmic bytes per cell per realization of each stage, at the
-# reference's data types (one pass over stage-boundary I/O; iterations of the
-# fixed-point stages are not counted).
+# Algorithmic bytes per cell per realization of each stage at the reference's
+# data types (one pass over stage-boundary I/O; iterations of the fixed-point
+# stages are not counted).  realistic_burn is one fused kernel: the unfused
+# figure of realistic DEM (12) + burning (12) is counted.
 STAGE_BYTES = {
-    "random_field": 4, "convolution_gaussian": 16, "convolution_exponential": 8,
-    "realistic_burn": 24, "pit_fill": 8, "flow_routing": 6, "flow_tracing": 10,
-    "border": 12,
+    "random_field": 4, "convolution": 16, "realistic_burn": 24, "pit_fill": 8,
+    "flow_routing": 6, "flow_tracing": 10, "border": 12,
 }
 STAGE_NAMES = ["random_field", "convolution", "realistic_burn", "pit_fill", "flow_routing",
                "flow_tracing", "border"]
+# Stages that are exactly one kernel launch (the events bracket that launch).
+STAGE_KERNEL = {
+    "random_field": "k_random_field", "realistic_burn": "k_realistic_burn_quant",
+    "pit_fill": "k_pf_persistent", "flow_routing": "k_fr_persistent",
+    "flow_tracing": "k_tr_persistent", "border": "k_border",
+}
 
 
 def parse():
@@ -53,9 +70,13 @@ def parse():
     p.add_argument("--exp-steps", type=int, default=20,
                    help="realizations timed with the exponential filter (0 = skip)")
     p.add_argument("--seed", type=int, default=20240601)
-    p.add_argument("--cpu-sample", type=int, default=2,
+    p.add_argument("--cpu-sample", type=int, default=3,
                    help="realizations timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-stage-times", action="store_true")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the rocprofv3 counter passes (traffic = null)")
+    p.add_argument("--probe", action="store_true",
+                   help="internal: the short run profiled by the counter passes")
     return p.parse_args()
 
 
@@ -67,7 +88,7 @@ sys.path.insert(0, {ROOT!r})
 import numpy as np, oracle, cageo_amd as ca
 dem, st = ca.synth_dem({size}, {size}, {seed}, 10.0, 2000, {NODATA})
 r = oracle.filter_radius(60.0, 10.0)
-f = oracle.create_filter(oracle.GAUSSIAN, 60.0, 10.0, r)
+f = ca.create_filter(oracle.GAUSSIAN, 60.0, 10.0, r)
 mc = np.zeros(dem.shape, np.float32)
 t = time.perf_counter()
 for k in range({sample}):
@@ -84,6 +105,76 @@ print(json.dumps({{"seconds": dt, "n": {sample}}}))
     return res["n"] / res["seconds"], res["seconds"]
 
 
+def probe(args):
+    """Short run for the counter passes: 2 Gaussian realizations at full size
+    and one elementwise kernel over 512 MiB of known traffic."""
+    import torch
+    import cageo_amd as ca
+    size = args.size
+    dem, stream = ca.synth_dem(size, size, args.seed, 10.0, 2000, NODATA)
+    tdem, tst = torch.from_numpy(dem).cuda(), torch.from_numpy(stream).cuda()
+    r = ca.filter_radius(60.0, 10.0)
+    with ca.Context(size, size, torch.cuda.current_stream()) as ctx:
+        ctx.set_filter(ca.GAUSSIAN, r, ca.create_filter(ca.GAUSSIAN, 60.0, 10.0, r))
+        ctx.mc_setup(tdem, tst, NODATA, NODATA, 2, 0.0, 1.8)
+        mc = torch.zeros((size, size), dtype=torch.float32, device="cuda")
+        ctx.run_realizations(mc, 2, args.seed, 0)
+    a = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print("probe done", flush=True)
+
+
+def counter_pass(counter, args, outdir):
+    """One rocprofv3 --pmc pass over the probe; returns {kernel: [values]}."""
+    d = os.path.join(outdir, counter)
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
+           sys.executable, os.path.abspath(__file__), "--probe", "--size", str(args.size),
+           "--seed", str(args.seed)]
+    subprocess.run(cmd, capture_output=True, text=True, timeout=60, check=False)
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        return None
+    vals = {}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            row = {k.lower(): v for k, v in row.items()}
+            if row.get("counter_name") != counter:
+                continue
+            name = row.get("kernel_name", "")
+            vals.setdefault(name, []).append(float(row["counter_value"]))
+    return vals
+
+
+def measure_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` (read, write), calibrated; None if the
+    profiler or its counters are unavailable."""
+    if not shutil.which("rocprofv3"):
+        return None
+    outdir = tempfile.mkdtemp(prefix="cageo_pmc_")
+    try:
+        res = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals = counter_pass(counter, args, outdir)
+            if not vals:
+                return None
+            k = [v for name, v in vals.items() if kernel in name]
+            cal = [v for name, v in vals.items() if "elementwise" in name]
+            if not k or not cal:
+                return None
+            per_launch_kib = sum(k[0]) / len(k[0])
+            cal_kib = max(cal[0])
+            factor = CALIB_BYTES / (cal_kib * 1024.0)
+            res[counter] = {"kib_per_launch": per_launch_kib, "calibration_kib": cal_kib,
+                            "factor": factor, "bytes": per_launch_kib * 1024.0 * factor}
+        return res
+    except (subprocess.SubprocessError, OSError, ValueError, KeyError):
+        return None
+    finally:
+        shutil.rmtree(outdir, ignore_errors=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -91,13 +182,16 @@ def main():
               "realizations are independent and would shard by realization index, but no "
               "multi-GPU run is part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        probe(args)
+        return
 
-    import numpy as np
     import torch
     import cageo_amd as ca
 
     oracle_ok = os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so"))
     size = args.size
+    cells = size * size
     dem, stream = ca.synth_dem(size, size, args.seed, 10.0, 2000, NODATA)
     torch.cuda.init()
     tdem = torch.from_numpy(dem).cuda()
@@ -111,6 +205,7 @@ def main():
         ctx.set_filter(model, r, taps)
         ctx.mc_setup(tdem, tst, NODATA, NODATA, 2, 0.0, 1.8)
         mc = torch.zeros((size, size), dtype=torch.float32, device="cuda")
+        ctx.stage_times(stage_times)
         ctx.run_realizations(mc, warmup, args.seed, 0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -119,10 +214,7 @@ def main():
         dt = time.perf_counter() - t0
         stages = None
         if stage_times:
-            ctx.stage_times(True)
-            n_t = max(1, min(steps, 10))
-            ctx.run_realizations(mc, n_t, args.seed, warmup + steps)
-            ms = ctx.stage_times(False) / n_t
+            ms = ctx.stage_times(False) / steps
             stages = dict(zip(STAGE_NAMES, [round(float(v), 4) for v in ms]))
         total = float(mc.sum().item())
         ctx.close()
@@ -131,7 +223,6 @@ def main():
     dt, stages, total = run(ca.GAUSSIAN, 60.0, args.steps, args.warmup,
                             not args.no_stage_times)
     real_s = args.steps / dt
-    cells = size * size
     line = {
         "metric": METRIC,
         "value": round(real_s, 3),
@@ -156,16 +247,23 @@ def main():
     }
     if stages:
         line["stage_ms"] = stages
-        # dominant stage by device time; bytes per realization at the
-        # reference's data types (STAGE_BYTES)
-        key = max(stages, key=lambda k: stages[k])
-        per_cell = STAGE_BYTES["convolution_gaussian" if key == "convolution" else key]
-        achieved = per_cell * cells / (stages[key] * 1e-3) / 1e9
+        key = max(STAGE_KERNEL, key=lambda k: stages[k])
+        kernel = STAGE_KERNEL[key]
+        bytes_per_launch = STAGE_BYTES[key] * cells
+        achieved = bytes_per_launch / (stages[key] * 1e-3) / 1e9
+        traffic = None if args.no_traffic else measure_traffic(args, kernel)
         line["roofline"] = {
-            "bound": "hbm", "kernel": key, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "bytes_per_cell": per_cell,
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": (round(traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"])
+                        if traffic else None),
+            "kernel": kernel, "launch_ms": stages[key],
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "bytes_per_cell": STAGE_BYTES[key],
         }
+        if traffic:
+            line["roofline"]["traffic_detail"] = {
+                k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in traffic.items()}
     if args.exp_steps > 0:
         dte, st_e, _ = run(ca.EXPONENTIAL, 200.0, args.exp_steps, 1, not args.no_stage_times)
         line["exponential_pr200"] = {
@@ -176,7 +274,11 @@ def main():
         if st_e:
             conv_ms = st_e["convolution"]
             flops = 2.0 * (2 * ca.filter_radius(200.0, 10.0) + 1) ** 2 * cells
-            line["exponential_pr200"]["convolution_tflops"] = round(flops / conv_ms / 1e9, 2)
+            tf = flops / conv_ms / 1e9
+            line["exponential_pr200"]["convolution_roofline"] = {
+                "bound": "fp32-vector", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
+                "note": "k_conv_2d plus its output copy; no FMA (exact reference order)"}
     if args.cpu_sample > 0 and oracle_ok:
         cps, secs = cpu_baseline(size, args.cpu_sample, args.seed)
         line["cpu_baseline"] = {

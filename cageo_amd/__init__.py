@@ -90,6 +90,15 @@ def lib():
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(f), ctypes.POINTER(f)], i),
         "cageo_write_jpeg_gray": ([ctypes.c_char_p, vp, i, i, i], i),
+        "cageo_iteration_counts": ([vp, vp], i),
+        "cageo_context_configure": ([vp, i, i], i),
+        "cageo_batch_create": ([ctypes.POINTER(vp), i, vp, vp, vp], i),
+        "cageo_batch_destroy": ([vp], i),
+        "cageo_batch_context": ([vp, i, ctypes.POINTER(vp)], i),
+        "cageo_batch_setup_tile": ([vp, i, vp, vp, f, f, i, i, f, f, f, f, u64], i),
+        "cageo_batch_run": ([vp, vp, i, u32, i], i),
+        "cageo_batch_tune": ([vp, vp, i, vp], i),
+        "cageo_batch_configure_tile": ([vp, i, i, i], i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -138,6 +147,66 @@ def synth_dem(width: int, height: int, seed: int, cell_size: float = 10.0,
     _check(lib().cageo_synth_dem(width, height, seed, cell_size, acc_threshold, nodata_stream,
                                  dem.ctypes.data, stream.ctypes.data), "synth_dem")
     return dem, stream
+
+
+BATCH_PER_TILE, BATCH_STREAM, BATCH_GRAPH = 0, 1, 2
+
+
+class Batch:
+    """Many tiles of uneven size x many realizations (cageo_batch)."""
+
+    def __init__(self, shapes, stream=None):
+        """shapes: list of (height, width)."""
+        self.shapes = [tuple(map(int, s)) for s in shapes]
+        n = len(self.shapes)
+        ws = (ctypes.c_int * n)(*[w for _, w in self.shapes])
+        hs = (ctypes.c_int * n)(*[h for h, _ in self.shapes])
+        h = ctypes.c_void_p()
+        sp = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream")
+                                          else stream)
+        _check(lib().cageo_batch_create(ctypes.byref(h), n, ws, hs, sp), "batch_create")
+        self._h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cageo_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def setup_tile(self, t, dem, stream, nodata_dem, nodata_stream, bevel_radius, model,
+                   practical_range, cell_size, mean, std, seed):
+        self._keep.append((dem, stream))
+        _check(lib().cageo_batch_setup_tile(self._h, t, _ptr(dem), _ptr(stream), nodata_dem,
+                                            nodata_stream, bevel_radius, model, practical_range,
+                                            cell_size, mean, std, seed), "batch_setup_tile")
+
+    @staticmethod
+    def _ptrs(mcs):
+        return (ctypes.c_void_p * len(mcs))(*[_ptr(m) for m in mcs])
+
+    def run(self, mcs, n, first=0, mode=BATCH_GRAPH):
+        _check(lib().cageo_batch_run(self._h, self._ptrs(mcs), n, first, mode), "batch_run")
+
+    def tune(self, scratch_mcs, reps=2):
+        out = (ctypes.c_int * (2 * len(self.shapes)))()
+        _check(lib().cageo_batch_tune(self._h, self._ptrs(scratch_mcs), reps, out), "batch_tune")
+        return [(out[2 * t], out[2 * t + 1]) for t in range(len(self.shapes))]
+
+    def configure_tile(self, t, persistent_workgroups=0, conv_rows=0):
+        _check(lib().cageo_batch_configure_tile(self._h, t, persistent_workgroups, conv_rows),
+               "batch_configure_tile")
 
 
 def read_header(path: str) -> dict:
@@ -241,6 +310,16 @@ class Context:
 
     def to_byte_array(self, out_bytes, values):
         _check(lib().cageo_to_byte_array(self._h, _ptr(out_bytes), _ptr(values)), "to_byte_array")
+
+    def iteration_counts(self):
+        """(pit-fill rounds, pit-fill passes, flat passes, tracing rounds) of the last run."""
+        out = np.zeros(4, np.int32)
+        _check(lib().cageo_iteration_counts(self._h, out.ctypes.data), "iteration_counts")
+        return tuple(int(v) for v in out)
+
+    def configure(self, persistent_workgroups=0, conv_rows=0):
+        _check(lib().cageo_context_configure(self._h, persistent_workgroups, conv_rows),
+               "context_configure")
 
     def stage_times(self, enable=True):
         out = np.zeros(7, np.float32)

@@ -16,10 +16,24 @@ __global__ void k_iota(int* __restrict__ out, int n) {
         out[i] = i;
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// min <- max ordered value, max and "bad" <- 0.  A kernel rather than memsets
+// so that a captured graph holds no memset nodes.
+__global__ void k_reset_minmax(uint32_t* __restrict__ scal) {
+    if (threadIdx.x == 0) {
+        scal[SC_MIN] = 0xffffffffu;
+        scal[SC_MAX] = 0u;
+        scal[SC_BAD] = 0u;
+    }
+}
+
 int reset_minmax(Context* c) {
-    CAGEO_HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->d_scal + SC_MIN),
-                                    0xffffffffu, 1, c->stream));
-    CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_MAX, 0, 2 * sizeof(uint32_t), c->stream));
+    hipLaunchKernelGGL(k_reset_minmax, dim3(1), dim3(64), 0, c->stream, c->d_scal);
+    CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
 
@@ -47,19 +61,25 @@ void free_all(Context* c) {
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+    c->evpool.clear();
 }
 
+}  // namespace
+
 // One realization from the error field already in c->d_a (stages a3..a9).
-int realization_body(Context* c, float* mc, bool timed, int first_event) {
-    int ev = first_event;
+// With `evs` (8 events), evs[2..7] are recorded after each stage.
+int realization_body(Context* c, float* mc, hipEvent_t* evs) {
+    int ev = 2;
     auto mark = [&]() -> int {
-        if (timed) CAGEO_HIP_TRY(hipEventRecord(c->ev[ev++], c->stream));
+        if (evs) CAGEO_HIP_TRY(hipEventRecord(evs[ev++], c->stream));
         return CAGEO_OK;
     };
-    CAGEO_TRY(launch_convolve(c, c->d_a, c->d_b));
+    float* e = nullptr;
+    CAGEO_TRY(launch_convolve_to(c, c->d_a, c->d_b, &e));
     CAGEO_TRY(mark());
     CAGEO_TRY(reset_minmax(c));
-    CAGEO_TRY(launch_realistic_burn_quant(c, c->dem, c->d_a, c->d_z));
+    CAGEO_TRY(launch_realistic_burn_quant(c, c->dem, e, c->d_z));
     CAGEO_TRY(mark());
     CAGEO_TRY(pit_fill_quantized(c, c->d_z, c->d_w, c->nodata_dem, true));
     CAGEO_TRY(mark());
@@ -72,7 +92,13 @@ int realization_body(Context* c, float* mc, bool timed, int first_event) {
     return CAGEO_OK;
 }
 
-}  // namespace
+
+int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, int count) {
+    const int grid = std::max(1, std::min((count + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, c->stream, p, value, count);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
 
 int worklist_all_tiles(Context* c, int buf) {
     hipLaunchKernelGGL(k_iota, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
@@ -343,25 +369,37 @@ int cageo_run_realizations(cageo_context* ctx, float* d_mc, int n, uint64_t seed
     Context* c = &ctx->c;
     if (!c->mc_ready || c->model < 0) return CAGEO_ERR_STATE;
     const bool timed = c->timing;
-    if (timed) std::fill(c->stage_ms, c->stage_ms + 7, 0.0f);
+    if (timed) {
+        // Events are recorded on the stream between the stages and read
+        // after the last realization, so timing adds no host round trips.
+        std::fill(c->stage_ms, c->stage_ms + 7, 0.0f);
+        while (c->evpool.size() < (size_t)8 * n) {
+            hipEvent_t e;
+            CAGEO_HIP_TRY(hipEventCreate(&e));
+            c->evpool.push_back(e);
+        }
+    }
     CAGEO_TRY(cageo::clear_status(c));
     for (int r = 0; r < n; ++r) {
-        if (timed) CAGEO_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        hipEvent_t* evs = timed ? &c->evpool[(size_t)8 * r] : nullptr;
+        if (timed) CAGEO_HIP_TRY(hipEventRecord(evs[0], c->stream));
         // RandomFieldAlgorithm::execute(rand, seed), main.cpp:189-192
         CAGEO_TRY(cageo::launch_random_field(c, c->d_a, seed, first + (uint32_t)r, 0u, c->mean,
                                              c->std_dev));
-        if (timed) CAGEO_HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-        CAGEO_TRY(cageo::realization_body(c, d_mc, timed, 2));
-        if (timed) {
-            CAGEO_HIP_TRY(hipEventSynchronize(c->ev[7]));
+        if (timed) CAGEO_HIP_TRY(hipEventRecord(evs[1], c->stream));
+        CAGEO_TRY(cageo::realization_body(c, d_mc, evs));
+    }
+    const int st = cageo::check_status(c);
+    if (timed && st == CAGEO_OK) {
+        for (int r = 0; r < n; ++r)
             for (int s = 0; s < 7; ++s) {
                 float ms = 0.0f;
-                CAGEO_HIP_TRY(hipEventElapsedTime(&ms, c->ev[s], c->ev[s + 1]));
+                CAGEO_HIP_TRY(hipEventElapsedTime(&ms, c->evpool[(size_t)8 * r + s],
+                                                  c->evpool[(size_t)8 * r + s + 1]));
                 c->stage_ms[s] += ms;
             }
-        }
     }
-    return cageo::check_status(c);
+    return st;
 }
 
 int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc,
@@ -373,7 +411,7 @@ int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc
     CAGEO_HIP_TRY(hipMemcpyAsync(c->d_a, d_noise, c->n * sizeof(float), hipMemcpyDeviceToDevice,
                                  c->stream));
     CAGEO_TRY(cageo::clear_status(c));
-    CAGEO_TRY(cageo::realization_body(c, d_mc, false, 0));
+    CAGEO_TRY(cageo::realization_body(c, d_mc, nullptr));
     CAGEO_TRY(cageo::check_status(c));
     if (d_filled_out)
         CAGEO_HIP_TRY(hipMemcpyAsync(d_filled_out, c->d_w, c->n * sizeof(float),
@@ -385,6 +423,40 @@ int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc
         CAGEO_HIP_TRY(hipMemcpyAsync(d_labels_out, c->d_lab, c->n * sizeof(int32_t),
                                      hipMemcpyDeviceToDevice, c->stream));
     return cageo::sync(c);
+}
+
+int cageo_iteration_counts(cageo_context* ctx, int32_t* out) {
+    if (!ctx || !out) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    uint32_t h[cageo::CTL_WORDS];
+    CAGEO_HIP_TRY(hipMemcpyAsync(h, c->d_ctrl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    out[0] = (int32_t)h[cageo::CTL_ROUNDS];
+    out[1] = (int32_t)h[cageo::CTL_PASSES];
+    out[2] = (int32_t)h[cageo::CTL_FR_PASSES];
+    out[3] = (int32_t)h[cageo::CTL_TR_ROUNDS];
+    return CAGEO_OK;
+}
+
+int cageo_context_configure(cageo_context* ctx, int persistent_workgroups, int conv_rows) {
+    if (!ctx || persistent_workgroups < 0) return CAGEO_ERR_ARG;
+    if (conv_rows != 0 && conv_rows != 4 && conv_rows != 8 && conv_rows != 16) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    int caps[3];
+    CAGEO_TRY(cageo::pit_fill_grid_cap(c, &caps[0]));
+    CAGEO_TRY(cageo::flow_routing_grid_cap(c, &caps[1]));
+    CAGEO_TRY(cageo::flow_tracing_grid_cap(c, &caps[2]));
+    const int cap = std::min(caps[0], std::min(caps[1], caps[2]));
+    if (persistent_workgroups > cap) return CAGEO_ERR_ARG;
+    if (persistent_workgroups == 0) {
+        c->pf_grid = caps[0];
+        c->fr_grid = caps[1];
+        c->tr_grid = caps[2];
+    } else {
+        c->pf_grid = c->fr_grid = c->tr_grid = persistent_workgroups;
+    }
+    c->conv_ty = conv_rows;
+    return CAGEO_OK;
 }
 
 int cageo_stage_times(cageo_context* ctx, float* out, int enable) {

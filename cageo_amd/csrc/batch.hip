@@ -1,0 +1,307 @@
+// Batched Monte Carlo: many DEM tiles of uneven size x many realizations.
+//
+// No reference counterpart: the reference runs one DEM per process
+// (main.cpp:180-283).  Each tile gets its own context (workspace sized to the
+// tile, all resident in HBM), its own error model and filter, and its own
+// Philox key.  Tile t, realization r is the same computation as
+// cageo_run_realizations on that tile alone, so every mode below gives the
+// same counter rasters:
+//   PER_TILE  one blocking run_realizations call per tile (the baseline);
+//   STREAM    every tile x realization enqueued on one stream, one wait;
+//   GRAPH     one realization of every tile captured once as a hipGraph and
+//             replayed n times; the random-field kernels read the
+//             realization index from device memory, which a last node bumps.
+// The autotuner picks, per tile, the workgroup count of the persistent
+// kernels (small tiles have few 64x64 tiles, so a full-chip grid mostly
+// waits at barriers) and the row-block height of the 2-D convolution.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cageo {
+
+namespace {
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+__global__ void k_bump_u32(uint32_t* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += 1u;
+}
+
+}  // namespace
+
+struct BatchTile {
+    cageo_context* ctx = nullptr;
+    uint64_t seed = 0;
+    bool ready = false;
+};
+
+}  // namespace cageo
+
+struct cageo_batch {
+    hipStream_t stream = nullptr;
+    std::vector<cageo::BatchTile> tiles;
+    uint32_t* d_iter = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<float*> graph_mc;  // counter rasters baked into the graph
+};
+
+namespace cageo {
+
+namespace {
+
+void drop_graph(cageo_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->graph_mc.clear();
+}
+
+// One realization of tile t: random field (realization = first + base) and
+// the per-realization stages, accumulated into mc.
+int enqueue_tile(cageo_batch* b, int t, float* mc, uint32_t first, const uint32_t* base) {
+    Context* c = &b->tiles[t].ctx->c;
+    CAGEO_TRY(launch_random_field(c, c->d_a, b->tiles[t].seed, first, 0u, c->mean, c->std_dev,
+                                  base));
+    return realization_body(c, mc, nullptr);
+}
+
+int check_all(cageo_batch* b) {
+    CAGEO_HIP_TRY(hipStreamSynchronize(b->stream));
+    for (auto& t : b->tiles) CAGEO_TRY(check_status(&t.ctx->c));
+    return CAGEO_OK;
+}
+
+// Kernel nodes of `g` that carry the cooperative launch attribute.
+int count_cooperative_nodes(hipGraph_t g, int* count) {
+    size_t n = 0;
+    CAGEO_HIP_TRY(hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    CAGEO_HIP_TRY(hipGraphGetNodes(g, nodes.data(), &n));
+    *count = 0;
+    for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType type;
+        CAGEO_HIP_TRY(hipGraphNodeGetType(node, &type));
+        if (type != hipGraphNodeTypeKernel) continue;
+        hipLaunchAttributeValue v = {};
+        CAGEO_HIP_TRY(hipGraphKernelNodeGetAttribute(node, hipLaunchAttributeCooperative, &v));
+        if (v.cooperative) ++*count;
+    }
+    return CAGEO_OK;
+}
+
+int build_graph(cageo_batch* b, float* const* mc) {
+    drop_graph(b);
+    // workgroup counts are fixed before capture (no device queries inside it)
+    for (auto& t : b->tiles) {
+        Context* c = &t.ctx->c;
+        if (c->pf_grid <= 0) CAGEO_TRY(pit_fill_grid_cap(c, &c->pf_grid));
+        if (c->fr_grid <= 0) CAGEO_TRY(flow_routing_grid_cap(c, &c->fr_grid));
+        if (c->tr_grid <= 0) CAGEO_TRY(flow_tracing_grid_cap(c, &c->tr_grid));
+    }
+    hipGraph_t g = nullptr;
+    CAGEO_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    int st = CAGEO_OK;
+    for (int t = 0; t < (int)b->tiles.size() && st == CAGEO_OK; ++t)
+        st = enqueue_tile(b, t, mc[t], 0u, b->d_iter);
+    if (st == CAGEO_OK) {
+        hipLaunchKernelGGL(k_bump_u32, dim3(1), dim3(64), 0, b->stream, b->d_iter);
+        if (hipGetLastError() != hipSuccess) st = CAGEO_ERR_HIP;
+    }
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st != CAGEO_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return st != CAGEO_OK ? st : CAGEO_ERR_HIP;
+    }
+    // The persistent kernels wait at grid barriers, so each of them (three
+    // per tile) must have been captured as a cooperative launch; a graph
+    // without that guarantee is refused rather than run.
+    int coop = 0;
+    const int cst = count_cooperative_nodes(g, &coop);
+    if (cst != CAGEO_OK || coop != 3 * (int)b->tiles.size()) {
+        (void)hipGraphDestroy(g);
+        return CAGEO_ERR_STATE;
+    }
+    const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        b->exec = nullptr;
+        return CAGEO_ERR_HIP;
+    }
+    b->graph_mc.assign(mc, mc + b->tiles.size());
+    return CAGEO_OK;
+}
+
+// Device time of `reps` realizations of tile t with the current settings.
+int time_tile(cageo_batch* b, int t, float* mc, int reps, float* ms) {
+    Context* c = &b->tiles[t].ctx->c;
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+    CAGEO_HIP_TRY(hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; ++r) CAGEO_TRY(enqueue_tile(b, t, mc, (uint32_t)r, nullptr));
+    CAGEO_HIP_TRY(hipEventRecord(e1, b->stream));
+    CAGEO_HIP_TRY(hipEventSynchronize(e1));
+    CAGEO_TRY(check_status(c));
+    CAGEO_HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+    return CAGEO_OK;
+}
+
+}  // namespace
+
+}  // namespace cageo
+
+using cageo::Context;
+
+extern "C" {
+
+int cageo_batch_create(cageo_batch** out, int ntiles, const int* widths, const int* heights,
+                       void* stream) {
+    if (!out || ntiles <= 0 || !widths || !heights) return CAGEO_ERR_ARG;
+    cageo_batch* b = new cageo_batch();
+    b->stream = reinterpret_cast<hipStream_t>(stream);
+    b->tiles.resize(ntiles);
+    int st = CAGEO_OK;
+    for (int t = 0; t < ntiles && st == CAGEO_OK; ++t)
+        st = cageo_context_create(&b->tiles[t].ctx, widths[t], heights[t], stream);
+    if (st == CAGEO_OK &&
+        hipMalloc(reinterpret_cast<void**>(&b->d_iter), sizeof(uint32_t)) != hipSuccess)
+        st = CAGEO_ERR_NOMEM;
+    if (st != CAGEO_OK) {
+        for (auto& t : b->tiles)
+            if (t.ctx) cageo_context_destroy(t.ctx);
+        delete b;
+        return st;
+    }
+    *out = b;
+    return CAGEO_OK;
+}
+
+int cageo_batch_destroy(cageo_batch* b) {
+    if (!b) return CAGEO_ERR_ARG;
+    (void)hipStreamSynchronize(b->stream);
+    cageo::drop_graph(b);
+    for (auto& t : b->tiles)
+        if (t.ctx) cageo_context_destroy(t.ctx);
+    if (b->d_iter) (void)hipFree(b->d_iter);
+    delete b;
+    return CAGEO_OK;
+}
+
+int cageo_batch_context(cageo_batch* b, int tile, cageo_context** ctx) {
+    if (!b || !ctx || tile < 0 || tile >= (int)b->tiles.size()) return CAGEO_ERR_ARG;
+    *ctx = b->tiles[tile].ctx;
+    return CAGEO_OK;
+}
+
+int cageo_batch_setup_tile(cageo_batch* b, int tile, const float* d_dem, const float* d_stream,
+                           float nodata_dem, float nodata_stream, int bevel_radius, int model,
+                           float practical_range, float cell_size, float mean, float std_dev,
+                           uint64_t seed) {
+    if (!b || tile < 0 || tile >= (int)b->tiles.size()) return CAGEO_ERR_ARG;
+    cageo_context* ctx = b->tiles[tile].ctx;
+    const int r = cageo_filter_radius(practical_range, cell_size);
+    if (r < 0) return CAGEO_ERR_ARG;
+    const int L = 2 * r + 1;
+    std::vector<float> taps(model == CAGEO_FILTER_GAUSSIAN ? (size_t)L : (size_t)L * L);
+    CAGEO_TRY(cageo_create_filter(model, practical_range, cell_size, r, taps.data()));
+    CAGEO_TRY(cageo_set_filter(ctx, model, r, taps.data()));
+    CAGEO_TRY(cageo_mc_setup(ctx, d_dem, d_stream, nodata_dem, nodata_stream, bevel_radius, mean,
+                             std_dev));
+    b->tiles[tile].seed = seed;
+    b->tiles[tile].ready = true;
+    cageo::drop_graph(b);
+    return CAGEO_OK;
+}
+
+int cageo_batch_run(cageo_batch* b, float* const* d_mc, int n, uint32_t first, int mode) {
+    if (!b || !d_mc || n < 0) return CAGEO_ERR_ARG;
+    for (auto& t : b->tiles)
+        if (!t.ready) return CAGEO_ERR_STATE;
+    for (auto& t : b->tiles) CAGEO_TRY(cageo::clear_status(&t.ctx->c));
+    const int T = (int)b->tiles.size();
+    if (mode == CAGEO_BATCH_PER_TILE) {
+        for (int t = 0; t < T; ++t)
+            CAGEO_TRY(cageo_run_realizations(b->tiles[t].ctx, d_mc[t], n, b->tiles[t].seed, first));
+        return CAGEO_OK;
+    }
+    if (mode == CAGEO_BATCH_STREAM) {
+        for (int r = 0; r < n; ++r)
+            for (int t = 0; t < T; ++t)
+                CAGEO_TRY(cageo::enqueue_tile(b, t, d_mc[t], first + (uint32_t)r, nullptr));
+        return cageo::check_all(b);
+    }
+    if (mode != CAGEO_BATCH_GRAPH) return CAGEO_ERR_ARG;
+    bool same = b->exec && b->graph_mc.size() == (size_t)T;
+    for (int t = 0; same && t < T; ++t) same = b->graph_mc[t] == d_mc[t];
+    if (!same) CAGEO_TRY(cageo::build_graph(b, d_mc));
+    hipLaunchKernelGGL(cageo::k_set_u32, dim3(1), dim3(64), 0, b->stream, b->d_iter, first);
+    CAGEO_HIP_TRY(hipGetLastError());
+    for (int r = 0; r < n; ++r) CAGEO_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    return cageo::check_all(b);
+}
+
+int cageo_batch_tune(cageo_batch* b, float* const* d_scratch_mc, int reps, int* out_config) {
+    if (!b || !d_scratch_mc || reps < 1) return CAGEO_ERR_ARG;
+    for (auto& t : b->tiles)
+        if (!t.ready) return CAGEO_ERR_STATE;
+    cageo::drop_graph(b);
+    for (int t = 0; t < (int)b->tiles.size(); ++t) {
+        Context* c = &b->tiles[t].ctx->c;
+        int caps[3];
+        CAGEO_TRY(cageo::pit_fill_grid_cap(c, &caps[0]));
+        CAGEO_TRY(cageo::flow_routing_grid_cap(c, &caps[1]));
+        CAGEO_TRY(cageo::flow_tracing_grid_cap(c, &caps[2]));
+        const int cap = std::min(caps[0], std::min(caps[1], caps[2]));
+        // candidate workgroup counts: around the tile count, and fractions of
+        // the full chip
+        std::vector<int> cand;
+        for (int g : {c->ntiles / 2, c->ntiles, 2 * c->ntiles, cap / 8, cap / 4, cap / 2, cap})
+            if (g >= 1 && g <= cap && std::find(cand.begin(), cand.end(), g) == cand.end())
+                cand.push_back(g);
+        float best = 3.4e38f;
+        int best_g = cap;
+        for (int g : cand) {
+            c->pf_grid = c->fr_grid = c->tr_grid = g;
+            float ms = 0.0f;
+            CAGEO_TRY(cageo::time_tile(b, t, d_scratch_mc[t], 1, &ms));  // warm
+            CAGEO_TRY(cageo::time_tile(b, t, d_scratch_mc[t], reps, &ms));
+            if (ms < best) {
+                best = ms;
+                best_g = g;
+            }
+        }
+        c->pf_grid = c->fr_grid = c->tr_grid = best_g;
+        int best_ty = 0;
+        if (c->model == CAGEO_FILTER_EXPONENTIAL) {
+            float bt = 3.4e38f;
+            for (int ty : {16, 8, 4}) {
+                int ty0, S;
+                size_t lds;
+                CAGEO_TRY(cageo::conv_2d_config(c->radius, &ty0, &S, &lds));
+                if ((size_t)(4 * ty + 2 * c->radius) * S * sizeof(float) > 160 * 1024) continue;
+                c->conv_ty = ty;
+                float ms = 0.0f;
+                CAGEO_TRY(cageo::time_tile(b, t, d_scratch_mc[t], reps, &ms));
+                if (ms < bt) {
+                    bt = ms;
+                    best_ty = ty;
+                }
+            }
+        }
+        c->conv_ty = best_ty;
+        if (out_config) {
+            out_config[2 * t] = best_g;
+            out_config[2 * t + 1] = best_ty;
+        }
+    }
+    return CAGEO_OK;
+}
+
+int cageo_batch_configure_tile(cageo_batch* b, int tile, int persistent_workgroups, int conv_rows) {
+    if (!b || tile < 0 || tile >= (int)b->tiles.size()) return CAGEO_ERR_ARG;
+    CAGEO_TRY(cageo_context_configure(b->tiles[tile].ctx, persistent_workgroups, conv_rows));
+    cageo::drop_graph(b);
+    return CAGEO_OK;
+}
+
+}  // extern "C"

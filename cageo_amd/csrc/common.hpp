@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <vector>
 
 #include "cageo_hip.h"
 
@@ -57,6 +58,7 @@ struct Context {
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
     uint32_t epoch = 0;
+    int conv_ty = 0;  // tuned row-block height of the 2-D convolution (0 = largest that fits)
 
     // scalars
     uint32_t* d_scal = nullptr;
@@ -96,6 +98,7 @@ struct Context {
     // timing
     bool timing = false;
     hipEvent_t ev[8] = {};
+    std::vector<hipEvent_t> evpool;  // 8 per timed realization, recorded without host syncs
     float stage_ms[7] = {};
 };
 
@@ -148,6 +151,8 @@ enum : int {
     CTL_PASSES = 9,
     CTL_FLAG = 10,      // 3 rotating "pending" flags (tracing)
     CTL_EPOCH_FR = 13,  // epoch counter of the flat-resolution worklists
+    CTL_FR_PASSES = 14, // flat-resolution passes of the last launch
+    CTL_TR_ROUNDS = 15, // exit-jump rounds of the last tracing launch
     CTL_WORDS = 16
 };
 
@@ -158,8 +163,13 @@ uint32_t next_epoch(Context* c);
 
 // Stage drivers implemented in the stage translation units -----------------
 int launch_random_field(Context* c, float* out, uint64_t seed, uint32_t realization,
-                        uint32_t stream_id, float mean, float std_dev);
+                        uint32_t stream_id, float mean, float std_dev,
+                        const uint32_t* realization_base = nullptr);
 int launch_convolve(Context* c, float* field, float* buf);
+// Convolution whose result may stay in `buf` (exponential model): returns in
+// *result the buffer that holds it, saving the copy back into `field`.
+int launch_convolve_to(Context* c, float* field, float* buf, float** result);
+int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, int count);
 int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes);
 int launch_realistic(Context* c, const float* dem, float* field, float nodata);
 int launch_burn(Context* c, const float* zin, float* zout, const uint8_t* sflags,
@@ -170,6 +180,12 @@ int launch_stream_flags(Context* c, const float* stream, float nodata_stream, ui
 int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq);
 int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool stats_ready);
 int check_status(Context* c);
+// One realization from the error field in c->d_a (stages a3..a9); with
+// `evs`, events evs[2..7] are recorded after each stage.
+int realization_body(Context* c, float* mc, hipEvent_t* evs);
+int pit_fill_grid_cap(Context* c, int* cap);
+int flow_routing_grid_cap(Context* c, int* cap);
+int flow_tracing_grid_cap(Context* c, int* cap);
 int clear_status(Context* c);
 int launch_quantize(Context* c, float* dem, float nodata);
 int flow_routing(Context* c, const float* z, int8_t* fd);

@@ -209,7 +209,7 @@ int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes) {
     return CAGEO_ERR_ARG;
 }
 
-int launch_convolve(Context* c, float* field, float* buf) {
+int launch_convolve_to(Context* c, float* field, float* buf, float** result) {
     const int W = c->W, H = c->H, r = c->radius;
     if (c->model < 0 || !c->d_filter) return CAGEO_ERR_STATE;
     dim3 grid((W + TILE - 1) / TILE, (H + TILE - 1) / TILE);
@@ -222,11 +222,20 @@ int launch_convolve(Context* c, float* field, float* buf) {
         hipLaunchKernelGGL(k_conv_rows, grid, dim3(256), lds, c->stream, field, buf,
                            c->d_filter, r, W, H);
         CAGEO_HIP_TRY(hipGetLastError());
+        *result = field;
         return CAGEO_OK;
     }
     int ty, S;
     size_t lds;
     CAGEO_TRY(conv_2d_config(r, &ty, &S, &lds));
+    // a tuned row-block height, when it fits the LDS budget
+    if (c->conv_ty == 16 || c->conv_ty == 8 || c->conv_ty == 4) {
+        const size_t b = (size_t)(4 * c->conv_ty + 2 * r) * S * sizeof(float);
+        if (b <= LDS_LIMIT) {
+            ty = c->conv_ty;
+            lds = b;
+        }
+    }
     const int fs = (2 * r + 1 + 3 + 3) & ~3;
     dim3 g2((W + TILE - 1) / TILE, (H + 4 * ty - 1) / (4 * ty));
     switch (ty) {
@@ -244,8 +253,16 @@ int launch_convolve(Context* c, float* field, float* buf) {
             break;
     }
     CAGEO_HIP_TRY(hipGetLastError());
-    CAGEO_HIP_TRY(hipMemcpyAsync(field, buf, c->n * sizeof(float), hipMemcpyDeviceToDevice,
-                                 c->stream));
+    *result = buf;
+    return CAGEO_OK;
+}
+
+int launch_convolve(Context* c, float* field, float* buf) {
+    float* res = nullptr;
+    CAGEO_TRY(launch_convolve_to(c, field, buf, &res));
+    if (res != field)
+        CAGEO_HIP_TRY(hipMemcpyAsync(field, res, c->n * sizeof(float), hipMemcpyDeviceToDevice,
+                                     c->stream));
     return CAGEO_OK;
 }
 

@@ -221,7 +221,8 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
 
     int li = 0;
     bool converged = false;
-    for (int pass = 0; pass < a.max_passes; ++pass) {
+    int pass = 0;
+    for (; pass < a.max_passes; ++pass) {
         const int cin = li, cout = (li + 1) % 3, cclr = (li + 2) % 3;
         const uint32_t count = ld(&a.ctrl[CTL_COUNT + cin]);
         if (count == 0) {
@@ -264,21 +265,28 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         if (sk >= 0)
             reinterpret_cast<char2*>(a.fd)[i] = make_char2((int8_t)FDX[sk], (int8_t)FDY[sk]);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_EPOCH_FR] = epoch;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl[CTL_EPOCH_FR] = epoch;
+        a.ctrl[CTL_FR_PASSES] = (uint32_t)pass;  // read by the host only
+    }
 }
 
 }  // namespace
 
+// Co-resident workgroups of the persistent kernel on this device.
+int flow_routing_grid_cap(Context* c, int* cap) {
+    int per_cu = 0;
+    CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_fr_persistent), TILE_THREADS, 0));
+    hipDeviceProp_t prop;
+    CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+    if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+    *cap = per_cu * prop.multiProcessorCount;
+    return CAGEO_OK;
+}
+
 int flow_routing(Context* c, const float* z, int8_t* fd) {
-    if (c->fr_grid <= 0) {
-        int per_cu = 0;
-        CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(k_fr_persistent), TILE_THREADS, 0));
-        hipDeviceProp_t prop;
-        CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-        if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
-        c->fr_grid = per_cu * prop.multiProcessorCount;
-    }
+    if (c->fr_grid <= 0) CAGEO_TRY(flow_routing_grid_cap(c, &c->fr_grid));
     FrArgs a;
     a.z = z;
     a.fd = fd;

@@ -165,7 +165,9 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
 
     const size_t n = (size_t)a.W * a.H;
     int cur = 0;
+    int rounds = 0;
     for (int r = 0; r < a.max_rounds; ++r) {
+        rounds = r + 1;
         const uint64_t* src = a.ent[cur];
         uint64_t* dst = a.ent[cur ^ 1];
         uint32_t pending = 0;
@@ -199,6 +201,7 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
     }
     for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
     if ((threadIdx.x & 63) == 0 && longp) atomicAdd(&a.scal[SC_LONGPATH], longp);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
     grid_barrier();
     // performConnectFlowPath: label[0] is final and never changes here.
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -215,17 +218,21 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
 
 // Needs two 64-bit rasters: c->d_ptr and c->d_ptr2.  The long-path count is
 // left in the scalar block.
+// Co-resident workgroups of the persistent kernel on this device.
+int flow_tracing_grid_cap(Context* c, int* cap) {
+    int per_cu = 0;
+    CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_tr_persistent), TILE_THREADS, 0));
+    hipDeviceProp_t prop;
+    CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+    if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+    *cap = per_cu * prop.multiProcessorCount;
+    return CAGEO_OK;
+}
+
 int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,
                  int32_t* lab, int64_t* long_paths) {
-    if (c->tr_grid <= 0) {
-        int per_cu = 0;
-        CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(k_tr_persistent), TILE_THREADS, 0));
-        hipDeviceProp_t prop;
-        CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-        if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
-        c->tr_grid = per_cu * prop.multiProcessorCount;
-    }
+    if (c->tr_grid <= 0) CAGEO_TRY(flow_tracing_grid_cap(c, &c->tr_grid));
     TrArgs a;
     a.fd = fd;
     a.sf = sflags;

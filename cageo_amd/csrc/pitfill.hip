@@ -390,6 +390,7 @@ __global__ __launch_bounds__(256) void k_pf_persistent(PfArgs a) {
         a.ctrl[CTL_CERT + 0] = 0;
         a.ctrl[CTL_CERT + 1] = 0;
         a.ctrl[CTL_CERT + 2] = 0;
+        a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
     grid_barrier();
@@ -458,25 +459,25 @@ __global__ __launch_bounds__(256) void k_pf_persistent(PfArgs a) {
 
 }  // namespace
 
-int pit_fill_persistent_grid(Context* c, int* grid) {
-    if (c->pf_grid > 0) {
-        *grid = c->pf_grid;
-        return CAGEO_OK;
-    }
+// Co-resident workgroups of the persistent kernel on this device: the most a
+// cooperative launch may use.
+int pit_fill_grid_cap(Context* c, int* cap) {
     int per_cu = 0;
     CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, reinterpret_cast<const void*>(k_pf_persistent), TILE_THREADS, 0));
     hipDeviceProp_t prop;
     CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
     if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
-    c->pf_grid = per_cu * prop.multiProcessorCount;
+    *cap = per_cu * prop.multiProcessorCount;
+    return CAGEO_OK;
+}
+
+int pit_fill_persistent_grid(Context* c, int* grid) {
+    if (c->pf_grid <= 0) CAGEO_TRY(pit_fill_grid_cap(c, &c->pf_grid));
     *grid = c->pf_grid;
     return CAGEO_OK;
 }
 
-// Pit fill of an already quantized raster whose min/max/bad scalars are in
-// the scalar block (written by the quantisation kernel).  Enqueues one
-// cooperative launch; the status word is checked by check_status().
 int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*stats_ready*/) {
     int grid = 0;
     CAGEO_TRY(pit_fill_persistent_grid(c, &grid));

@@ -155,6 +155,55 @@ int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc
  * burn, pit fill, flow routing, flow tracing, border.  `out` holds 7 floats. */
 int cageo_stage_times(cageo_context* ctx, float* out, int enable);
 
+/* ---- batched form: many tiles of uneven size x many realizations ---------
+ * No reference counterpart (the reference runs one DEM per process).  Each
+ * tile has its own context, error model, filter and Philox key; tile t,
+ * realization r computes exactly what cageo_run_realizations does on that
+ * tile alone, in every mode:
+ *   CAGEO_BATCH_PER_TILE  one blocking cageo_run_realizations per tile;
+ *   CAGEO_BATCH_STREAM    all tiles x realizations on one stream, one wait;
+ *   CAGEO_BATCH_GRAPH     one realization of all tiles captured as a hipGraph
+ *                         (cooperative kernel nodes) and replayed n times.
+ * d_mc is an array of ntiles device pointers (one counter raster per tile). */
+#define CAGEO_BATCH_PER_TILE 0
+#define CAGEO_BATCH_STREAM 1
+#define CAGEO_BATCH_GRAPH 2
+typedef struct cageo_batch cageo_batch;
+int cageo_batch_create(cageo_batch** out, int ntiles, const int* widths, const int* heights,
+                       void* hip_stream);
+int cageo_batch_destroy(cageo_batch* batch);
+/* The per-tile context (for stage calls on one tile). */
+int cageo_batch_context(cageo_batch* batch, int tile, cageo_context** ctx);
+/* Inputs and error model of one tile; the filter is built from
+ * (model, practical_range, cell_size) as main.cpp:110-136 does. */
+int cageo_batch_setup_tile(cageo_batch* batch, int tile, const float* d_dem, const float* d_stream,
+                           float nodata_dem, float nodata_stream, int bevel_radius, int model,
+                           float practical_range, float cell_size, float mean, float std_dev,
+                           uint64_t seed);
+int cageo_batch_run(cageo_batch* batch, float* const* d_mc, int n, uint32_t first_realization,
+                    int mode);
+/* Autotuner: per tile, times `reps` realizations (accumulated into the
+ * scratch counters d_scratch_mc) for candidate workgroup counts of the
+ * persistent kernels and row-block heights of the 2-D convolution, and keeps
+ * the fastest.  out_config (optional, 2 ints per tile) receives them.
+ * Results do not depend on these settings. */
+int cageo_batch_tune(cageo_batch* batch, float* const* d_scratch_mc, int reps, int* out_config);
+/* Set one tile's settings by hand: persistent_workgroups (0 = full chip) and
+ * conv_rows in {0 (auto), 4, 8, 16}. */
+int cageo_batch_configure_tile(cageo_batch* batch, int tile, int persistent_workgroups,
+                               int conv_rows);
+
+/* Iteration counts of the last fixed-point launches on this context: pit
+ * fill rounds, pit fill passes, flat-resolution passes, tracing exit-jump
+ * rounds (the reference's PitFillingAlgorithm::iter-style counters).
+ * `out` holds 4 ints. */
+int cageo_iteration_counts(cageo_context* ctx, int32_t* out);
+/* Launch settings of this context: persistent_workgroups for the persistent
+ * kernels (0 = every co-resident workgroup of the device) and conv_rows, the
+ * row-block height of the 2-D convolution (0 = largest that fits, 4, 8, 16).
+ * Results do not depend on them. */
+int cageo_context_configure(cageo_context* ctx, int persistent_workgroups, int conv_rows);
+
 /* ---- synthetic inputs (host) -----------------------------------------------
  * Seeded DEM: 100 m base + 0.5 % tilt along +x + 40 Gaussian hills
  * (height U(5,60) m, sigma U(50,400) m) + N(0, 0.3^2) noise; streams where

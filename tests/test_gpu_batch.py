@@ -1,0 +1,103 @@
+"""Batched form on the GPU: tiles of uneven size x realizations.
+
+Every batch mode (per-tile blocking calls, one stream, a replayed hipGraph)
+and any autotuned setting must give the counter rasters that each tile gets
+from cageo_run_realizations alone, which in turn equal the oracle's Monte
+Carlo loop.  Each case prints its name before it touches the GPU.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import NODATA, synth_case
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(96, 200), (257, 130), (64, 64), (190, 333), (128, 80)]
+PARAMS = [  # (model, practical range, error std)
+    (1, 60.0, 1.8), (0, 200.0, 2.5), (1, 30.0, 0.5), (1, 200.0, 3.0), (0, 60.0, 1.0)]
+
+
+@pytest.fixture(scope="module")
+def tiles(torch_cuda):
+    torch = torch_cuda
+    out = []
+    for k, (h, w) in enumerate(SHAPES):
+        dem, st = synth_case(max(h, w), seed=40 + k)
+        dem, st = np.ascontiguousarray(dem[:h, :w]), np.ascontiguousarray(st[:h, :w])
+        out.append((dem, st, torch.from_numpy(dem).cuda(), torch.from_numpy(st).cuda()))
+    return out
+
+
+def announce(*what):
+    print("CASE", *what, flush=True)
+
+
+def make_batch(ca, tiles):
+    b = ca.Batch(SHAPES)
+    for t, ((model, pr, std), (_, _, ddem, dst)) in enumerate(zip(PARAMS, tiles)):
+        # radius must fit the tile: PR 200 m -> r = 50 <= 64
+        b.setup_tile(t, ddem, dst, float(NODATA), float(NODATA), 2, model, pr, 10.0, 0.0, std,
+                     1000 + t)
+    return b
+
+
+def zeros(torch):
+    return [torch.zeros(s, dtype=torch.float32, device="cuda") for s in SHAPES]
+
+
+def test_batch_modes_agree_with_single_tiles_and_oracle(torch_cuda, tiles):
+    import cageo_amd as ca
+    torch = torch_cuda
+    n = 3
+    announce("batch per-tile/stream/graph", SHAPES)
+    with make_batch(ca, tiles) as b:
+        results = {}
+        for mode in (ca.BATCH_PER_TILE, ca.BATCH_STREAM, ca.BATCH_GRAPH):
+            mcs = zeros(torch)
+            b.run(mcs, n, first=5, mode=mode)
+            results[mode] = [m.cpu().numpy() for m in mcs]
+        # a second graph replay series continues the realization index
+        mcs = zeros(torch)
+        b.run(mcs, 1, first=5, mode=ca.BATCH_GRAPH)
+        b.run(mcs, 2, first=6, mode=ca.BATCH_GRAPH)
+        results["graph_split"] = [m.cpu().numpy() for m in mcs]
+    for t, ((model, pr, std), (dem, st, ddem, dst)) in enumerate(zip(PARAMS, tiles)):
+        h, w = SHAPES[t]
+        r = ca.filter_radius(pr, 10.0)
+        taps = ca.create_filter(model, pr, 10.0, r)
+        with ca.Context(w, h) as ctx:
+            ctx.set_filter(model, r, taps)
+            ctx.mc_setup(ddem, dst, float(NODATA), float(NODATA), 2, 0.0, std)
+            mc = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+            ctx.run_realizations(mc, n, 1000 + t, 5)
+            single = mc.cpu().numpy()
+        ref = np.zeros((h, w), np.float32)
+        for k in range(n):
+            oracle.lib().or_realization(dem, st, w, h, taps, r, model, 0.0, std, 1000 + t, 5 + k,
+                                        float(NODATA), float(NODATA), 2, ref)
+        assert np.array_equal(single, ref), f"tile {t}"
+        for mode, res in results.items():
+            assert np.array_equal(res[t], single), f"tile {t} mode {mode}"
+
+
+def test_batch_tuned_settings_do_not_change_results(torch_cuda, tiles):
+    import cageo_amd as ca
+    torch = torch_cuda
+    announce("batch tune")
+    with make_batch(ca, tiles) as b:
+        base = zeros(torch)
+        b.run(base, 2, first=0, mode=ca.BATCH_STREAM)
+        cfg = b.tune(zeros(torch), reps=1)
+        assert len(cfg) == len(SHAPES) and all(g >= 1 for g, _ in cfg)
+        tuned = zeros(torch)
+        b.run(tuned, 2, first=0, mode=ca.BATCH_GRAPH)
+        for a, c in zip(base, tuned):
+            assert torch.equal(a, c)
+        # hand-set extremes: a single workgroup and the full chip
+        for t in range(len(SHAPES)):
+            b.configure_tile(t, persistent_workgroups=1 if t % 2 else 0, conv_rows=4 if t == 1 else 0)
+        other = zeros(torch)
+        b.run(other, 2, first=0, mode=ca.BATCH_STREAM)
+        for a, c in zip(base, other):
+            assert torch.equal(a, c)
