@@ -56,6 +56,7 @@ struct Context {
     uint32_t* d_mark_pf = nullptr;  // epoch stamps of the pit-fill kernel
     int* d_work3 = nullptr;         // 3 worklists for the persistent kernels
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
+    uint32_t* d_bar = nullptr;      // grid barrier words (BAR_WORDS), see grid_sync
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
     uint32_t epoch = 0;
     int conv_ty = 0;  // tuned row-block height of the 2-D convolution (0 = largest that fits)
@@ -139,6 +140,47 @@ __host__ __device__ inline float pf_to_elev(uint64_t index, float min_elev) {
 // flood only if v survives the bucket round trip.
 __host__ __device__ inline bool pf_live(float v, float min_elev) {
     return pf_to_elev(pf_to_index(v, min_elev), min_elev) == v;
+}
+
+// Grid-wide barrier of the persistent kernels (all workgroups co-resident:
+// they are launched cooperatively with at most the occupancy limit).
+//
+// Cost model on a multi-XCD part: each XCD has its own L2, so making data
+// written before the barrier visible to every workgroup after it takes an
+// agent-scope release (L2 write-back) and acquire (L2 + L1 invalidate).
+// Those are done once per workgroup, by thread 0, around a relaxed arrival
+// counter and a relaxed poll of a generation word:
+//  - __syncthreads() first: it waits for every wave's stores to reach L2,
+//    so thread 0's release writes them back as well;
+//  - thread 0's acquire invalidates the CU's L1, which all waves of the
+//    workgroup share, and the closing __syncthreads() orders their loads
+//    after it.
+// Polling with acquire loads, or fencing in every thread, costs an L2
+// invalidate per poll or per wave and made the barrier several times slower.
+constexpr int BAR_STRIDE = 32;  // words: one 128-byte line per word
+constexpr int BAR_COUNT = 0;
+constexpr int BAR_GEN = BAR_STRIDE;
+constexpr int BAR_WORDS = 2 * BAR_STRIDE;
+
+__device__ inline void grid_sync(uint32_t* bar) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t* cnt = &bar[BAR_COUNT];
+        uint32_t* gen = &bar[BAR_GEN];
+        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (__hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            gridDim.x - 1) {
+            // last arrival: reset the counter, then open the next generation
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g)
+            __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
 }
 
 // Words of the control block (d_ctrl), shared by the persistent kernels.

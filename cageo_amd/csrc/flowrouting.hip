@@ -17,8 +17,6 @@
 // z_n <= z_c.  d is computed by monotone min-relaxation over 64x64 LDS
 // tiles (dirty-bit sweeps) inside one persistent cooperative kernel, with the
 // same worklist protocol as the pit fill.
-#include <hip/hip_cooperative_groups.h>
-
 #include "tiles.hpp"
 
 namespace cageo {
@@ -34,10 +32,37 @@ __device__ constexpr int FDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 
 struct FrShared {
     int2 dz[HALO_W * HALO_W];  // x: distance, y: bits of z
-    unsigned long long dirty[2][TILE];
+    uint32_t rowdirty[TILE];   // rows due for another exact row solve
     uint32_t bits;
     uint32_t any;
 };
+
+// Min-plus map v -> min(a, v + k) on distances saturated at D_INF (k = D_INF
+// means the map ignores v).  Replaces (a, k) by "self after prev".
+__device__ inline int sat_add(int x, int y) {
+    const int s = x + y;  // both <= D_INF < 2^30: no overflow
+    return s < D_INF ? s : D_INF;
+}
+__device__ inline void minplus_after(int& a, int& k, int pa, int pk) {
+    a = min(a, sat_add(pa, k));
+    k = sat_add(pk, k);
+}
+__device__ inline void scan_minplus_up(int& a, int& k) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int pa = __shfl_up(a, d, 64), pk = __shfl_up(k, d, 64);
+        if (lane >= d) minplus_after(a, k, pa, pk);
+    }
+}
+__device__ inline void scan_minplus_down(int& a, int& k) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int pa = __shfl_down(a, d, 64), pk = __shfl_down(k, d, 64);
+        if (lane + d < 64) minplus_after(a, k, pa, pk);
+    }
+}
 
 struct FrArgs {
     const float* z;
@@ -47,25 +72,13 @@ struct FrArgs {
     int* work;
     uint32_t* mark;
     uint32_t* ctrl;
+    uint32_t* bar;  // grid barrier words
     int max_passes;
 };
 
-__device__ inline void grid_barrier() {
-    __threadfence();
-    cooperative_groups::this_grid().sync();
-    __threadfence();
-}
 
 __device__ inline uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline void mark_dirty(unsigned long long* d, int lx, int ly) {
-    const unsigned long long m =
-        (lx == 0) ? 3ull : (lx == TILE - 1 ? (3ull << (TILE - 2)) : (7ull << (lx - 1)));
-    if (ly > 0) atomicOr(&d[ly - 1], m);
-    atomicOr(&d[ly], m & ~(1ull << lx));
-    if (ly < TILE - 1) atomicOr(&d[ly + 1], m);
 }
 
 // D8 of one cell (FlowRoutingAlgorithmCPUOpenMP.h:60-150); returns the
@@ -129,10 +142,7 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& a, int t, int* work_out, uin
         }
         S.dz[k] = v;
     }
-    if (threadIdx.x < TILE) {
-        S.dirty[0][threadIdx.x] = ~0ull;
-        S.dirty[1][threadIdx.x] = 0ull;
-    }
+    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 1u;
     if (threadIdx.x == 0) S.bits = 0;
     __syncthreads();
 
@@ -143,33 +153,58 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& a, int t, int* work_out, uin
         const int li = (strip * STRIP + k + 1) * HALO_W + lx + 1;
         if (S.dz[li].x >= 1) flat |= 1u << k;
     }
+    // Row step: a flat cell takes d = 1 + min d(n) over eligible neighbours
+    // (d(n) >= 0, z_n <= z_c).  Rows above and below and the halo cells
+    // enter as a constant; along the row each flat cell is the map
+    // v -> min(own, v + 1) when the row neighbour it reads is not higher,
+    // cells with a direction relay their 0, stream and outside cells relay
+    // nothing.  Scans from both sides give the row's exact fixed point.
     uint32_t changed_cells = 0;
     for (int sweep = 0;; ++sweep) {
-        unsigned long long* cur = S.dirty[sweep & 1];
-        unsigned long long* nxt = S.dirty[(sweep + 1) & 1];
         int changed = 0;
 #pragma unroll
         for (int kk = 0; kk < STRIP; ++kk) {
             const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
-            const unsigned long long bit = 1ull << lx;
-            if (!(cur[ly] & bit)) continue;
-            atomicAnd(&cur[ly], ~bit);
-            if (!(flat & (1u << k))) continue;
+            if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
+            S.rowdirty[ly] = 0u;
             const int li = (ly + 1) * HALO_W + lx + 1;
             const int2 c = S.dz[li];
             const float zc = __int_as_float(c.y);
-            int best = c.x;
+            const bool isflat = (flat >> k) & 1u;
+            const float zl = __shfl_up(zc, 1, 64), zrt = __shfl_down(zc, 1, 64);
+            int a, kl = D_INF, kr = D_INF;
+            if (isflat) {
+                a = c.x;
 #pragma unroll
-            for (int d = 0; d < 8; ++d) {
-                const int2 nv = S.dz[li + FDY[d] * HALO_W + FDX[d]];
-                if (nv.x >= 0 && __int_as_float(nv.y) <= zc && nv.x + 1 < best) best = nv.x + 1;
+                for (int d = 0; d < 8; ++d) {
+                    // rows above and below; the halo columns for the end lanes
+                    const bool in_row = FDY[d] == 0;
+                    if (in_row && !((lx == 0 && FDX[d] < 0) || (lx == TILE - 1 && FDX[d] > 0)))
+                        continue;
+                    const int2 nv = S.dz[li + FDY[d] * HALO_W + FDX[d]];
+                    if (nv.x >= 0 && __int_as_float(nv.y) <= zc) a = min(a, nv.x + 1);
+                }
+                if (lx > 0 && zl <= zc) kl = 1;
+                if (lx < TILE - 1 && zrt <= zc) kr = 1;
+            } else {
+                a = c.x >= 0 ? c.x : D_INF;  // a direction relays 0
             }
-            if (best < c.x) {
-                S.dz[li].x = best;
+            int al = a, ar = a;
+            scan_minplus_up(al, kl);
+            scan_minplus_down(ar, kr);
+            bool ch = false;
+            if (isflat) {
+                const int nd = min(al, ar);
+                if (nd < c.x) {
+                    S.dz[li].x = nd;
+                    changed_cells |= 1u << k;
+                    ch = true;
+                }
+            }
+            if (__ballot(ch)) {
                 changed = 1;
-                changed_cells |= 1u << k;
-                mark_dirty(nxt, lx, ly);
+                mark_rows(S.rowdirty, ly, false);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -199,7 +234,7 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         a.ctrl[CTL_COUNT + 1] = 0;
         a.ctrl[CTL_COUNT + 2] = 0;
     }
-    grid_barrier();
+    grid_sync(a.bar);
 
     // D8 tile by tile; tiles holding flats open list 0.
     for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
@@ -217,7 +252,7 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         if (__syncthreads_or(has_flat) && threadIdx.x == 0)
             a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = t;
     }
-    grid_barrier();
+    grid_sync(a.bar);
 
     int li = 0;
     bool converged = false;
@@ -235,7 +270,7 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         int* wout = a.work + (size_t)cout * a.ntiles;
         for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
             bfs_tile(S, a, win[i], wout, &a.ctrl[CTL_COUNT + cout], epoch);
-        grid_barrier();
+        grid_sync(a.bar);
         li = cout;
     }
     // Nothing below writes a control word that the pass loop reads, so no
@@ -299,6 +334,7 @@ int flow_routing(Context* c, const float* z, int8_t* fd) {
     a.work = c->d_work3;
     a.mark = c->d_mark;
     a.ctrl = c->d_ctrl;
+    a.bar = c->d_bar;
     a.max_passes = 1 << 20;
     void* args[] = {&a};
     CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_fr_persistent),

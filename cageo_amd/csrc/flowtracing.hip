@@ -19,8 +19,6 @@
 // than 100 001 steps away (where the reference's result depends on its
 // raster order) are counted.  performConnectFlowPath (:115-134) then maps a
 // label >= 0 to label[label] when that is negative.
-#include <hip/hip_cooperative_groups.h>
-
 #include "common.hpp"
 
 namespace cageo {
@@ -125,11 +123,6 @@ __device__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
     __syncthreads();
 }
 
-__device__ inline void grid_barrier() {
-    __threadfence();
-    cooperative_groups::this_grid().sync();
-    __threadfence();
-}
 
 __device__ inline uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -144,6 +137,7 @@ struct TrArgs {
     int W, H, ntx, ntiles;
     int max_rounds;
     uint32_t* ctrl;
+    uint32_t* bar;  // grid barrier words
     uint32_t* scal;
 };
 
@@ -161,7 +155,7 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
     }
     for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x)
         trace_tile(L, a.fd, a.sf, a.slabel, a.ent[0], a.W, a.H, a.ntx, t);
-    grid_barrier();
+    grid_sync(a.bar);
 
     const size_t n = (size_t)a.W * a.H;
     int cur = 0;
@@ -182,7 +176,7 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         }
         if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&a.ctrl[CTL_FLAG + r % 3], 1u);
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_FLAG + (r + 1) % 3] = 0;
-        grid_barrier();
+        grid_sync(a.bar);
         cur ^= 1;
         if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
     }
@@ -202,7 +196,7 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
     for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
     if ((threadIdx.x & 63) == 0 && longp) atomicAdd(&a.scal[SC_LONGPATH], longp);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
-    grid_barrier();
+    grid_sync(a.bar);
     // performConnectFlowPath: label[0] is final and never changes here.
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
@@ -248,6 +242,7 @@ int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int3
     for (size_t m = 1; m < c->n; m <<= 1) ++max_rounds;
     a.max_rounds = max_rounds;
     a.ctrl = c->d_ctrl;
+    a.bar = c->d_bar;
     a.scal = c->d_scal;
     void* args[] = {&a};
     CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tr_persistent),

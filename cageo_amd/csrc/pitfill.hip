@@ -36,8 +36,6 @@
 //
 // Every comparison is on floats; the only arithmetic is the quantisation and
 // the round-trip test, both evaluated exactly as the reference does.
-#include <hip/hip_cooperative_groups.h>
-
 #include "tiles.hpp"
 
 namespace cageo {
@@ -48,7 +46,8 @@ constexpr float PF_INF = __builtin_huge_valf();
 
 struct PfShared {
     float2 rel[HALO_W * HALO_W];  // x: relay (OPEN or live FIXED), y: live FIXED only
-    unsigned long long dirty[2][TILE];
+    float zt[TILE * TILE];        // quantized elevation Z of the tile's OPEN cells
+    uint32_t rowdirty[TILE];      // rows due for another exact row solve
     uint32_t bits;
     uint32_t any;
 };
@@ -60,6 +59,30 @@ __device__ inline float min8x(const float2* s, int i) {
     float d = fminf(s[i + HALO_W].x, s[i + HALO_W + 1].x);
     return fminf(fminf(a, b), fminf(c, d));
 }
+// Cells of a row reached from the `seed` lanes by steps x-1 -> x allowed
+// by `gate_up` and x+1 -> x allowed by `gate_down` (64-bit lane masks; the
+// reach doubles each step, so 6 steps cover the row).
+__device__ inline unsigned long long spread_row(unsigned long long seed, unsigned long long gate_up,
+                                                unsigned long long gate_down) {
+    unsigned long long fr = seed, fl = seed;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        fr |= (fr << d) & gate_up;
+        gate_up &= gate_up << d;
+        fl |= (fl >> d) & gate_down;
+        gate_down &= gate_down >> d;
+    }
+    return fr | fl;
+}
+
+// Relays of the six cells in the rows above and below (the row neighbours
+// enter through the row scan).
+__device__ inline float min6x(const float2* s, int i) {
+    float a = fminf(s[i - HALO_W - 1].x, s[i - HALO_W].x);
+    float b = fminf(s[i - HALO_W + 1].x, s[i + HALO_W - 1].x);
+    float c = fminf(s[i + HALO_W].x, s[i + HALO_W + 1].x);
+    return fminf(fminf(a, b), c);
+}
 __device__ inline float min8y(const float2* s, int i) {
     float a = fminf(s[i - HALO_W - 1].y, s[i - HALO_W].y);
     float b = fminf(s[i - HALO_W + 1].y, s[i - 1].y);
@@ -68,20 +91,6 @@ __device__ inline float min8y(const float2* s, int i) {
     return fminf(fminf(a, b), fminf(c, d));
 }
 
-// Marks the in-tile 8-neighbours of (lx, ly) dirty in `d`.
-__device__ inline void mark_dirty(unsigned long long* d, int lx, int ly) {
-    const unsigned long long m =
-        (lx == 0) ? 3ull : (lx == TILE - 1 ? (3ull << (TILE - 2)) : (7ull << (lx - 1)));
-    if (ly > 0) atomicOr(&d[ly - 1], m);
-    atomicOr(&d[ly], m & ~(1ull << lx));
-    if (ly < TILE - 1) atomicOr(&d[ly + 1], m);
-}
-
-__device__ inline void grid_barrier() {
-    __threadfence();
-    cooperative_groups::this_grid().sync();
-    __threadfence();
-}
 
 __device__ inline uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -97,6 +106,7 @@ struct PfArgs {
     uint32_t* mark;       // per-tile epoch stamps
     uint32_t* topen;      // per-tile: tile holds OPEN cells
     uint32_t* ctrl;       // control block (CTL_*)
+    uint32_t* bar;        // grid barrier words
     const uint32_t* scal; // min/max/bad of the quantized raster
     int max_passes;
 };
@@ -114,10 +124,7 @@ __device__ inline void load_tile(PfShared& S, const PfArgs& a, int x0, int y0) {
         }
         S.rel[k] = make_float2(r, cr);
     }
-    if (threadIdx.x < TILE) {
-        S.dirty[0][threadIdx.x] = ~0ull;
-        S.dirty[1][threadIdx.x] = 0ull;
-    }
+    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 1u;
     if (threadIdx.x == 0) {
         S.bits = 0;
         S.any = 0;
@@ -148,71 +155,102 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, int t, uint32_t
     __syncthreads();
 }
 
-// Fill relaxation of one tile (with immediate fixing at Z).
+// Fill relaxation of one tile (with immediate fixing at Z).  Cell state
+// lives in LDS (relays in rel, Z in zt); each thread keeps bit masks of its
+// strip's cells only, so the row loop can run in either direction without
+// per-thread arrays.
 __device__ void fill_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_t* cert_ctr,
                           int* work_out, uint32_t* count_out, uint32_t epoch) {
     const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
     load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
-    float zr[STRIP], wr[STRIP];
     uint32_t open = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        const int gy = y0 + strip * STRIP + k;
-        zr[k] = 0.0f;
-        wr[k] = PF_INF;
+        const int ly = strip * STRIP + k, gy = y0 + ly;
+        float z = 0.0f;
         if (gx < a.W && gy < a.H) {
             const size_t gi = (size_t)gy * a.W + gx;
             if (a.st[gi] == PF_OPEN) {
                 open |= 1u << k;
-                zr[k] = a.zq[gi];
-                wr[k] = a.w[gi];
+                z = a.zq[gi];
             }
         }
+        S.zt[ly * TILE + lx] = z;
     }
     __syncthreads();
 
+    // Row step (row ly, all 64 columns at once):
+    //  1. Z-rule: an OPEN cell with a live FIXED neighbour no higher than Z
+    //     is fixed at Z and, if Z is live, relays Z; along the row this
+    //     spreads from a cell so fixed to an OPEN neighbour whose Z is not
+    //     lower (segmented propagation over the lane masks);
+    //  2. every other OPEN cell is the map v -> max(Z, min(c, v)), c = min(its
+    //     value, the relays above and below); FIXED cells relay a constant
+    //     (their value if live, nothing if dead).  The fixed point of the row,
+    //     W(x) = max(Z, min(c, W(x-1), W(x+1))), is min(L(x), R(x)) with L, R
+    //     the map chains from the left and right halo cells.
+    const int hl = 0, hr = HALO_W - 1;  // halo columns
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
     for (int sweep = 0;; ++sweep) {
-        unsigned long long* cur = S.dirty[sweep & 1];
-        unsigned long long* nxt = S.dirty[(sweep + 1) & 1];
         int changed = 0;
-#pragma unroll
         for (int kk = 0; kk < STRIP; ++kk) {
             const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
-            const unsigned long long bit = 1ull << lx;
-            if (!(cur[ly] & bit)) continue;
-            atomicAnd(&cur[ly], ~bit);
-            if (!(open & (1u << k))) continue;
+            if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
+            S.rowdirty[ly] = 0u;
             const int li = (ly + 1) * HALO_W + lx + 1;
+            const uint32_t bit = 1u << k;
+            const bool op = (open & bit) != 0;
+            const float z = S.zt[ly * TILE + lx];
+            const float wcur = S.rel[li].x;
+            const bool zlive = op && pf_live(z, mn);
+            const bool seed = op && min8y(S.rel, li) <= z;
+            const float zl = __shfl_up(z, 1, 64), zrt = __shfl_down(z, 1, 64);
+            const int livel = __shfl_up((int)zlive, 1, 64), liver = __shfl_down((int)zlive, 1, 64);
+            const unsigned long long sd = __ballot(seed);
+            unsigned long long f = sd;
+            if (sd)
+                f = spread_row(sd, __ballot(op && lx > 0 && livel && zl <= z),
+                               __ballot(op && lx < TILE - 1 && liver && zrt <= z));
             bool ch = false;
-            if (min8y(S.rel, li) <= zr[k]) {
-                // a live fixed neighbour no higher than Z: P = Z
-                open &= ~(1u << k);
-                fixed |= 1u << k;
-                if (wr[k] != zr[k]) lowered |= 1u << k;
-                wr[k] = zr[k];
-                if (pf_live(zr[k], mn)) {
-                    fixed_live |= 1u << k;
-                    S.rel[li] = make_float2(zr[k], zr[k]);
+            float lo, hi;
+            if ((f >> lx) & 1ull) {
+                open &= ~bit;
+                fixed |= bit;
+                if (wcur != z) lowered |= bit;
+                if (zlive) {
+                    fixed_live |= bit;
+                    S.rel[li] = make_float2(z, z);
+                    lo = hi = z;
                 } else {
                     S.rel[li] = make_float2(PF_INF, PF_INF);
+                    lo = hi = PF_INF;
                 }
                 ch = true;
+            } else if (op) {
+                lo = z;
+                hi = fmaxf(z, fminf(wcur, min6x(S.rel, li)));
             } else {
-                const float nw = fmaxf(zr[k], fminf(wr[k], min8x(S.rel, li)));
-                if (nw < wr[k]) {
-                    wr[k] = nw;
+                lo = hi = S.rel[li].y;  // live FIXED: its value; otherwise no relay
+            }
+            float llo = lo, lhi = hi, rlo = lo, rhi = hi;
+            scan_clamp_up(llo, lhi);
+            scan_clamp_down(rlo, rhi);
+            if (open & bit) {
+                const float L = fminf(fmaxf(S.rel[(ly + 1) * HALO_W + hl].x, llo), lhi);
+                const float R = fminf(fmaxf(S.rel[(ly + 1) * HALO_W + hr].x, rlo), rhi);
+                const float nw = fminf(L, R);
+                if (nw < wcur) {
                     S.rel[li].x = nw;
-                    lowered |= 1u << k;
+                    lowered |= bit;
                     ch = true;
                 }
             }
-            if (ch) {
+            if (__ballot(ch)) {
                 changed = 1;
-                mark_dirty(nxt, lx, ly);
+                mark_rows(S.rowdirty, ly, false);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -223,7 +261,9 @@ __device__ void fill_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
     for (int k = 0; k < STRIP; ++k) {
         const int ly = strip * STRIP + k;
         const size_t gi = (size_t)(y0 + ly) * a.W + gx;
-        if (lowered & (1u << k)) a.w[gi] = wr[k];
+        if (lowered & (1u << k))
+            a.w[gi] = (fixed & (1u << k)) ? S.zt[ly * TILE + lx]
+                                          : S.rel[(ly + 1) * HALO_W + lx + 1].x;
         if (fixed & (1u << k)) {
             a.st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
             ++nfixed;
@@ -241,45 +281,58 @@ __device__ void cert_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
     uint32_t open = 0, open_inf = 0;
-    float wr[STRIP];
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         const int gy = y0 + strip * STRIP + k;
-        wr[k] = PF_INF;
         if (gx < a.W && gy < a.H) {
             const size_t gi = (size_t)gy * a.W + gx;
             if (a.st[gi] == PF_OPEN) {
-                wr[k] = a.w[gi];
-                if (wr[k] < PF_INF) open |= 1u << k;
+                if (a.w[gi] < PF_INF) open |= 1u << k;
                 else open_inf |= 1u << k;
             }
         }
     }
     __syncthreads();
 
+    // Row step: seeds are OPEN cells with a live FIXED neighbour no higher
+    // than their value; along the row, fixing spreads from a cell fixed live
+    // to an OPEN neighbour whose value is not lower (segmented propagation
+    // over the lane masks).
     uint32_t fixed = 0, fixed_live = 0;
     for (int sweep = 0;; ++sweep) {
-        unsigned long long* cur = S.dirty[sweep & 1];
-        unsigned long long* nxt = S.dirty[(sweep + 1) & 1];
         int changed = 0;
-#pragma unroll
         for (int kk = 0; kk < STRIP; ++kk) {
             const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
-            const unsigned long long bit = 1ull << lx;
-            if (!(cur[ly] & bit)) continue;
-            atomicAnd(&cur[ly], ~bit);
-            if (!(open & (1u << k))) continue;
+            if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
+            S.rowdirty[ly] = 0u;
             const int li = (ly + 1) * HALO_W + lx + 1;
-            if (min8y(S.rel, li) <= wr[k]) {
-                open &= ~(1u << k);
-                fixed |= 1u << k;
-                if (pf_live(wr[k], mn)) {
-                    fixed_live |= 1u << k;
-                    S.rel[li].y = wr[k];
-                    changed = 1;
-                    mark_dirty(nxt, lx, ly);
+            const uint32_t bit = 1u << k;
+            const bool op = (open & bit) != 0;
+            const float w = S.rel[li].x;
+            const bool live = op && pf_live(w, mn);
+            const bool seed = op && min8y(S.rel, li) <= w;
+            // from the left neighbour x-1 (open, live) into x, and from x+1
+            const float wl = __shfl_up(w, 1, 64), wrt = __shfl_down(w, 1, 64);
+            const int livel = __shfl_up((int)live, 1, 64), liver = __shfl_down((int)live, 1, 64);
+            const bool gl = op && lx > 0 && livel && wl <= w;
+            const bool gr = op && lx < TILE - 1 && liver && wrt <= w;
+            const unsigned long long sd = __ballot(seed);
+            unsigned long long f = sd;
+            if (sd) f = spread_row(sd, __ballot(gl), __ballot(gr));
+            bool ch = false;
+            if ((f >> lx) & 1ull) {
+                open &= ~bit;
+                fixed |= bit;
+                if (live) {
+                    fixed_live |= bit;
+                    S.rel[li].y = w;
+                    ch = true;
                 }
+            }
+            if (__ballot(ch)) {
+                changed = 1;
+                mark_rows(S.rowdirty, ly, false);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -323,7 +376,7 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, float mn, uint32
             else
                 cert_tile(S, a, t, mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
         }
-        grid_barrier();
+        grid_sync(a.bar);
         li = cout;
     }
     return -1;
@@ -393,7 +446,7 @@ __global__ __launch_bounds__(256) void k_pf_persistent(PfArgs a) {
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
-    grid_barrier();
+    grid_sync(a.bar);
 
     int li = 0;
     int round = 0;
@@ -418,17 +471,17 @@ __global__ __launch_bounds__(256) void k_pf_persistent(PfArgs a) {
                     }
             }
             if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_CERT + (round + 1) % 3] = 0;
-            grid_barrier();
+            grid_sync(a.bar);
         }
         li = run_passes<0>(S, a, li, mn, cert_ctr, epoch);
         if (li < 0) break;
         li = build_open_list(a, li);
-        grid_barrier();
+        grid_sync(a.bar);
         if (ld(&a.ctrl[CTL_COUNT + li]) == 0) break;
         li = run_passes<1>(S, a, li, mn, cert_ctr, epoch);
         if (li < 0) break;
         li = build_open_list(a, li);
-        grid_barrier();
+        grid_sync(a.bar);
         const uint32_t open_tiles = ld(&a.ctrl[CTL_COUNT + li]);
         if (open_tiles == 0) break;
         if (ld(cert_ctr) == 0) {
@@ -495,6 +548,7 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.mark = c->d_mark_pf;
     a.topen = c->d_topen;
     a.ctrl = c->d_ctrl;
+    a.bar = c->d_bar;
     a.scal = c->d_scal;
     a.max_passes = 1 << 20;
     void* args[] = {&a};

@@ -16,6 +16,56 @@ namespace cageo {
 __device__ constexpr int TDX[8] = {0, 1, 1, 1, 0, -1, -1, -1};
 __device__ constexpr int TDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 
+// ---- exact row solves of the LDS tile sweeps -------------------------------
+// A wave owns a strip of STRIP tile rows, one lane per column.  Each step
+// solves one row exactly with its two neighbour rows held at their current
+// values: along the row the relaxation is a chain of per-cell monotone maps,
+// composed for all 64 columns at once by wave scans (6 shuffle steps in
+// each direction), so information crosses the whole row in one step rather
+// than one column per sweep.  Rows are visited in alternating order (Gauss-
+// Seidel down, then up); a row whose cells changed marks its neighbour rows
+// for another solve, and a tile is done after a sweep without changes.
+
+// Clamp map v -> min(max(v, lo), hi) with lo <= hi.  Replaces (lo, hi) by
+// the composition "self after prev".  Compositions of clamps are clamps.
+__device__ inline void clamp_after(float& lo, float& hi, float plo, float phi) {
+    const float nlo = fminf(fmaxf(plo, lo), hi);
+    const float nhi = fminf(fmaxf(phi, lo), hi);
+    lo = nlo;
+    hi = nhi;
+}
+
+// Inclusive scan towards higher lanes: lane x ends with f_x o ... o f_0.
+__device__ inline void scan_clamp_up(float& lo, float& hi) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float plo = __shfl_up(lo, d, 64), phi = __shfl_up(hi, d, 64);
+        if (lane >= d) clamp_after(lo, hi, plo, phi);
+    }
+}
+
+// Inclusive scan towards lower lanes: lane x ends with f_x o ... o f_63.
+__device__ inline void scan_clamp_down(float& lo, float& hi) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float plo = __shfl_down(lo, d, 64), phi = __shfl_down(hi, d, 64);
+        if (lane + d < 64) clamp_after(lo, hi, plo, phi);
+    }
+}
+
+// Row flags of the sweeps: rows ly-1 and ly+1, and ly itself when `self`,
+// need another solve.  Plain stores of 1 (the flags only ever go 0 -> 1
+// between the owner's read-and-clear steps).
+__device__ inline void mark_rows(uint32_t* rd, int ly, bool self) {
+    if ((threadIdx.x & 63) == 0) {
+        if (ly > 0) rd[ly - 1] = 1u;
+        if (ly < TILE - 1) rd[ly + 1] = 1u;
+        if (self) rd[ly] = 1u;
+    }
+}
+
 // Bits of the neighbour tiles whose halo contains local cell (lx, ly).
 __device__ inline uint32_t border_bits(int lx, int ly) {
     uint32_t b = 0;
