@@ -111,18 +111,41 @@ struct PfArgs {
     int max_passes;
 };
 
+// Loads the tile's relays with a one-cell halo.  The state byte and the
+// value of a cell are loaded together (no load waits on another), over a
+// fully unrolled loop, so the loads of all iterations are in flight at once.
 __device__ inline void load_tile(PfShared& S, const PfArgs& a, int x0, int y0) {
-    for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
-        int hy = k / HALO_W, hx = k - hy * HALO_W;
-        int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-        float r = PF_INF, cr = PF_INF;
-        if (gx >= 0 && gy >= 0 && gx < a.W && gy < a.H) {
-            size_t gi = (size_t)gy * a.W + gx;
-            uint8_t s = a.st[gi];
-            if (s == PF_OPEN || s == PF_FIXED_LIVE) r = a.w[gi];
-            if (s == PF_FIXED_LIVE) cr = r;
+    constexpr int N = HALO_W * HALO_W;
+    constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
+    constexpr int CHUNK = (ITER + 1) / 2;  // two batches keep the register use down
+#pragma unroll
+    for (int c0 = 0; c0 < ITER; c0 += CHUNK) {
+        uint8_t sv[CHUNK];
+        float wv[CHUNK];
+#pragma unroll
+        for (int j = 0; j < CHUNK; ++j) {
+            const int k = threadIdx.x + (c0 + j) * TILE_THREADS;
+            sv[j] = PF_NODATA;
+            wv[j] = PF_INF;
+            if (c0 + j < ITER && k < N) {
+                const int hy = k / HALO_W, hx = k - hy * HALO_W;
+                const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+                if (gx >= 0 && gy >= 0 && gx < a.W && gy < a.H) {
+                    const size_t gi = (size_t)gy * a.W + gx;
+                    sv[j] = a.st[gi];
+                    wv[j] = a.w[gi];
+                }
+            }
         }
-        S.rel[k] = make_float2(r, cr);
+#pragma unroll
+        for (int j = 0; j < CHUNK; ++j) {
+            const int k = threadIdx.x + (c0 + j) * TILE_THREADS;
+            if (c0 + j < ITER && k < N) {
+                const uint8_t st = sv[j];
+                const float r = (st == PF_OPEN || st == PF_FIXED_LIVE) ? wv[j] : PF_INF;
+                S.rel[k] = make_float2(r, st == PF_FIXED_LIVE ? r : PF_INF);
+            }
+        }
     }
     if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 1u;
     if (threadIdx.x == 0) {
@@ -167,17 +190,25 @@ __device__ void fill_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
     const int gx = x0 + lx;
     uint32_t open = 0;
 #pragma unroll
-    for (int k = 0; k < STRIP; ++k) {
-        const int ly = strip * STRIP + k, gy = y0 + ly;
-        float z = 0.0f;
-        if (gx < a.W && gy < a.H) {
-            const size_t gi = (size_t)gy * a.W + gx;
-            if (a.st[gi] == PF_OPEN) {
-                open |= 1u << k;
-                z = a.zq[gi];
+    for (int k0 = 0; k0 < STRIP; k0 += STRIP / 2) {  // two batches of independent loads
+        uint8_t sv[STRIP / 2];
+        float zv[STRIP / 2];
+#pragma unroll
+        for (int j = 0; j < STRIP / 2; ++j) {
+            const int gy = y0 + strip * STRIP + k0 + j;
+            sv[j] = PF_NODATA;
+            zv[j] = 0.0f;
+            if (gx < a.W && gy < a.H) {
+                const size_t gi = (size_t)gy * a.W + gx;
+                sv[j] = a.st[gi];
+                zv[j] = a.zq[gi];
             }
         }
-        S.zt[ly * TILE + lx] = z;
+#pragma unroll
+        for (int j = 0; j < STRIP / 2; ++j) {
+            if (sv[j] == PF_OPEN) open |= 1u << (k0 + j);
+            S.zt[(strip * STRIP + k0 + j) * TILE + lx] = zv[j];
+        }
     }
     __syncthreads();
 
@@ -281,18 +312,21 @@ __device__ void cert_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
     uint32_t open = 0, open_inf = 0;
+    uint8_t sv[STRIP];
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         const int gy = y0 + strip * STRIP + k;
-        if (gx < a.W && gy < a.H) {
-            const size_t gi = (size_t)gy * a.W + gx;
-            if (a.st[gi] == PF_OPEN) {
-                if (a.w[gi] < PF_INF) open |= 1u << k;
-                else open_inf |= 1u << k;
-            }
+        sv[k] = PF_NODATA;
+        if (gx < a.W && gy < a.H) sv[k] = a.st[(size_t)gy * a.W + gx];
+    }
+    __syncthreads();  // the relays of load_tile are in LDS
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) {
+        if (sv[k] == PF_OPEN) {
+            if (S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1].x < PF_INF) open |= 1u << k;
+            else open_inf |= 1u << k;
         }
     }
-    __syncthreads();
 
     // Row step: seeds are OPEN cells with a live FIXED neighbour no higher
     // than their value; along the row, fixing spreads from a cell fixed live
@@ -397,7 +431,7 @@ __device__ inline int build_open_list(const PfArgs& a, int b) {
     return l;
 }
 
-__global__ __launch_bounds__(256) void k_pf_persistent(PfArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pf_persistent(PfArgs a) {
     __shared__ PfShared S;
     const uint32_t* scal = a.scal;
     const uint32_t bad = ld(&scal[SC_BAD]);
