@@ -14,10 +14,11 @@ Prints one JSON line.  `value` is realizations/s of the Gaussian workload on
 one GPU.  Per-stage device times come from HIP events that the library
 records on its stream between the stages of every timed realization (no host
 synchronisation inside the timed region).  `roofline` covers the dominant
-kernel: algorithmic bytes per launch / its average launch time.  `traffic`
-is the HBM bytes per launch of that kernel from two rocprofv3 counter passes
-(FETCH_SIZE, then WRITE_SIZE) over a short probe run, scaled by what the same
-counters read for an elementwise kernel that reads and writes 512 MiB.
+kernel: algorithmic bytes per launch / its average launch time.  With
+--traffic, `traffic` is the HBM bytes per launch of that kernel from two
+rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over a short probe
+run (`--probe`), scaled by what the same counters read for an elementwise
+kernel that reads and writes 512 MiB; otherwise it is null.
 `cpu_baseline` times the CPU oracle (a restatement of the reference's OpenMP
 CPU path) on a bounded sample, in a subprocess pinned to one thread.
 """
@@ -73,8 +74,10 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=3,
                    help="realizations timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-stage-times", action="store_true")
-    p.add_argument("--no-traffic", action="store_true",
-                   help="skip the rocprofv3 counter passes (traffic = null)")
+    p.add_argument("--traffic", action="store_true",
+                   help="run the two rocprofv3 counter passes for roofline.traffic (off by "
+                        "default: the profiler can crash in its own exit path, so the passes "
+                        "are normally run separately and summarized in profiles/)")
     p.add_argument("--probe", action="store_true",
                    help="internal: the short run profiled by the counter passes")
     return p.parse_args()
@@ -127,50 +130,54 @@ def probe(args):
 
 
 def counter_pass(counter, args, outdir):
-    """One rocprofv3 --pmc pass over the probe; returns {kernel: [values]}."""
+    """One rocprofv3 --pmc pass over the probe; returns ({kernel: [values]}, error)."""
     d = os.path.join(outdir, counter)
     cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
            sys.executable, os.path.abspath(__file__), "--probe", "--size", str(args.size),
            "--seed", str(args.seed)]
-    subprocess.run(cmd, capture_output=True, text=True, timeout=60, check=False)
+    try:
+        # rocprofv3 may crash at its own exit after writing complete results
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=60, check=False)
+    except subprocess.TimeoutExpired:
+        return None, f"{counter}: profiler pass timed out"
     files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
     if not files:
-        return None
+        return None, f"{counter}: no counter file (rc {res.returncode}): {res.stderr[-300:]}"
     vals = {}
     with open(files[0]) as f:
         for row in csv.DictReader(f):
             row = {k.lower(): v for k, v in row.items()}
             if row.get("counter_name") != counter:
                 continue
-            name = row.get("kernel_name", "")
-            vals.setdefault(name, []).append(float(row["counter_value"]))
-    return vals
+            vals.setdefault(row.get("kernel_name", ""), []).append(float(row["counter_value"]))
+    return vals, None
 
 
 def measure_traffic(args, kernel):
-    """HBM bytes per launch of `kernel` (read, write), calibrated; None if the
-    profiler or its counters are unavailable."""
+    """HBM bytes per launch of `kernel` (read and write), each counter scaled
+    by what it reads for the calibration kernel (an elementwise multiply that
+    reads and writes CALIB_BYTES).  Returns (result or None, reason)."""
     if not shutil.which("rocprofv3"):
-        return None
+        return None, "rocprofv3 not found"
     outdir = tempfile.mkdtemp(prefix="cageo_pmc_")
     try:
         res = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            vals = counter_pass(counter, args, outdir)
-            if not vals:
-                return None
+            vals, err = counter_pass(counter, args, outdir)
+            if vals is None:
+                return None, err
             k = [v for name, v in vals.items() if kernel in name]
-            cal = [v for name, v in vals.items() if "elementwise" in name]
+            cal = [v for name, v in vals.items() if "AUnaryFunctor" in name]
             if not k or not cal:
-                return None
+                return None, f"{counter}: kernels not found in {sorted(vals)[:8]}"
             per_launch_kib = sum(k[0]) / len(k[0])
             cal_kib = max(cal[0])
             factor = CALIB_BYTES / (cal_kib * 1024.0)
             res[counter] = {"kib_per_launch": per_launch_kib, "calibration_kib": cal_kib,
                             "factor": factor, "bytes": per_launch_kib * 1024.0 * factor}
-        return res
-    except (subprocess.SubprocessError, OSError, ValueError, KeyError):
-        return None
+        return res, None
+    except (OSError, ValueError, KeyError) as e:
+        return None, f"counter parsing failed: {e}"
     finally:
         shutil.rmtree(outdir, ignore_errors=True)
 
@@ -251,7 +258,9 @@ def main():
         kernel = STAGE_KERNEL[key]
         bytes_per_launch = STAGE_BYTES[key] * cells
         achieved = bytes_per_launch / (stages[key] * 1e-3) / 1e9
-        traffic = None if args.no_traffic else measure_traffic(args, kernel)
+        traffic, why = measure_traffic(args, kernel) if args.traffic else (None, "not requested")
+        if traffic is None and args.traffic:
+            print(f"bench.py: traffic not measured: {why}", file=sys.stderr)
         line["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
