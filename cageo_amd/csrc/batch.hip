@@ -11,6 +11,8 @@
 //   GRAPH     one realization of every tile captured once as a hipGraph and
 //             replayed n times; the random-field kernels read the
 //             realization index from device memory, which a last node bumps.
+//             The persistent kernels are ordinary kernel nodes: their grids
+//             are within the occupancy cap and graph nodes run one at a time.
 // The autotuner picks, per tile, the workgroup count of the persistent
 // kernels (small tiles have few 64x64 tiles, so a full-chip grid mostly
 // waits at barriers) and the row-block height of the 2-D convolution.
@@ -42,6 +44,7 @@ struct BatchTile {
 
 struct cageo_batch {
     hipStream_t stream = nullptr;
+    bool own_stream = false;  // created here: the NULL stream cannot be captured
     std::vector<cageo::BatchTile> tiles;
     uint32_t* d_iter = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -73,24 +76,6 @@ int check_all(cageo_batch* b) {
     return CAGEO_OK;
 }
 
-// Kernel nodes of `g` that carry the cooperative launch attribute.
-int count_cooperative_nodes(hipGraph_t g, int* count) {
-    size_t n = 0;
-    CAGEO_HIP_TRY(hipGraphGetNodes(g, nullptr, &n));
-    std::vector<hipGraphNode_t> nodes(n);
-    CAGEO_HIP_TRY(hipGraphGetNodes(g, nodes.data(), &n));
-    *count = 0;
-    for (hipGraphNode_t node : nodes) {
-        hipGraphNodeType type;
-        CAGEO_HIP_TRY(hipGraphNodeGetType(node, &type));
-        if (type != hipGraphNodeTypeKernel) continue;
-        hipLaunchAttributeValue v = {};
-        CAGEO_HIP_TRY(hipGraphKernelNodeGetAttribute(node, hipLaunchAttributeCooperative, &v));
-        if (v.cooperative) ++*count;
-    }
-    return CAGEO_OK;
-}
-
 int build_graph(cageo_batch* b, float* const* mc) {
     drop_graph(b);
     // workgroup counts are fixed before capture (no device queries inside it)
@@ -113,15 +98,6 @@ int build_graph(cageo_batch* b, float* const* mc) {
     if (st != CAGEO_OK || e != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return st != CAGEO_OK ? st : CAGEO_ERR_HIP;
-    }
-    // The persistent kernels wait at grid barriers, so each of them (three
-    // per tile) must have been captured as a cooperative launch; a graph
-    // without that guarantee is refused rather than run.
-    int coop = 0;
-    const int cst = count_cooperative_nodes(g, &coop);
-    if (cst != CAGEO_OK || coop != 3 * (int)b->tiles.size()) {
-        (void)hipGraphDestroy(g);
-        return CAGEO_ERR_STATE;
     }
     const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -159,16 +135,26 @@ int cageo_batch_create(cageo_batch** out, int ntiles, const int* widths, const i
     if (!out || ntiles <= 0 || !widths || !heights) return CAGEO_ERR_ARG;
     cageo_batch* b = new cageo_batch();
     b->stream = reinterpret_cast<hipStream_t>(stream);
+    if (!b->stream) {
+        // a blocking stream (default flags and priority): it still orders
+        // after work on the NULL stream, and unlike it, it can be captured
+        if (hipStreamCreate(&b->stream) != hipSuccess) {
+            delete b;
+            return CAGEO_ERR_HIP;
+        }
+        b->own_stream = true;
+    }
     b->tiles.resize(ntiles);
     int st = CAGEO_OK;
     for (int t = 0; t < ntiles && st == CAGEO_OK; ++t)
-        st = cageo_context_create(&b->tiles[t].ctx, widths[t], heights[t], stream);
+        st = cageo_context_create(&b->tiles[t].ctx, widths[t], heights[t], b->stream);
     if (st == CAGEO_OK &&
         hipMalloc(reinterpret_cast<void**>(&b->d_iter), sizeof(uint32_t)) != hipSuccess)
         st = CAGEO_ERR_NOMEM;
     if (st != CAGEO_OK) {
         for (auto& t : b->tiles)
             if (t.ctx) cageo_context_destroy(t.ctx);
+        if (b->own_stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return st;
     }
@@ -183,6 +169,7 @@ int cageo_batch_destroy(cageo_batch* b) {
     for (auto& t : b->tiles)
         if (t.ctx) cageo_context_destroy(t.ctx);
     if (b->d_iter) (void)hipFree(b->d_iter);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return CAGEO_OK;
 }

@@ -142,8 +142,11 @@ __host__ __device__ inline bool pf_live(float v, float min_elev) {
     return pf_to_elev(pf_to_index(v, min_elev), min_elev) == v;
 }
 
-// Grid-wide barrier of the persistent kernels (all workgroups co-resident:
-// they are launched cooperatively with at most the occupancy limit).
+// Grid-wide barrier of the persistent kernels.  All workgroups must be
+// co-resident: the grids are capped at the device's occupancy limit for the
+// kernel, so every workgroup runs at once when the device is free of other
+// work (other work can only delay a workgroup's start, and the barrier then
+// waits for it).
 //
 // Cost model on a multi-XCD part: each XCD has its own L2, so making data
 // written before the barrier visible to every workgroup after it takes an
@@ -226,6 +229,9 @@ int check_status(Context* c);
 // `evs`, events evs[2..7] are recorded after each stage.
 int realization_body(Context* c, float* mc, hipEvent_t* evs);
 int pit_fill_grid_cap(Context* c, int* cap);
+const void* pit_fill_kernel();
+const void* flow_routing_kernel();
+const void* flow_tracing_kernel();
 int flow_routing_grid_cap(Context* c, int* cap);
 int flow_tracing_grid_cap(Context* c, int* cap);
 int clear_status(Context* c);

@@ -15,8 +15,8 @@
 //                            n has a direction (d = 0) or is a flat },
 // followed by direction = last n in order with d(n) = d(c) - 1 and
 // z_n <= z_c.  d is computed by monotone min-relaxation over 64x64 LDS
-// tiles (dirty-bit sweeps) inside one persistent cooperative kernel, with the
-// same worklist protocol as the pit fill.
+// tiles (exact row solves by min-plus wave scans) inside one persistent
+// kernel, with the same worklist protocol as the pit fill.
 #include "tiles.hpp"
 
 namespace cageo {
@@ -308,6 +308,9 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
 
 }  // namespace
 
+// Device function of the persistent kernel (to find its node in a graph).
+const void* flow_routing_kernel() { return reinterpret_cast<const void*>(k_fr_persistent); }
+
 // Co-resident workgroups of the persistent kernel on this device.
 int flow_routing_grid_cap(Context* c, int* cap) {
     int per_cu = 0;
@@ -315,7 +318,7 @@ int flow_routing_grid_cap(Context* c, int* cap) {
         &per_cu, reinterpret_cast<const void*>(k_fr_persistent), TILE_THREADS, 0));
     hipDeviceProp_t prop;
     CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-    if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+    if (per_cu < 1) return CAGEO_ERR_HIP;
     *cap = per_cu * prop.multiProcessorCount;
     return CAGEO_OK;
 }
@@ -336,10 +339,9 @@ int flow_routing(Context* c, const float* z, int8_t* fd) {
     a.ctrl = c->d_ctrl;
     a.bar = c->d_bar;
     a.max_passes = 1 << 20;
-    void* args[] = {&a};
-    CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_fr_persistent),
-                                             dim3(c->fr_grid), dim3(TILE_THREADS), args, 0,
-                                             c->stream));
+    // ordinary launch within the occupancy cap (see pit_fill_quantized)
+    hipLaunchKernelGGL(k_fr_persistent, dim3(c->fr_grid), dim3(TILE_THREADS), 0, c->stream, a);
+    CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
 

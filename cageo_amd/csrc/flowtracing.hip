@@ -141,7 +141,7 @@ struct TrArgs {
     uint32_t* scal;
 };
 
-// Tile pass, exit jumping, labels and the connect step in one cooperative
+// Tile pass, exit jumping, labels and the connect step in one persistent
 // launch.  Round r of the exit jumping raises flag r % 3 while exits remain
 // and clears flag (r+1) % 3, whose last read (end of round r-2) is separated
 // from the clear by barriers.
@@ -212,6 +212,9 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
 
 // Needs two 64-bit rasters: c->d_ptr and c->d_ptr2.  The long-path count is
 // left in the scalar block.
+// Device function of the persistent kernel (to find its node in a graph).
+const void* flow_tracing_kernel() { return reinterpret_cast<const void*>(k_tr_persistent); }
+
 // Co-resident workgroups of the persistent kernel on this device.
 int flow_tracing_grid_cap(Context* c, int* cap) {
     int per_cu = 0;
@@ -219,7 +222,7 @@ int flow_tracing_grid_cap(Context* c, int* cap) {
         &per_cu, reinterpret_cast<const void*>(k_tr_persistent), TILE_THREADS, 0));
     hipDeviceProp_t prop;
     CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-    if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+    if (per_cu < 1) return CAGEO_ERR_HIP;
     *cap = per_cu * prop.multiProcessorCount;
     return CAGEO_OK;
 }
@@ -244,10 +247,9 @@ int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int3
     a.ctrl = c->d_ctrl;
     a.bar = c->d_bar;
     a.scal = c->d_scal;
-    void* args[] = {&a};
-    CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tr_persistent),
-                                             dim3(c->tr_grid), dim3(TILE_THREADS), args, 0,
-                                             c->stream));
+    // ordinary launch within the occupancy cap (see pit_fill_quantized)
+    hipLaunchKernelGGL(k_tr_persistent, dim3(c->tr_grid), dim3(TILE_THREADS), 0, c->stream, a);
+    CAGEO_HIP_TRY(hipGetLastError());
     if (long_paths) {
         CAGEO_TRY(read_scalars(c));
         *long_paths = (int64_t)c->h_scal[SC_LONGPATH];

@@ -28,11 +28,12 @@
 // relay path into it through OPEN cells has bottleneck >= its value), so a
 // round that fixes nothing leaves only unreached cells; they keep Z.
 //
-// Execution: one persistent cooperative kernel runs every round.  Passes
-// over a de-duplicated worklist of 64x64 tiles are separated by grid
-// barriers; inside a tile the relaxation sweeps only cells whose
-// neighbourhood changed (dirty bits in LDS).  Every decision that ends a
-// loop is read by all workgroups after a barrier, so they leave together.
+// Execution: one persistent kernel runs every round.  Passes over a
+// de-duplicated worklist of 64x64 tiles are separated by grid barriers;
+// inside a tile, rows are solved exactly one at a time (wave scans along the
+// row, rows revisited while a neighbouring row changed).  Every decision that
+// ends a loop is read by all workgroups after a barrier, so they leave
+// together.
 //
 // Every comparison is on floats; the only arithmetic is the quantisation and
 // the round-trip test, both evaluated exactly as the reference does.
@@ -546,15 +547,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 }  // namespace
 
-// Co-resident workgroups of the persistent kernel on this device: the most a
-// cooperative launch may use.
+// Device function of the persistent kernel (to find its node in a graph).
+const void* pit_fill_kernel() { return reinterpret_cast<const void*>(k_pf_persistent); }
+
+// Co-resident workgroups of the persistent kernel on this device: the most
+// its grid barrier allows.
 int pit_fill_grid_cap(Context* c, int* cap) {
     int per_cu = 0;
     CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, reinterpret_cast<const void*>(k_pf_persistent), TILE_THREADS, 0));
     hipDeviceProp_t prop;
     CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-    if (per_cu < 1 || !prop.cooperativeLaunch) return CAGEO_ERR_HIP;
+    if (per_cu < 1) return CAGEO_ERR_HIP;
     *cap = per_cu * prop.multiProcessorCount;
     return CAGEO_OK;
 }
@@ -585,9 +589,12 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.bar = c->d_bar;
     a.scal = c->d_scal;
     a.max_passes = 1 << 20;
-    void* args[] = {&a};
-    CAGEO_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pf_persistent),
-                                             dim3(grid), dim3(TILE_THREADS), args, 0, c->stream));
+    // An ordinary launch: the grid barrier needs its workgroups co-resident,
+    // which the occupancy cap guarantees (work of other kernels on the device
+    // can only delay the start of some workgroups).  Cooperative launches are
+    // not used: the runtime allows them to one process at a time.
+    hipLaunchKernelGGL(k_pf_persistent, dim3(grid), dim3(TILE_THREADS), 0, c->stream, a);
+    CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
 

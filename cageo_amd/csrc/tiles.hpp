@@ -1,11 +1,11 @@
 // Tile worklists for the iterative stencil stages (pit fill, certification,
-// flat resolution).  A pass launches one 256-thread workgroup per listed
-// 64x64 tile; the tile iterates to a local fixed point in LDS against the
-// halo it loaded, writes back, and lists the neighbour tiles whose halo it
-// changed (border cells only) for the next pass.  A per-tile epoch stamp
-// de-duplicates the next list without clearing it.  The host loop stops when
-// a pass lists nothing: then every tile is at a fixed point with respect to
-// its neighbours' final values.
+// flat resolution).  In a pass, each listed 64x64 tile is taken by one
+// 256-thread workgroup of the persistent kernel; the tile iterates to a local
+// fixed point in LDS against the halo it loaded, writes back, and lists the
+// neighbour tiles whose halo it changed (border cells only) for the next
+// pass.  A per-tile epoch stamp de-duplicates the next list without clearing
+// it.  The passes stop when one lists nothing: then every tile is at a fixed
+// point with respect to its neighbours' final values.
 #pragma once
 
 #include "common.hpp"

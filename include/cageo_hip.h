@@ -163,8 +163,10 @@ int cageo_stage_times(cageo_context* ctx, float* out, int enable);
  *   CAGEO_BATCH_PER_TILE  one blocking cageo_run_realizations per tile;
  *   CAGEO_BATCH_STREAM    all tiles x realizations on one stream, one wait;
  *   CAGEO_BATCH_GRAPH     one realization of all tiles captured as a hipGraph
- *                         (cooperative kernel nodes) and replayed n times.
- * d_mc is an array of ntiles device pointers (one counter raster per tile). */
+ *                         and replayed n times.
+ * d_mc is an array of ntiles device pointers (one counter raster per tile).
+ * With hip_stream NULL the batch creates its own (blocking, default-priority)
+ * stream, since the NULL stream cannot be captured into a graph. */
 #define CAGEO_BATCH_PER_TILE 0
 #define CAGEO_BATCH_STREAM 1
 #define CAGEO_BATCH_GRAPH 2
