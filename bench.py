@@ -21,6 +21,9 @@ run (`--probe`), scaled by what the same counters read for an elementwise
 kernel that reads and writes 512 MiB; otherwise it is null.
 `cpu_baseline` times the CPU oracle (a restatement of the reference's OpenMP
 CPU path) on a bounded sample, in a subprocess pinned to one thread.
+`surrogate` judges the packaged border-probability network (with its
+Monte Carlo check) against a 1000-realization Monte Carlo of the same DEM:
+per-cell deviation of the border probability and wall time of each.
 """
 from __future__ import annotations
 
@@ -74,6 +77,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=3,
                    help="realizations timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-stage-times", action="store_true")
+    p.add_argument("--surrogate-realizations", type=int, default=1000,
+                   help="Monte Carlo size the surrogate is judged against (0: skip)")
     p.add_argument("--traffic", action="store_true",
                    help="run the two rocprofv3 counter passes for roofline.traffic (off by "
                         "default: the profiler can crash in its own exit path, so the passes "
@@ -182,6 +187,40 @@ def measure_traffic(args, kernel):
         shutil.rmtree(outdir, ignore_errors=True)
 
 
+def surrogate_leg(torch, ca, dem, stream, args):
+    """Checked surrogate prediction against an N-realization Monte Carlo of the
+    same DEM and error model: mean absolute and tail per-cell deviation of the
+    border probability, and the wall time of each (device work bracketed by
+    synchronisations; the Monte Carlo is the batched graph form)."""
+    from cageo_amd import surrogate as sg
+    n = args.surrogate_realizations
+    net = sg.load_net(torch)
+    # warm-up: kernels, allocator and graph code paths on a 512^2 crop
+    sg.checked_prediction(torch, net, dem[:512, :512], stream[:512, :512], 1.8, 60.0, 1,
+                          check_realizations=2, fallback_realizations=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p_mc = sg.mc_frequency(torch, [dem], [stream], [1.8], [60.0], n, args.seed,
+                           on_device=True)[0]
+    torch.cuda.synchronize()
+    mc_s = time.perf_counter() - t0
+    q, info = sg.checked_prediction(torch, net, dem, stream, 1.8, 60.0, args.seed + 7,
+                                    fallback_realizations=n)
+    sur_s = info["predict_s"] + info["check_s"] + info.get("fallback_s", 0.0)
+    dev = (q - p_mc).abs()
+    border = p_mc > 0
+    return {
+        "mc_realizations": n, "mc_s": round(mc_s, 4), "surrogate_s": round(sur_s, 4),
+        "speedup_vs_mc": round(mc_s / sur_s, 1),
+        "l1": round(float(dev.mean()), 5),
+        "l1_border_cells": round(float(dev[border].mean()), 5) if bool(border.any()) else 0.0,
+        "p999_dev": round(float(torch.quantile(dev.flatten()[::16].float(), 0.999)), 4),
+        "max_dev": round(float(dev.max()), 4),
+        "check": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in info.items()},
+        "weights": os.path.relpath(sg.DEFAULT_WEIGHTS, ROOT),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -288,6 +327,8 @@ def main():
                 "bound": "fp32-vector", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
                 "note": "k_conv_2d plus its output copy; no FMA (exact reference order)"}
+    if args.surrogate_realizations > 0:
+        line["surrogate"] = surrogate_leg(torch, ca, dem, stream, args)
     if args.cpu_sample > 0 and oracle_ok:
         cps, secs = cpu_baseline(size, args.cpu_sample, args.seed)
         line["cpu_baseline"] = {

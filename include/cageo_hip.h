@@ -206,6 +206,55 @@ int cageo_iteration_counts(cageo_context* ctx, int32_t* out);
  * Results do not depend on them. */
 int cageo_context_configure(cageo_context* ctx, int persistent_workgroups, int conv_rows);
 
+/* ---- border-probability surrogate: layer kernels ---------------------------
+ * No reference counterpart (the reference has no learned part).  BF16
+ * operands with FP32 accumulation on the MFMA units; activations NHWC bf16
+ * (uint16 bit patterns) with channel counts that are multiples of 32.
+ * All calls enqueue on `stream` (a hipStream_t) and return at once.
+ *
+ * 3x3 convolution, stride 1, zero padding 1, over the channel concatenation
+ * [in1 | in2] (in2 may be NULL with c2 = 0); with up1 the first input is
+ * read at half resolution (nearest 2x upsampling).  w: [9][cout][c1+c2]
+ * bf16; bias fp32 or NULL; writes bf16 `out` (ReLU when relu) and/or fp32
+ * `outf`.  With weights from cageo_nn_flip_transpose it computes the
+ * gradient with respect to the input. */
+int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const uint16_t* in2,
+                     int c2, const uint16_t* w, const float* bias, uint16_t* out, float* outf,
+                     int n, int h, int w_, int cout, int relu);
+/* dw[tap][co][ci] += sum_p g[p][co] * in[p + offset(tap)][ci] (fp32 dw). */
+int cageo_nn_conv3x3_wgrad(void* stream, const uint16_t* in1, int c1, int up1,
+                           const uint16_t* in2, int c2, const uint16_t* g, float* dw, int n, int h,
+                           int w_, int cout);
+/* wt[8 - tap][ci][co] = w[tap][co][ci]. */
+int cageo_nn_flip_transpose(void* stream, const uint16_t* w, uint16_t* wt, int cout, int cin);
+int cageo_nn_avgpool2(void* stream, const uint16_t* in, uint16_t* out, int n, int h, int w_, int c);
+/* dx (fp32, h x w) += gradient of the 2x2 average pool from dy (h/2 x w/2). */
+int cageo_nn_avgpool2_back(void* stream, const float* dy, float* dx, int n, int h, int w_, int c);
+/* dx (half resolution) += 2x2 sums of channels [coff, coff+c) of dy. */
+int cageo_nn_upsample2_back(void* stream, const float* dy, float* dx, int n, int h, int w_, int c,
+                            int ctot, int coff);
+/* dx[p][0..c) (+)= dy[p][coff..coff+c) */
+int cageo_nn_slice_add(void* stream, const float* dy, float* dx, long long p, int c, int ctot,
+                       int coff, int accumulate);
+/* g = bf16(dy * (act > 0)) */
+int cageo_nn_relu_back(void* stream, const float* dy, const uint16_t* act, uint16_t* g,
+                       long long n);
+/* db[c] += sum_p g[p][c] */
+int cageo_nn_bias_grad(void* stream, const uint16_t* g, float* db, long long p, int c);
+/* logit[p] = b + sum_c w[c] act[p][c]; with sigmoid != 0 the probability
+ * 1 / (1 + exp(-logit)) is written instead. */
+int cageo_nn_head(void* stream, const uint16_t* act, const float* w, const float* b, float* logit,
+                  long long p, int c, int sigmoid);
+/* q = sigmoid(logit); loss += mean BCE(q, target); d(act), dw, db of the head. */
+int cageo_nn_head_back(void* stream, const uint16_t* act, const float* w, const float* logit,
+                       const float* target, float* q, float* dact, float* dw, float* db,
+                       float* loss, long long p, int c);
+/* Adam on fp32 master weights p (bias-corrected, step >= 1); refreshes the
+ * bf16 copy pb when not NULL. */
+int cageo_nn_adam(void* stream, float* p, const float* g, float* m, float* v, uint16_t* pb,
+                  long long n, float lr, float b1, float b2, float eps, int step);
+int cageo_nn_to_bf16(void* stream, const float* x, uint16_t* y, long long n);
+
 /* ---- synthetic inputs (host) -----------------------------------------------
  * Seeded DEM: 100 m base + 0.5 % tilt along +x + 40 Gaussian hills
  * (height U(5,60) m, sigma U(50,400) m) + N(0, 0.3^2) noise; streams where
