@@ -21,6 +21,8 @@ run (`--probe`), scaled by what the same counters read for an elementwise
 kernel that reads and writes 512 MiB; otherwise it is null.
 `cpu_baseline` times the CPU oracle (a restatement of the reference's OpenMP
 CPU path) on a bounded sample, in a subprocess pinned to one thread.
+`batched` runs 64 uneven tiles x 100 realizations as graph replays and as
+per-tile calls (identical counters required).
 `surrogate` judges the packaged border-probability network (with its
 Monte Carlo check) against a 1000-realization Monte Carlo of the same DEM:
 per-cell deviation of the border probability and wall time of each.
@@ -77,6 +79,9 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=3,
                    help="realizations timed on the CPU oracle (0 = skip)")
     p.add_argument("--no-stage-times", action="store_true")
+    p.add_argument("--batch-tiles", type=int, default=64,
+                   help="tiles of the batched workload (0: skip)")
+    p.add_argument("--batch-realizations", type=int, default=100)
     p.add_argument("--surrogate-realizations", type=int, default=1000,
                    help="Monte Carlo size the surrogate is judged against (0: skip)")
     p.add_argument("--traffic", action="store_true",
@@ -185,6 +190,56 @@ def measure_traffic(args, kernel):
         return None, f"counter parsing failed: {e}"
     finally:
         shutil.rmtree(outdir, ignore_errors=True)
+
+
+def batched_leg(torch, ca, args):
+    """Batched form: `--batch-tiles` synthetic DEM tiles with sides drawn
+    U(256, 2048), per-tile error RMSE U(0.5, 3) m and practical range
+    U(30, 200) m (Gaussian filter), `--batch-realizations` each.  Autotuned,
+    then timed as one graph replay per realization of all tiles and as
+    per-tile blocking calls; the two must give identical counter rasters."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(args.seed + 3)
+    n = args.batch_tiles
+    shapes = [tuple(int(v) for v in rng.integers(256, 2049, size=2)) for _ in range(n)]
+    rmse = rng.uniform(0.5, 3.0, n)
+    pr = rng.uniform(30.0, 200.0, n)
+    with ThreadPoolExecutor(16) as ex:  # the generator is reentrant and releases the GIL
+        dems = list(ex.map(lambda k: ca.synth_dem(shapes[k][1], shapes[k][0], args.seed + 100 + k,
+                                                  10.0, 2000, NODATA), range(n)))
+    cells = sum(h * w for h, w in shapes)
+    dev = [(torch.from_numpy(d).cuda(), torch.from_numpy(s).cuda()) for d, s in dems]
+    out = {"tiles": n, "realizations": args.batch_realizations, "cells": cells,
+           "sides": "U(256,2048)", "filter": "gaussian"}
+    with ca.Batch(shapes) as b:
+        for t, (dd, ds) in enumerate(dev):
+            b.setup_tile(t, dd, ds, NODATA, NODATA, 2, ca.GAUSSIAN, float(pr[t]), 10.0, 0.0,
+                         float(rmse[t]), args.seed + 1000 + t)
+        scratch = [torch.zeros(sh, dtype=torch.float32, device="cuda") for sh in shapes]
+        t0 = time.perf_counter()
+        cfg = b.tune(scratch, reps=1)
+        torch.cuda.synchronize()
+        out["tune_s"] = round(time.perf_counter() - t0, 2)
+        out["tuned_workgroups"] = sorted(set(g for g, _ in cfg))
+        res = {}
+        for name, mode in [("graph", ca.BATCH_GRAPH), ("per_tile", ca.BATCH_PER_TILE)]:
+            mcs = [torch.zeros(sh, dtype=torch.float32, device="cuda") for sh in shapes]
+            b.run(mcs, 1, 0, mode)  # warm-up (graph build)
+            for m in mcs:
+                m.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.run(mcs, args.batch_realizations, 0, mode)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            res[name] = mcs
+            out[name] = {"s": round(dt, 3),
+                         "tile_realizations_per_s": round(n * args.batch_realizations / dt, 1),
+                         "mcells_per_s": round(cells * args.batch_realizations / dt / 1e6, 1)}
+        out["identical"] = all(torch.equal(a, c) for a, c in zip(res["graph"], res["per_tile"]))
+        out["graph_speedup_vs_per_tile"] = round(out["per_tile"]["s"] / out["graph"]["s"], 3)
+    return out
 
 
 def surrogate_leg(torch, ca, dem, stream, args):
@@ -327,6 +382,8 @@ def main():
                 "bound": "fp32-vector", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
                 "note": "k_conv_2d plus its output copy; no FMA (exact reference order)"}
+    if args.batch_tiles > 0:
+        line["batched"] = batched_leg(torch, ca, args)
     if args.surrogate_realizations > 0:
         line["surrogate"] = surrogate_leg(torch, ca, dem, stream, args)
     if args.cpu_sample > 0 and oracle_ok:

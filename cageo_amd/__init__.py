@@ -99,6 +99,7 @@ def lib():
         "cageo_batch_run": ([vp, vp, i, u32, i], i),
         "cageo_batch_tune": ([vp, vp, i, vp], i),
         "cageo_batch_configure_tile": ([vp, i, i, i], i),
+        "cageo_batch_info": ([vp, vp], i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -203,6 +204,13 @@ class Batch:
         out = (ctypes.c_int * (2 * len(self.shapes)))()
         _check(lib().cageo_batch_tune(self._h, self._ptrs(scratch_mcs), reps, out), "batch_tune")
         return [(out[2 * t], out[2 * t + 1]) for t in range(len(self.shapes))]
+
+    def info(self):
+        """Blocks over all tiles and the workgroups of the all-tile launches."""
+        out = (ctypes.c_int * 4)()
+        _check(lib().cageo_batch_info(self._h, out), "batch_info")
+        return {"blocks": out[0], "pit_fill_workgroups": out[1],
+                "flow_routing_workgroups": out[2], "flow_tracing_workgroups": out[3]}
 
     def configure_tile(self, t, persistent_workgroups=0, conv_rows=0):
         _check(lib().cageo_batch_configure_tile(self._h, t, persistent_workgroups, conv_rows),

@@ -67,6 +67,18 @@ void free_all(Context* c) {
 
 }  // namespace
 
+// Convolution, realistic DEM, burning and quantisation of the error field
+// in c->d_a into c->d_z (with evs: events evs[2], evs[3] after the first two).
+int realization_pre(Context* c, hipEvent_t* evs) {
+    float* e = nullptr;
+    CAGEO_TRY(launch_convolve_to(c, c->d_a, c->d_b, &e));
+    if (evs) CAGEO_HIP_TRY(hipEventRecord(evs[2], c->stream));
+    CAGEO_TRY(reset_minmax(c));
+    CAGEO_TRY(launch_realistic_burn_quant(c, c->dem, e, c->d_z));
+    if (evs) CAGEO_HIP_TRY(hipEventRecord(evs[3], c->stream));
+    return CAGEO_OK;
+}
+
 // One realization from the error field already in c->d_a (stages a3..a9).
 // With `evs` (8 events), evs[2..7] are recorded after each stage.
 int realization_body(Context* c, float* mc, hipEvent_t* evs) {
@@ -75,12 +87,8 @@ int realization_body(Context* c, float* mc, hipEvent_t* evs) {
         if (evs) CAGEO_HIP_TRY(hipEventRecord(evs[ev++], c->stream));
         return CAGEO_OK;
     };
-    float* e = nullptr;
-    CAGEO_TRY(launch_convolve_to(c, c->d_a, c->d_b, &e));
-    CAGEO_TRY(mark());
-    CAGEO_TRY(reset_minmax(c));
-    CAGEO_TRY(launch_realistic_burn_quant(c, c->dem, e, c->d_z));
-    CAGEO_TRY(mark());
+    CAGEO_TRY(realization_pre(c, evs));
+    ev = 4;
     CAGEO_TRY(pit_fill_quantized(c, c->d_z, c->d_w, c->nodata_dem, true));
     CAGEO_TRY(mark());
     CAGEO_TRY(flow_routing(c, c->d_w, c->d_fd));
@@ -183,7 +191,7 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
     A(cageo::alloc(&c->d_mark, c->ntiles));
     A(cageo::alloc(&c->d_topen, c->ntiles));
     A(cageo::alloc(&c->d_mark_pf, c->ntiles));
-    A(cageo::alloc(&c->d_work3, 3 * (size_t)c->ntiles));
+    A(cageo::alloc(&c->d_work3, 4 * (size_t)c->ntiles));
     A(cageo::alloc(&c->d_ctrl, cageo::CTL_WORDS));
     A(cageo::alloc(&c->d_bar, cageo::BAR_WORDS));
     A(cageo::alloc(&c->d_scal, cageo::SC_NSLOTS));

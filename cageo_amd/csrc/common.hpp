@@ -198,7 +198,43 @@ enum : int {
     CTL_EPOCH_FR = 13,  // epoch counter of the flat-resolution worklists
     CTL_FR_PASSES = 14, // flat-resolution passes of the last launch
     CTL_TR_ROUNDS = 15, // exit-jump rounds of the last tracing launch
-    CTL_WORDS = 16
+    CTL_FINAL = 16,     // count of the pit fill's final list
+    CTL_WORDS = 17
+};
+
+// Several rasters in one persistent launch: raster r owns the global 64x64
+// block numbers [base[r], base[r+1]), row-major within the raster;
+// bmap[g] = (raster, block number within it).
+//
+// Sweeps over every cell of every raster go by units of UNIT_ROWS rows x 64
+// columns of a block, one wave per unit, grid-strided: even shares whatever
+// the raster sizes, and the raster lookup is paid once per unit.
+constexpr int UNIT_ROWS = 4;
+constexpr int UNITS_PER_BLOCK = 64 / UNIT_ROWS;
+__device__ inline long long wave_unit_first() {
+    return (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+}
+__device__ inline long long wave_unit_stride() { return (long long)gridDim.x * (blockDim.x / 64); }
+
+// Device state of the multi-raster persistent launches of a batch (the
+// per-raster buffers stay in each raster's context).
+struct MultiState {
+    int nr = 0, total = 0;            // rasters, global 64x64 blocks
+    int max_cells = 0;                // largest raster (cells)
+    int* d_base = nullptr;            // nr + 1 block offsets
+    int2* d_bmap = nullptr;           // per block: (raster, block within it)
+    uint2* d_rinfo = nullptr;         // per raster: pit-fill minimum, domain ok
+    int* d_work = nullptr;            // 4 lists of `total` entries
+    uint32_t* d_mark_pf = nullptr;    // per-block epoch stamps (pit fill)
+    uint32_t* d_mark_fr = nullptr;    // per-block epoch stamps (flats)
+    uint32_t* d_topen = nullptr;      // per-block OPEN flag (pit fill)
+    uint32_t* d_ctrl = nullptr;       // CTL_WORDS
+    uint32_t* d_cert = nullptr;       // 3 rotating counters per raster
+    uint32_t* d_bar = nullptr;        // grid barrier words
+    void* d_pf = nullptr;             // raster tables of the three kernels
+    void* d_fr = nullptr;
+    void* d_tr = nullptr;
+    int pf_grid = 0, fr_grid = 0, tr_grid = 0;
 };
 
 // Worklist helpers --------------------------------------------------------
@@ -228,10 +264,9 @@ int check_status(Context* c);
 // One realization from the error field in c->d_a (stages a3..a9); with
 // `evs`, events evs[2..7] are recorded after each stage.
 int realization_body(Context* c, float* mc, hipEvent_t* evs);
+// Its first part: stages a3..a5 and the quantisation into c->d_z.
+int realization_pre(Context* c, hipEvent_t* evs);
 int pit_fill_grid_cap(Context* c, int* cap);
-const void* pit_fill_kernel();
-const void* flow_routing_kernel();
-const void* flow_tracing_kernel();
 int flow_routing_grid_cap(Context* c, int* cap);
 int flow_tracing_grid_cap(Context* c, int* cap);
 int clear_status(Context* c);
@@ -241,6 +276,17 @@ int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int3
                  int32_t* lab, int64_t* long_paths);
 int launch_border(Context* c, float* mc, const int32_t* lab);
 int launch_to_bytes(Context* c, uint8_t* out, const float* in);
+// Multi-raster launches over the contexts of a batch (tables built once by
+// multi_setup; each context's working buffers are used as in the one-raster
+// stages: pit fill d_z -> d_w, routing d_w -> d_fd, tracing d_fd -> d_lab).
+int multi_setup(MultiState* m, Context* const* ctxs, int nr);
+void multi_release(MultiState* m);
+int pit_fill_multi(MultiState* m, hipStream_t s);
+int flow_routing_multi(MultiState* m, hipStream_t s);
+int flow_tracing_multi(MultiState* m, hipStream_t s);
+int pit_fill_tables(MultiState* m, Context* const* ctxs);
+int flow_routing_tables(MultiState* m, Context* const* ctxs);
+int flow_tracing_tables(MultiState* m, Context* const* ctxs);
 
 }  // namespace cageo
 

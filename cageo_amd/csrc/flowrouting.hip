@@ -17,6 +17,8 @@
 // z_n <= z_c.  d is computed by monotone min-relaxation over 64x64 LDS
 // tiles (exact row solves by min-plus wave scans) inside one persistent
 // kernel, with the same worklist protocol as the pit fill.
+#include <vector>
+
 #include "tiles.hpp"
 
 namespace cageo {
@@ -47,34 +49,56 @@ __device__ inline void minplus_after(int& a, int& k, int pa, int pk) {
     a = min(a, sat_add(pa, k));
     k = sat_add(pk, k);
 }
+// Inclusive min-plus scans towards higher / lower lanes.
 __device__ inline void scan_minplus_up(int& a, int& k) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int pa = __shfl_up(a, d, 64), pk = __shfl_up(k, d, 64);
-        if (lane >= d) minplus_after(a, k, pa, pk);
-    }
+    CAGEO_SCAN_UP(dpp_i, a, k, D_INF, 0, minplus_after);
 }
 __device__ inline void scan_minplus_down(int& a, int& k) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int pa = __shfl_down(a, d, 64), pk = __shfl_down(k, d, 64);
-        if (lane + d < 64) minplus_after(a, k, pa, pk);
-    }
+    CAGEO_SCAN_DOWN(dpp_i, lane_read, int, a, k, D_INF, 0, minplus_after);
 }
 
-struct FrArgs {
+// One raster of a launch.
+struct FrRaster {
     const float* z;
     int8_t* fd;
     int32_t* dist;
-    int W, H, ntx, nty, ntiles;
-    int* work;
-    uint32_t* mark;
+    int W, H, ntx, nty;
+};
+
+// A launch over one raster (`one`, rs null) or over the nr rasters of a
+// table, whose 64x64 blocks are numbered globally (bmap).
+struct FrArgs {
+    FrRaster one;
+    const FrRaster* __restrict__ rs;
+    const int2* __restrict__ bmap;  // per block: (raster, block within it)
+    int nr, total;
+    int* work;      // 3 lists of `total` entries
+    uint32_t* mark; // per-block epoch stamps
     uint32_t* ctrl;
     uint32_t* bar;  // grid barrier words
     int max_passes;
 };
+
+struct FrBlock {
+    FrRaster R;
+    int lb, gbase;
+};
+template <bool MULTI>
+__device__ inline FrBlock fr_block(const FrArgs& a, int g) {
+    FrBlock b;
+    g = __builtin_amdgcn_readfirstlane(g);
+    if constexpr (!MULTI) {
+        b.R = a.one;
+        b.gbase = 0;
+    } else {
+        const int2 m = a.bmap[g];
+        const int r = __builtin_amdgcn_readfirstlane(m.x);
+        b.gbase = g - __builtin_amdgcn_readfirstlane(m.y);
+        b.R = a.rs[r];
+    }
+    b.lb = g - b.gbase;
+    return b;
+}
 
 
 __device__ inline uint32_t ld(const uint32_t* p) {
@@ -128,9 +152,10 @@ __device__ inline int32_t d8_cell(const float* __restrict__ z, int8_t* __restric
     return d;
 }
 
-__device__ void bfs_tile(FrShared& S, const FrArgs& a, int t, int* work_out, uint32_t* count_out,
-                         uint32_t epoch) {
-    const int tx = t % a.ntx, ty = t / a.ntx;
+__device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int* work_out,
+                         uint32_t* count_out, uint32_t epoch) {
+    const FrRaster& a = b.R;
+    const int tx = b.lb % a.ntx, ty = b.lb / a.ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
     for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
         const int hy = k / HALO_W, hx = k - hy * HALO_W;
@@ -172,7 +197,7 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& a, int t, int* work_out, uin
             const int2 c = S.dz[li];
             const float zc = __int_as_float(c.y);
             const bool isflat = (flat >> k) & 1u;
-            const float zl = __shfl_up(zc, 1, 64), zrt = __shfl_down(zc, 1, 64);
+            const float zl = lane_prev(zc), zrt = lane_next(zc);
             int a, kl = D_INF, kr = D_INF;
             if (isflat) {
                 a = c.x;
@@ -222,10 +247,13 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& a, int t, int* work_out, uin
     if ((threadIdx.x & 63) == 0 && bits) atomicOr(&S.bits, bits);
     __syncthreads();
     if (threadIdx.x == 0 && S.bits)
-        activate_tiles(S.bits, tx, ty, a.ntx, a.nty, work_out, count_out, a.mark, epoch);
+        activate_tiles(S.bits, tx, ty, a.ntx, a.nty, b.gbase, work_out, count_out, args.mark,
+                       epoch);
     __syncthreads();
 }
 
+// MULTI: a table of rasters (else the one raster of the arguments).
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
     __shared__ FrShared S;
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH_FR]);
@@ -236,21 +264,23 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
     }
     grid_sync(a.bar);
 
-    // D8 tile by tile; tiles holding flats open list 0.
-    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-        const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+    // D8 block by block; blocks holding flats open list 0.
+    for (int g = blockIdx.x; g < a.total; g += gridDim.x) {
+        const FrBlock b = fr_block<MULTI>(a, g);
+        const FrRaster& R = b.R;
+        const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
         const int x = x0 + (threadIdx.x & 63);
         int has_flat = 0;
-        if (x < a.W)
+        if (x < R.W)
             for (int k = 0; k < STRIP; ++k) {
                 const int y = y0 + (threadIdx.x >> 6) * STRIP + k;
-                if (y >= a.H) break;
-                const int32_t d = d8_cell(a.z, a.fd, x, y, a.W, a.H);
-                a.dist[(size_t)y * a.W + x] = d;
+                if (y >= R.H) break;
+                const int32_t d = d8_cell(R.z, R.fd, x, y, R.W, R.H);
+                R.dist[(size_t)y * R.W + x] = d;
                 has_flat |= d == D_INF;
             }
         if (__syncthreads_or(has_flat) && threadIdx.x == 0)
-            a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = t;
+            a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
     }
     grid_sync(a.bar);
 
@@ -266,10 +296,10 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         }
         ++epoch;
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_COUNT + cclr] = 0;
-        const int* win = a.work + (size_t)cin * a.ntiles;
-        int* wout = a.work + (size_t)cout * a.ntiles;
+        const int* win = a.work + (size_t)cin * a.total;
+        int* wout = a.work + (size_t)cout * a.total;
         for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
-            bfs_tile(S, a, win[i], wout, &a.ctrl[CTL_COUNT + cout], epoch);
+            bfs_tile(S, a, fr_block<MULTI>(a, win[i]), wout, &a.ctrl[CTL_COUNT + cout], epoch);
         grid_sync(a.bar);
         li = cout;
     }
@@ -281,24 +311,36 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         return;
     }
 
-    // Directions of the resolved flats.
-    const size_t n = (size_t)a.W * a.H;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int32_t d = a.dist[i];
-        if (d < 1 || d >= D_INF) continue;
-        const int x = (int)(i % a.W), y = (int)(i / a.W);
-        const float zc = a.z[i];
-        int sk = -1;
+    // Directions of the resolved flats, in units of UNIT_ROWS x 64 cells.
+    const long long units = (long long)a.total * UNITS_PER_BLOCK;
+    for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
+        const FrBlock b = fr_block<MULTI>(a, (int)(u / UNITS_PER_BLOCK));
+        const FrRaster& R = b.R;
+        const int x = (b.lb % R.ntx) * TILE + (threadIdx.x & 63);
+        const int y0 = (b.lb / R.ntx) * TILE + (int)(u % UNITS_PER_BLOCK) * UNIT_ROWS;
+        if (x >= R.W) continue;
+        int32_t dv[UNIT_ROWS];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int nx = x + FDX[k], ny = y + FDY[k];
-            if (nx < 0 || ny < 0 || nx >= a.W || ny >= a.H) continue;
-            const size_t ni = (size_t)ny * a.W + nx;
-            if (a.dist[ni] == d - 1 && a.z[ni] <= zc) sk = k;
+        for (int k = 0; k < UNIT_ROWS; ++k)
+            dv[k] = y0 + k < R.H ? R.dist[(size_t)(y0 + k) * R.W + x] : 0;
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k) {
+            const int y = y0 + k;
+            const int32_t d = dv[k];
+            if (y >= R.H || d < 1 || d >= D_INF) continue;
+            const size_t i = (size_t)y * R.W + x;
+            const float zc = R.z[i];
+            int sk = -1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int nx = x + FDX[q], ny = y + FDY[q];
+                if (nx < 0 || ny < 0 || nx >= R.W || ny >= R.H) continue;
+                const size_t ni = (size_t)ny * R.W + nx;
+                if (R.dist[ni] == d - 1 && R.z[ni] <= zc) sk = q;
+            }
+            if (sk >= 0)
+                reinterpret_cast<char2*>(R.fd)[i] = make_char2((int8_t)FDX[sk], (int8_t)FDY[sk]);
         }
-        if (sk >= 0)
-            reinterpret_cast<char2*>(a.fd)[i] = make_char2((int8_t)FDX[sk], (int8_t)FDY[sk]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_EPOCH_FR] = epoch;
@@ -306,16 +348,25 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
     }
 }
 
-}  // namespace
+FrRaster fr_raster(Context* c, const float* z, int8_t* fd) {
+    FrRaster R;
+    R.z = z;
+    R.fd = fd;
+    R.dist = c->d_dist;
+    R.W = c->W;
+    R.H = c->H;
+    R.ntx = c->ntx;
+    R.nty = c->nty;
+    return R;
+}
 
-// Device function of the persistent kernel (to find its node in a graph).
-const void* flow_routing_kernel() { return reinterpret_cast<const void*>(k_fr_persistent); }
+}  // namespace
 
 // Co-resident workgroups of the persistent kernel on this device.
 int flow_routing_grid_cap(Context* c, int* cap) {
     int per_cu = 0;
     CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k_fr_persistent), TILE_THREADS, 0));
+        &per_cu, reinterpret_cast<const void*>(k_fr_persistent<false>), TILE_THREADS, 0));
     hipDeviceProp_t prop;
     CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
     if (per_cu < 1) return CAGEO_ERR_HIP;
@@ -326,21 +377,54 @@ int flow_routing_grid_cap(Context* c, int* cap) {
 int flow_routing(Context* c, const float* z, int8_t* fd) {
     if (c->fr_grid <= 0) CAGEO_TRY(flow_routing_grid_cap(c, &c->fr_grid));
     FrArgs a;
-    a.z = z;
-    a.fd = fd;
-    a.dist = c->d_dist;
-    a.W = c->W;
-    a.H = c->H;
-    a.ntx = c->ntx;
-    a.nty = c->nty;
-    a.ntiles = c->ntiles;
+    a.one = fr_raster(c, z, fd);
+    a.rs = nullptr;
+    a.bmap = nullptr;
+    a.nr = 1;
+    a.total = c->ntiles;
     a.work = c->d_work3;
     a.mark = c->d_mark;
     a.ctrl = c->d_ctrl;
     a.bar = c->d_bar;
     a.max_passes = 1 << 20;
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
-    hipLaunchKernelGGL(k_fr_persistent, dim3(c->fr_grid), dim3(TILE_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(k_fr_persistent<false>, dim3(c->fr_grid), dim3(TILE_THREADS), 0, c->stream,
+                       a);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+// Raster table of a batch: each context's d_w -> d_fd.
+int flow_routing_tables(MultiState* m, Context* const* ctxs) {
+    std::vector<FrRaster> t(m->nr);
+    for (int r = 0; r < m->nr; ++r) t[r] = fr_raster(ctxs[r], ctxs[r]->d_w, ctxs[r]->d_fd);
+    CAGEO_HIP_TRY(hipMalloc(&m->d_fr, t.size() * sizeof(FrRaster)));
+    CAGEO_HIP_TRY(hipMemcpy(m->d_fr, t.data(), t.size() * sizeof(FrRaster), hipMemcpyHostToDevice));
+    if (m->fr_grid <= 0) {
+        int per_cu = 0;
+        CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(k_fr_persistent<true>), TILE_THREADS, 0));
+        hipDeviceProp_t prop;
+        CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, ctxs[0]->device));
+        if (per_cu < 1) return CAGEO_ERR_HIP;
+        m->fr_grid = per_cu * prop.multiProcessorCount;
+    }
+    return CAGEO_OK;
+}
+
+int flow_routing_multi(MultiState* m, hipStream_t s) {
+    FrArgs a;
+    a.one = FrRaster{};
+    a.rs = static_cast<const FrRaster*>(m->d_fr);
+    a.bmap = m->d_bmap;
+    a.nr = m->nr;
+    a.total = m->total;
+    a.work = m->d_work;
+    a.mark = m->d_mark_fr;
+    a.ctrl = m->d_ctrl;
+    a.bar = m->d_bar;
+    a.max_passes = 1 << 20;
+    hipLaunchKernelGGL(k_fr_persistent<true>, dim3(m->fr_grid), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }

@@ -19,6 +19,8 @@
 // than 100 001 steps away (where the reference's result depends on its
 // raster order) are counted.  performConnectFlowPath (:115-134) then maps a
 // label >= 0 to label[label] when that is negative.
+#include <vector>
+
 #include "common.hpp"
 
 namespace cageo {
@@ -128,51 +130,114 @@ __device__ inline uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-struct TrArgs {
+// One raster of a launch.
+struct TrRaster {
     const int8_t* fd;
     const uint8_t* sf;
     const int32_t* slabel;
-    uint64_t* ent[2];
+    uint64_t* ent0;  // entries, ping-pong pair
+    uint64_t* ent1;
     int32_t* lab;
-    int W, H, ntx, ntiles;
-    int max_rounds;
-    uint32_t* ctrl;
-    uint32_t* bar;  // grid barrier words
+    int W, H, ntx;
     uint32_t* scal;
 };
 
+// A launch over one raster (`one`, rs null) or over the nr rasters of a
+// table, whose 64x64 blocks are numbered globally (bmap).
+struct TrArgs {
+    TrRaster one;
+    const TrRaster* __restrict__ rs;
+    const int2* __restrict__ bmap;  // per block: (raster, block within it)
+    int nr, total;
+    int max_rounds;
+    uint32_t* ctrl;
+    uint32_t* bar;  // grid barrier words
+};
+
+struct TrBlock {
+    TrRaster R;
+    int lb;
+};
+template <bool MULTI>
+__device__ inline TrBlock tr_block(const TrArgs& a, int g) {
+    TrBlock b;
+    g = __builtin_amdgcn_readfirstlane(g);
+    if constexpr (!MULTI) {
+        b.R = a.one;
+        b.lb = g;
+    } else {
+        const int2 m = a.bmap[g];
+        b.R = a.rs[__builtin_amdgcn_readfirstlane(m.x)];
+        b.lb = __builtin_amdgcn_readfirstlane(m.y);
+    }
+    return b;
+}
+
+// A unit of UNIT_ROWS x 64 cells: the raster, the wave's column x, the first
+// row y0 and how many of the unit's rows lie in the raster (0 when the
+// wave's column is outside it).
+struct TrUnit {
+    TrRaster R;
+    int x, y0, rows;
+};
+template <bool MULTI>
+__device__ inline TrUnit tr_unit(const TrArgs& a, long long u) {
+    const TrBlock b = tr_block<MULTI>(a, (int)(u / UNITS_PER_BLOCK));
+    TrUnit t;
+    t.R = b.R;
+    t.x = (b.lb % b.R.ntx) * TILE + (threadIdx.x & 63);
+    t.y0 = (b.lb / b.R.ntx) * TILE + (int)(u % UNITS_PER_BLOCK) * UNIT_ROWS;
+    t.rows = t.x < b.R.W ? min(UNIT_ROWS, b.R.H - t.y0) : 0;
+    return t;
+}
+
 // Tile pass, exit jumping, labels and the connect step in one persistent
-// launch.  Round r of the exit jumping raises flag r % 3 while exits remain
-// and clears flag (r+1) % 3, whose last read (end of round r-2) is separated
-// from the clear by barriers.
+// launch over every raster.  Sweeps over all cells go by units of UNIT_ROWS
+// x 64 cells, one wave each, with each unit's loads issued before its
+// stores.  Round r of the exit jumping raises flag r % 3 while exits remain
+// and clears flag (r+1) % 3, whose last read (end of round r-2) is
+// separated from the clear by barriers.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
     __shared__ uint64_t L[TILE * TILE];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_FLAG + 0] = 0;
         a.ctrl[CTL_FLAG + 1] = 0;
         a.ctrl[CTL_FLAG + 2] = 0;
-        a.scal[SC_LONGPATH] = 0;
     }
-    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x)
-        trace_tile(L, a.fd, a.sf, a.slabel, a.ent[0], a.W, a.H, a.ntx, t);
+    for (int g = blockIdx.x; g < a.total; g += gridDim.x) {
+        const TrBlock b = tr_block<MULTI>(a, g);
+        const TrRaster& R = b.R;
+        if (b.lb == 0 && threadIdx.x == 0) R.scal[SC_LONGPATH] = 0;
+        trace_tile(L, R.fd, R.sf, R.slabel, R.ent0, R.W, R.H, R.ntx, b.lb);
+    }
     grid_sync(a.bar);
 
-    const size_t n = (size_t)a.W * a.H;
+    const long long units = (long long)a.total * UNITS_PER_BLOCK;
     int cur = 0;
     int rounds = 0;
     for (int r = 0; r < a.max_rounds; ++r) {
         rounds = r + 1;
-        const uint64_t* src = a.ent[cur];
-        uint64_t* dst = a.ent[cur ^ 1];
         uint32_t pending = 0;
-        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-             i += (size_t)gridDim.x * blockDim.x) {
-            uint64_t e = src[i];
-            if (kind_of(e) == K_EXIT) {
-                e = hop(e, src[value_of(e)]);
-                pending |= kind_of(e) == K_EXIT;
+        for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
+            const TrUnit t = tr_unit<MULTI>(a, u);
+            const uint64_t* src = cur ? t.R.ent1 : t.R.ent0;
+            uint64_t* dst = cur ? t.R.ent0 : t.R.ent1;
+            uint64_t e[UNIT_ROWS], f[UNIT_ROWS];
+#pragma unroll
+            for (int k = 0; k < UNIT_ROWS; ++k)
+                e[k] = k < t.rows ? src[(size_t)(t.y0 + k) * t.R.W + t.x] : pack(K_STUCK, 0, 0);
+#pragma unroll
+            for (int k = 0; k < UNIT_ROWS; ++k) f[k] = kind_of(e[k]) == K_EXIT ? src[value_of(e[k])] : 0;
+#pragma unroll
+            for (int k = 0; k < UNIT_ROWS; ++k) {
+                if (k >= t.rows) continue;
+                if (kind_of(e[k]) == K_EXIT) {
+                    e[k] = hop(e[k], f[k]);
+                    pending |= kind_of(e[k]) == K_EXIT;
+                }
+                dst[(size_t)(t.y0 + k) * t.R.W + t.x] = e[k];
             }
-            dst[i] = e;
         }
         if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&a.ctrl[CTL_FLAG + r % 3], 1u);
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_FLAG + (r + 1) % 3] = 0;
@@ -181,45 +246,79 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
     }
 
-    uint32_t longp = 0;
-    const uint64_t* fin = a.ent[cur];
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t e = fin[i];
-        int32_t v = 0;
-        if (kind_of(e) == K_RESOLVED) {
-            v = value_of(e);
-            if (steps_of(e) > 100001u) ++longp;
+    for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
+        const TrUnit t = tr_unit<MULTI>(a, u);
+        const uint64_t* fin = cur ? t.R.ent1 : t.R.ent0;
+        uint64_t e[UNIT_ROWS];
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k)
+            e[k] = k < t.rows ? fin[(size_t)(t.y0 + k) * t.R.W + t.x] : pack(K_STUCK, 0, 0);
+        uint32_t longp = 0;
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k) {
+            if (k >= t.rows) continue;
+            int32_t v = 0;
+            if (kind_of(e[k]) == K_RESOLVED) {
+                v = value_of(e[k]);
+                if (steps_of(e[k]) > 100001u) ++longp;  // past the reference's walk guard
+            }
+            t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] = v;
         }
-        a.lab[i] = v;
+        if (__any(longp)) {
+            for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&t.R.scal[SC_LONGPATH], longp);
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
-    if ((threadIdx.x & 63) == 0 && longp) atomicAdd(&a.scal[SC_LONGPATH], longp);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
     grid_sync(a.bar);
     // performConnectFlowPath: label[0] is final and never changes here.
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int32_t v = a.lab[i];
-        if (v >= 0 && (size_t)v < n) {
-            const int32_t t = a.lab[v];
-            if (t < 0) a.lab[i] = t;
-        }
+    for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
+        const TrUnit t = tr_unit<MULTI>(a, u);
+        const size_t n = (size_t)t.R.W * t.R.H;
+        int32_t v[UNIT_ROWS], w[UNIT_ROWS];
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k)
+            v[k] = k < t.rows ? t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] : -1;
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k) w[k] = (v[k] >= 0 && (size_t)v[k] < n) ? t.R.lab[v[k]] : 0;
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k)
+            if (k < t.rows && v[k] >= 0 && (size_t)v[k] < n && w[k] < 0)
+                t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] = w[k];
     }
+}
+
+int max_rounds_for(size_t cells) {
+    int m = 2;
+    for (size_t k = 1; k < cells; k <<= 1) ++m;
+    return m;
+}
+
+TrRaster tr_raster(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,
+                   int32_t* lab) {
+    TrRaster R;
+    R.fd = fd;
+    R.sf = sflags;
+    R.slabel = slabel;
+    R.ent0 = c->d_ptr;
+    R.ent1 = c->d_ptr2;
+    R.lab = lab;
+    R.W = c->W;
+    R.H = c->H;
+    R.ntx = c->ntx;
+    R.scal = c->d_scal;
+    return R;
 }
 
 }  // namespace
 
 // Needs two 64-bit rasters: c->d_ptr and c->d_ptr2.  The long-path count is
 // left in the scalar block.
-// Device function of the persistent kernel (to find its node in a graph).
-const void* flow_tracing_kernel() { return reinterpret_cast<const void*>(k_tr_persistent); }
-
 // Co-resident workgroups of the persistent kernel on this device.
 int flow_tracing_grid_cap(Context* c, int* cap) {
     int per_cu = 0;
     CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k_tr_persistent), TILE_THREADS, 0));
+        &per_cu, reinterpret_cast<const void*>(k_tr_persistent<false>), TILE_THREADS, 0));
     hipDeviceProp_t prop;
     CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
     if (per_cu < 1) return CAGEO_ERR_HIP;
@@ -231,29 +330,59 @@ int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int3
                  int32_t* lab, int64_t* long_paths) {
     if (c->tr_grid <= 0) CAGEO_TRY(flow_tracing_grid_cap(c, &c->tr_grid));
     TrArgs a;
-    a.fd = fd;
-    a.sf = sflags;
-    a.slabel = slabel;
-    a.ent[0] = c->d_ptr;
-    a.ent[1] = c->d_ptr2;
-    a.lab = lab;
-    a.W = c->W;
-    a.H = c->H;
-    a.ntx = c->ntx;
-    a.ntiles = c->ntiles;
-    int max_rounds = 2;
-    for (size_t m = 1; m < c->n; m <<= 1) ++max_rounds;
-    a.max_rounds = max_rounds;
+    a.one = tr_raster(c, fd, sflags, slabel, lab);
+    a.rs = nullptr;
+    a.bmap = nullptr;
+    a.nr = 1;
+    a.total = c->ntiles;
+    a.max_rounds = max_rounds_for(c->n);
     a.ctrl = c->d_ctrl;
     a.bar = c->d_bar;
-    a.scal = c->d_scal;
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
-    hipLaunchKernelGGL(k_tr_persistent, dim3(c->tr_grid), dim3(TILE_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(k_tr_persistent<false>, dim3(c->tr_grid), dim3(TILE_THREADS), 0, c->stream,
+                       a);
     CAGEO_HIP_TRY(hipGetLastError());
     if (long_paths) {
         CAGEO_TRY(read_scalars(c));
         *long_paths = (int64_t)c->h_scal[SC_LONGPATH];
     }
+    return CAGEO_OK;
+}
+
+// Raster table of a batch: each context's d_fd -> d_lab with its stream
+// flags and labels.
+int flow_tracing_tables(MultiState* m, Context* const* ctxs) {
+    std::vector<TrRaster> t(m->nr);
+    for (int r = 0; r < m->nr; ++r) {
+        Context* c = ctxs[r];
+        t[r] = tr_raster(c, c->d_fd, c->d_sflags, c->d_slabel, c->d_lab);
+    }
+    CAGEO_HIP_TRY(hipMalloc(&m->d_tr, t.size() * sizeof(TrRaster)));
+    CAGEO_HIP_TRY(hipMemcpy(m->d_tr, t.data(), t.size() * sizeof(TrRaster), hipMemcpyHostToDevice));
+    if (m->tr_grid <= 0) {
+        int per_cu = 0;
+        CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(k_tr_persistent<true>), TILE_THREADS, 0));
+        hipDeviceProp_t prop;
+        CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, ctxs[0]->device));
+        if (per_cu < 1) return CAGEO_ERR_HIP;
+        m->tr_grid = per_cu * prop.multiProcessorCount;
+    }
+    return CAGEO_OK;
+}
+
+int flow_tracing_multi(MultiState* m, hipStream_t s) {
+    TrArgs a;
+    a.one = TrRaster{};
+    a.rs = static_cast<const TrRaster*>(m->d_tr);
+    a.bmap = m->d_bmap;
+    a.nr = m->nr;
+    a.total = m->total;
+    a.max_rounds = max_rounds_for((size_t)m->max_cells);
+    a.ctrl = m->d_ctrl;
+    a.bar = m->d_bar;
+    hipLaunchKernelGGL(k_tr_persistent<true>, dim3(m->tr_grid), dim3(TILE_THREADS), 0, s, a);
+    CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
 

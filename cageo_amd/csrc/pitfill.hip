@@ -37,6 +37,8 @@
 //
 // Every comparison is on floats; the only arithmetic is the quantisation and
 // the round-trip test, both evaluated exactly as the reference does.
+#include <vector>
+
 #include "tiles.hpp"
 
 namespace cageo {
@@ -97,25 +99,83 @@ __device__ inline uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-struct PfArgs {
+// One raster of a launch.
+struct PfRaster {
     const float* zq;
     float* w;
     uint8_t* st;
-    int W, H, ntx, nty, ntiles;
+    int W, H, ntx, nty;
     float nodata;
-    int* work;            // 3 lists of ntiles entries
-    uint32_t* mark;       // per-tile epoch stamps
-    uint32_t* topen;      // per-tile: tile holds OPEN cells
+    const uint32_t* __restrict__ scal;  // min/max/bad of the quantized raster
+    uint32_t* status;      // the raster's sticky status word
+};
+
+// A launch over one raster (`one`, rs null) or over the nr rasters of a
+// table, whose 64x64 blocks are numbered globally (bmap).
+struct PfArgs {
+    PfRaster one;
+    const PfRaster* __restrict__ rs;
+    const int2* __restrict__ bmap;  // per block: (raster, block within it)
+    uint2* rinfo;                   // per raster: minimum (float bits), domain ok
+    int nr, total;
+    int* work;            // 4 lists of `total` entries: 3 rotating, 1 final
+    uint32_t* mark;       // per-block epoch stamps
+    uint32_t* topen;      // per-block: block holds OPEN cells
     uint32_t* ctrl;       // control block (CTL_*)
+    uint32_t* cert;       // per raster: 3 rotating per-round fixed-cell counters
     uint32_t* bar;        // grid barrier words
-    const uint32_t* scal; // min/max/bad of the quantized raster
     int max_passes;
 };
+
+// The raster of global block g, g's number within it, and the raster's
+// minimum and domain flag (`mn1`, `ok1` for a one-raster launch; the
+// per-raster words of the table once the launch has written them).
+struct PfBlock {
+    PfRaster R;
+    int r, lb, gbase;
+    float mn;
+    bool ok;
+};
+template <bool MULTI>
+__device__ inline PfBlock pf_block(const PfArgs& a, int g, float mn1, bool ok1) {
+    PfBlock b;
+    g = __builtin_amdgcn_readfirstlane(g);
+    if constexpr (!MULTI) {
+        b.R = a.one;
+        b.r = 0;
+        b.lb = g;
+        b.mn = mn1;
+        b.ok = ok1;
+    } else {
+        const int2 m = a.bmap[g];
+        b.r = __builtin_amdgcn_readfirstlane(m.x);
+        b.lb = __builtin_amdgcn_readfirstlane(m.y);
+        b.R = a.rs[b.r];
+        const uint2 inf = a.rinfo[b.r];
+        b.mn = __uint_as_float(__builtin_amdgcn_readfirstlane(inf.x));
+        b.ok = __builtin_amdgcn_readfirstlane(inf.y) != 0u;
+    }
+    b.gbase = g - b.lb;
+    return b;
+}
+
+// Domain of the reference's arithmetic: non-negative elevations and a bucket
+// array fl(fl(max-min)*1000)+1 that float arithmetic indexes exactly (beyond
+// 2^24 the reference reads past it).  Sets the raster's minimum.
+// The scalars were written by an earlier launch and are read-only here, so
+// plain (cacheable) loads are exact.
+__device__ inline bool pf_raster_ok(const PfRaster& R, float& mn) {
+    const uint32_t bad = R.scal[SC_BAD];
+    mn = ordered_to_float(R.scal[SC_MIN]);
+    const float mx = ordered_to_float(R.scal[SC_MAX]);
+    const float span = (mx - mn) * 1000.0f;
+    return !bad && span < 16777216.0f;
+}
 
 // Loads the tile's relays with a one-cell halo.  The state byte and the
 // value of a cell are loaded together (no load waits on another), over a
 // fully unrolled loop, so the loads of all iterations are in flight at once.
-__device__ inline void load_tile(PfShared& S, const PfArgs& a, int x0, int y0) {
+__device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0) {
     constexpr int N = HALO_W * HALO_W;
     constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
     constexpr int CHUNK = (ITER + 1) / 2;  // two batches keep the register use down
@@ -156,7 +216,7 @@ __device__ inline void load_tile(PfShared& S, const PfArgs& a, int x0, int y0) {
 }
 
 // Ends a tile: OR of border bits, tile-open flag, neighbour activation.
-__device__ inline void finish_tile(PfShared& S, const PfArgs& a, int t, uint32_t bits,
+__device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& b, uint32_t bits,
                                    uint32_t open_left, uint32_t nfixed, uint32_t* cert_ctr,
                                    int* work_out, uint32_t* count_out, uint32_t epoch) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -171,10 +231,10 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, int t, uint32_t
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        a.topen[t] = S.any;
+        a.topen[b.gbase + b.lb] = S.any;
         if (S.bits)
-            activate_tiles(S.bits, t % a.ntx, t / a.ntx, a.ntx, a.nty, work_out, count_out, a.mark,
-                           epoch);
+            activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
+                           work_out, count_out, a.mark, epoch);
     }
     __syncthreads();
 }
@@ -183,9 +243,10 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, int t, uint32_t
 // lives in LDS (relays in rel, Z in zt); each thread keeps bit masks of its
 // strip's cells only, so the row loop can run in either direction without
 // per-thread arrays.
-__device__ void fill_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_t* cert_ctr,
-                          int* work_out, uint32_t* count_out, uint32_t epoch) {
-    const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+__device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch) {
+    const PfRaster& a = b.R;
+    const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
@@ -239,8 +300,8 @@ __device__ void fill_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
             const float wcur = S.rel[li].x;
             const bool zlive = op && pf_live(z, mn);
             const bool seed = op && min8y(S.rel, li) <= z;
-            const float zl = __shfl_up(z, 1, 64), zrt = __shfl_down(z, 1, 64);
-            const int livel = __shfl_up((int)zlive, 1, 64), liver = __shfl_down((int)zlive, 1, 64);
+            const float zl = lane_prev(z), zrt = lane_next(z);
+            const int livel = lane_prev((int)zlive), liver = lane_next((int)zlive);
             const unsigned long long sd = __ballot(seed);
             unsigned long long f = sd;
             if (sd)
@@ -302,13 +363,14 @@ __device__ void fill_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
         }
         if ((lowered | fixed) & (1u << k)) bits |= border_bits(lx, ly);
     }
-    finish_tile(S, a, t, bits, open, nfixed, cert_ctr, work_out, count_out, epoch);
+    finish_tile(S, args, b, bits, open, nfixed, cert_ctr, work_out, count_out, epoch);
 }
 
 // Certification of one tile after the fill has converged.
-__device__ void cert_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_t* cert_ctr,
-                          int* work_out, uint32_t* count_out, uint32_t epoch) {
-    const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
+__device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch) {
+    const PfRaster& a = b.R;
+    const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
@@ -348,8 +410,8 @@ __device__ void cert_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
             const bool live = op && pf_live(w, mn);
             const bool seed = op && min8y(S.rel, li) <= w;
             // from the left neighbour x-1 (open, live) into x, and from x+1
-            const float wl = __shfl_up(w, 1, 64), wrt = __shfl_down(w, 1, 64);
-            const int livel = __shfl_up((int)live, 1, 64), liver = __shfl_down((int)live, 1, 64);
+            const float wl = lane_prev(w), wrt = lane_next(w);
+            const int livel = lane_prev((int)live), liver = lane_next((int)live);
             const bool gl = op && lx > 0 && livel && wl <= w;
             const bool gr = op && lx < TILE - 1 && liver && wrt <= w;
             const unsigned long long sd = __ballot(seed);
@@ -384,15 +446,16 @@ __device__ void cert_tile(PfShared& S, const PfArgs& a, int t, float mn, uint32_
             if (fixed_live & (1u << k)) bits |= border_bits(lx, ly);
         }
     }
-    finish_tile(S, a, t, bits, open | open_inf, nfixed, cert_ctr, work_out, count_out, epoch);
+    finish_tile(S, args, b, bits, open | open_inf, nfixed, cert_ctr, work_out, count_out, epoch);
 }
 
 // Runs passes of `kind` (0 fill, 1 certify) from worklist `li` until a pass
 // lists nothing; returns the index of the (empty) list it stopped at, or -1
-// when the pass cap is hit.
-template <int KIND>
-__device__ int run_passes(PfShared& S, const PfArgs& a, int li, float mn, uint32_t* cert_ctr,
-                          uint32_t& epoch) {
+// when the pass cap is hit.  Fixed cells count into counter `slot` of their
+// raster.
+template <int KIND, bool MULTI>
+__device__ int run_passes(PfShared& S, const PfArgs& a, int li, int slot, uint32_t& epoch,
+                          float mn1, bool ok1) {
     for (int pass = 0; pass < a.max_passes; ++pass) {
         const int cin = li, cout = (li + 1) % 3, cclr = (li + 2) % 3;
         const uint32_t count = ld(&a.ctrl[CTL_COUNT + cin]);
@@ -402,14 +465,16 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, float mn, uint32
             a.ctrl[CTL_COUNT + cclr] = 0;
             atomicAdd(&a.ctrl[CTL_PASSES], 1u);
         }
-        const int* win = a.work + (size_t)cin * a.ntiles;
-        int* wout = a.work + (size_t)cout * a.ntiles;
+        const int* win = a.work + (size_t)cin * a.total;
+        int* wout = a.work + (size_t)cout * a.total;
         for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
-            const int t = win[i];
+            const PfBlock b = pf_block<MULTI>(a, win[i], mn1, ok1);
+            if (!b.ok) continue;  // uniform; such blocks never open
+            uint32_t* cert_ctr = &a.cert[3 * b.r + slot];
             if (KIND == 0)
-                fill_tile(S, a, t, mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
+                fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
             else
-                cert_tile(S, a, t, mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
+                cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
         }
         grid_sync(a.bar);
         li = cout;
@@ -417,67 +482,129 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, float mn, uint32
     return -1;
 }
 
-// Lists the tiles flagged open into list (b+1) % 3 and returns its index.
+// Lists the blocks flagged open into list (b+1) % 3 and returns its index.
 // `b` is the list run_passes stopped at: some workgroups may still be reading
 // its (zero) count, so it is left untouched; (b+1) % 3 was cleared during the
 // last pass and nobody reads it; (b+2) % 3 was last read before the last
-// barrier and is cleared here for the first pass of the next phase.  The
+// barrier and is cleared here for the first pass of the next phase.
+// With `slot` >= 0 (after certification), open blocks of a raster whose
+// round fixed nothing (counter `slot` zero) go to the final list instead:
+// that raster is done.  Without, the final list's count is cleared (its last
+// reads were before the barrier that ends the round's restart phase).  The
 // caller ends with a grid barrier.
-__device__ inline int build_open_list(const PfArgs& a, int b) {
+template <bool MULTI>
+__device__ inline int build_open_list(const PfArgs& a, int b, int slot) {
     const int l = (b + 1) % 3;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_COUNT + (b + 2) % 3] = 0;
-    int* out = a.work + (size_t)l * a.ntiles;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.ntiles; t += gridDim.x * blockDim.x)
-        if (a.topen[t]) out[atomicAdd(&a.ctrl[CTL_COUNT + l], 1u)] = t;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl[CTL_COUNT + (b + 2) % 3] = 0;
+        if (slot < 0) a.ctrl[CTL_FINAL] = 0;
+    }
+    int* out = a.work + (size_t)l * a.total;
+    int* fin = a.work + (size_t)3 * a.total;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x) {
+        if (!a.topen[g]) continue;
+        if (slot >= 0) {
+            const int r = MULTI ? a.bmap[g].x : 0;
+            if (ld(&a.cert[3 * r + slot]) == 0) {
+                fin[atomicAdd(&a.ctrl[CTL_FINAL], 1u)] = g;
+                continue;
+            }
+        }
+        out[atomicAdd(&a.ctrl[CTL_COUNT + l], 1u)] = g;
+    }
     return l;
 }
 
+// OPEN cells of global block g: restart at +inf (`value` false) or keep
+// their quantized elevation (`value` true: never reached).
+template <bool MULTI>
+__device__ inline void open_cells_set(const PfArgs& a, int g, bool to_z) {
+    const PfBlock b = pf_block<MULTI>(a, g, 0.0f, true);
+    const PfRaster& R = b.R;
+    const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
+    const int gx = x0 + (threadIdx.x & 63);
+    if (gx >= R.W) return;
+    for (int k = 0; k < STRIP; ++k) {
+        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
+        if (gy >= R.H) break;
+        const size_t gi = (size_t)gy * R.W + gx;
+        if (R.st[gi] == PF_OPEN) R.w[gi] = to_z ? R.zq[gi] : PF_INF;
+    }
+}
+
+// MULTI: a table of rasters (else the one raster of the arguments; two
+// instantiations keep the one-raster code free of table lookups).
+template <bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pf_persistent(PfArgs a) {
     __shared__ PfShared S;
-    const uint32_t* scal = a.scal;
-    const uint32_t bad = ld(&scal[SC_BAD]);
-    const float mn = ordered_to_float(ld(&scal[SC_MIN]));
-    const float mx = ordered_to_float(ld(&scal[SC_MAX]));
-    // Domain of the reference's arithmetic: non-negative elevations and a
-    // bucket array fl(fl(max-min)*1000)+1 that float arithmetic indexes
-    // exactly (beyond 2^24 the reference reads past it).
-    const float span = (mx - mn) * 1000.0f;
-    if (bad || !(span < 16777216.0f)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_STATUS] = (uint32_t)(-CAGEO_ERR_RANGE);
-        return;  // uniform: every workgroup takes this branch
-    }
 
-    // initial states (PitFillingAlgorithmCPUOpenMP.h:121-144 seeds) and the
-    // all-tiles worklist
-    const size_t n = (size_t)a.W * a.H;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % a.W), y = (int)(i / a.W);
-        const float z = a.zq[i];
-        uint8_t s;
-        float v;
-        if (z == a.nodata) {
-            s = PF_NODATA;
-            v = z;
-        } else if (x == 0 || y == 0 || x == a.W - 1 || y == a.H - 1 || z == 0.0f) {
-            s = pf_live(z, mn) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
-            v = z;
-        } else {
-            s = PF_OPEN;
-            v = PF_INF;
+    // One raster: its minimum and domain flag, once.
+    float mn1 = 0.0f;
+    bool ok1 = true;
+    if (!MULTI) ok1 = pf_raster_ok(a.one, mn1);
+
+    // Initial states (PitFillingAlgorithmCPUOpenMP.h:121-144 seeds) in units
+    // of UNIT_ROWS x 64 cells, one wave each.  A raster outside the domain
+    // gets its status and no work: its blocks are skipped and never open.
+    {
+        const int lane = threadIdx.x & 63;
+        const long long units = (long long)a.total * UNITS_PER_BLOCK;
+        for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
+            const int g = (int)(u / UNITS_PER_BLOCK);
+            const int row0 = (int)(u % UNITS_PER_BLOCK) * UNIT_ROWS;
+            const PfBlock b = pf_block<MULTI>(a, g, mn1, ok1);
+            const PfRaster& R = b.R;
+            float mn = mn1;
+            bool ok = ok1;
+            if (MULTI) {
+                ok = pf_raster_ok(R, mn);  // the per-raster words are written here
+                if (b.lb == 0 && row0 == 0 && lane == 0)
+                    a.rinfo[b.r] = make_uint2(__float_as_uint(mn), ok ? 1u : 0u);
+            }
+            if (!ok) {
+                if (row0 == 0 && lane == 0) {
+                    a.topen[g] = 0;
+                    if (b.lb == 0) *R.status = (uint32_t)(-CAGEO_ERR_RANGE);
+                }
+                continue;
+            }
+            const int x = (b.lb % R.ntx) * TILE + lane, y0 = (b.lb / R.ntx) * TILE + row0;
+            float z[UNIT_ROWS];
+#pragma unroll
+            for (int k = 0; k < UNIT_ROWS; ++k)
+                z[k] = (x < R.W && y0 + k < R.H) ? R.zq[(size_t)(y0 + k) * R.W + x] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < UNIT_ROWS; ++k) {
+                const int y = y0 + k;
+                if (x >= R.W || y >= R.H) continue;
+                uint8_t s;
+                float v;
+                if (z[k] == R.nodata) {
+                    s = PF_NODATA;
+                    v = z[k];
+                } else if (x == 0 || y == 0 || x == R.W - 1 || y == R.H - 1 || z[k] == 0.0f) {
+                    s = pf_live(z[k], mn) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+                    v = z[k];
+                } else {
+                    s = PF_OPEN;
+                    v = PF_INF;
+                }
+                const size_t i = (size_t)y * R.W + x;
+                R.st[i] = s;
+                R.w[i] = v;
+            }
         }
-        a.st[i] = s;
-        a.w[i] = v;
     }
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.ntiles; t += gridDim.x * blockDim.x)
-        a.work[t] = t;
+    // list 0: every block (those of rasters outside the domain are skipped)
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x)
+        a.work[g] = g;
+    for (int k = threadIdx.x; k < 3 * a.nr; k += blockDim.x)
+        if (blockIdx.x == 0) a.cert[k] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctrl[CTL_COUNT + 0] = (uint32_t)a.ntiles;
+        a.ctrl[CTL_COUNT + 0] = (uint32_t)a.total;
         a.ctrl[CTL_COUNT + 1] = 0;
         a.ctrl[CTL_COUNT + 2] = 0;
-        a.ctrl[CTL_CERT + 0] = 0;
-        a.ctrl[CTL_CERT + 1] = 0;
-        a.ctrl[CTL_CERT + 2] = 0;
+        a.ctrl[CTL_FINAL] = 0;
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
@@ -486,57 +613,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     int li = 0;
     int round = 0;
     for (;; ++round) {
-        // Round r counts into counter r % 3 and clears counter (r+1) % 3, whose
-        // last read (end of round r-2) is separated from here by barriers.
-        uint32_t* cert_ctr = &a.ctrl[CTL_CERT + round % 3];
+        // Round r counts into counter slot r % 3 of each raster and clears slot
+        // (r+1) % 3, whose last read (end of round r-2) is separated from here
+        // by barriers.
+        const int slot = round % 3;
         if (round > 0) {
-            // OPEN cells of the listed tiles restart from +inf
+            // OPEN cells of the listed blocks restart from +inf
             const uint32_t count = ld(&a.ctrl[CTL_COUNT + li]);
-            const int* lst = a.work + (size_t)li * a.ntiles;
-            for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
-                const int t = lst[i];
-                const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
-                const int gx = x0 + (threadIdx.x & 63);
-                if (gx < a.W)
-                    for (int k = 0; k < STRIP; ++k) {
-                        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
-                        if (gy >= a.H) break;
-                        const size_t gi = (size_t)gy * a.W + gx;
-                        if (a.st[gi] == PF_OPEN) a.w[gi] = PF_INF;
-                    }
-            }
-            if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_CERT + (round + 1) % 3] = 0;
+            const int* lst = a.work + (size_t)li * a.total;
+            for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+                open_cells_set<MULTI>(a, lst[i], false);
+            if (blockIdx.x == 0)
+                for (int r = threadIdx.x; r < a.nr; r += blockDim.x) a.cert[3 * r + (round + 1) % 3] = 0;
             grid_sync(a.bar);
         }
-        li = run_passes<0>(S, a, li, mn, cert_ctr, epoch);
+        li = run_passes<0, MULTI>(S, a, li, slot, epoch, mn1, ok1);
         if (li < 0) break;
-        li = build_open_list(a, li);
+        li = build_open_list<MULTI>(a, li, -1);
         grid_sync(a.bar);
         if (ld(&a.ctrl[CTL_COUNT + li]) == 0) break;
-        li = run_passes<1>(S, a, li, mn, cert_ctr, epoch);
+        li = run_passes<1, MULTI>(S, a, li, slot, epoch, mn1, ok1);
         if (li < 0) break;
-        li = build_open_list(a, li);
+        li = build_open_list<MULTI>(a, li, slot);
         grid_sync(a.bar);
-        const uint32_t open_tiles = ld(&a.ctrl[CTL_COUNT + li]);
-        if (open_tiles == 0) break;
-        if (ld(cert_ctr) == 0) {
-            // nothing fixed this round: the OPEN cells are never reached and
-            // keep their quantized elevation
-            const int* lst = a.work + (size_t)li * a.ntiles;
-            for (uint32_t i = blockIdx.x; i < open_tiles; i += gridDim.x) {
-                const int t = lst[i];
-                const int x0 = (t % a.ntx) * TILE, y0 = (t / a.ntx) * TILE;
-                const int gx = x0 + (threadIdx.x & 63);
-                if (gx < a.W)
-                    for (int k = 0; k < STRIP; ++k) {
-                        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
-                        if (gy >= a.H) break;
-                        const size_t gi = (size_t)gy * a.W + gx;
-                        if (a.st[gi] == PF_OPEN) a.w[gi] = a.zq[gi];
-                    }
-            }
-            break;
-        }
+        // rasters whose round fixed nothing: their OPEN cells are never
+        // reached and keep their quantized elevation
+        const uint32_t nfinal = ld(&a.ctrl[CTL_FINAL]);
+        const int* fin = a.work + (size_t)3 * a.total;
+        for (uint32_t i = blockIdx.x; i < nfinal; i += gridDim.x) open_cells_set<MULTI>(a, fin[i], true);
+        if (ld(&a.ctrl[CTL_COUNT + li]) == 0) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_EPOCH] = epoch;
@@ -545,22 +650,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
 }
 
-}  // namespace
+PfRaster pf_raster(Context* c, const float* zq, float* w, float nodata) {
+    PfRaster R;
+    R.zq = zq;
+    R.w = w;
+    R.st = c->d_st;
+    R.W = c->W;
+    R.H = c->H;
+    R.ntx = c->ntx;
+    R.nty = c->nty;
+    R.nodata = nodata;
+    R.scal = c->d_scal;
+    R.status = c->d_ctrl + CTL_STATUS;
+    return R;
+}
 
-// Device function of the persistent kernel (to find its node in a graph).
-const void* pit_fill_kernel() { return reinterpret_cast<const void*>(k_pf_persistent); }
+int occupancy_cap(const void* fn, int device, int* cap) {
+    int per_cu = 0;
+    CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE_THREADS, 0));
+    hipDeviceProp_t prop;
+    CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (per_cu < 1) return CAGEO_ERR_HIP;
+    *cap = per_cu * prop.multiProcessorCount;
+    return CAGEO_OK;
+}
+
+}  // namespace
 
 // Co-resident workgroups of the persistent kernel on this device: the most
 // its grid barrier allows.
 int pit_fill_grid_cap(Context* c, int* cap) {
-    int per_cu = 0;
-    CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k_pf_persistent), TILE_THREADS, 0));
-    hipDeviceProp_t prop;
-    CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-    if (per_cu < 1) return CAGEO_ERR_HIP;
-    *cap = per_cu * prop.multiProcessorCount;
-    return CAGEO_OK;
+    return occupancy_cap(reinterpret_cast<const void*>(k_pf_persistent<false>), c->device, cap);
 }
 
 int pit_fill_persistent_grid(Context* c, int* grid) {
@@ -573,27 +693,56 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     int grid = 0;
     CAGEO_TRY(pit_fill_persistent_grid(c, &grid));
     PfArgs a;
-    a.zq = zq;
-    a.w = w;
-    a.st = c->d_st;
-    a.W = c->W;
-    a.H = c->H;
-    a.ntx = c->ntx;
-    a.nty = c->nty;
-    a.ntiles = c->ntiles;
-    a.nodata = nodata;
+    a.one = pf_raster(c, zq, w, nodata);
+    a.rs = nullptr;
+    a.bmap = nullptr;
+    a.rinfo = nullptr;
+    a.nr = 1;
+    a.total = c->ntiles;
     a.work = c->d_work3;
     a.mark = c->d_mark_pf;
     a.topen = c->d_topen;
     a.ctrl = c->d_ctrl;
+    a.cert = c->d_ctrl + CTL_CERT;
     a.bar = c->d_bar;
-    a.scal = c->d_scal;
     a.max_passes = 1 << 20;
     // An ordinary launch: the grid barrier needs its workgroups co-resident,
     // which the occupancy cap guarantees (work of other kernels on the device
     // can only delay the start of some workgroups).  Cooperative launches are
     // not used: the runtime allows them to one process at a time.
-    hipLaunchKernelGGL(k_pf_persistent, dim3(grid), dim3(TILE_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(k_pf_persistent<false>, dim3(grid), dim3(TILE_THREADS), 0, c->stream, a);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+// Raster table of a batch: each context's d_z -> d_w with its DEM nodata.
+int pit_fill_tables(MultiState* m, Context* const* ctxs) {
+    std::vector<PfRaster> t(m->nr);
+    for (int r = 0; r < m->nr; ++r) t[r] = pf_raster(ctxs[r], ctxs[r]->d_z, ctxs[r]->d_w, ctxs[r]->nodata_dem);
+    CAGEO_HIP_TRY(hipMalloc(&m->d_pf, t.size() * sizeof(PfRaster)));
+    CAGEO_HIP_TRY(hipMemcpy(m->d_pf, t.data(), t.size() * sizeof(PfRaster), hipMemcpyHostToDevice));
+    if (m->pf_grid <= 0)
+        CAGEO_TRY(occupancy_cap(reinterpret_cast<const void*>(k_pf_persistent<true>),
+                                ctxs[0]->device, &m->pf_grid));
+    return CAGEO_OK;
+}
+
+int pit_fill_multi(MultiState* m, hipStream_t s) {
+    PfArgs a;
+    a.one = PfRaster{};
+    a.rs = static_cast<const PfRaster*>(m->d_pf);
+    a.bmap = m->d_bmap;
+    a.rinfo = m->d_rinfo;
+    a.nr = m->nr;
+    a.total = m->total;
+    a.work = m->d_work;
+    a.mark = m->d_mark_pf;
+    a.topen = m->d_topen;
+    a.ctrl = m->d_ctrl;
+    a.cert = m->d_cert;
+    a.bar = m->d_bar;
+    a.max_passes = 1 << 20;
+    hipLaunchKernelGGL(k_pf_persistent<true>, dim3(m->pf_grid), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }

@@ -20,9 +20,9 @@ __device__ constexpr int TDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 // A wave owns a strip of STRIP tile rows, one lane per column.  Each step
 // solves one row exactly with its two neighbour rows held at their current
 // values: along the row the relaxation is a chain of per-cell monotone maps,
-// composed for all 64 columns at once by wave scans (6 shuffle steps in
-// each direction), so information crosses the whole row in one step rather
-// than one column per sweep.  Rows are visited in alternating order (Gauss-
+// composed for all 64 columns at once by wave scans (6 DPP steps in each
+// direction), so information crosses the whole row in one step rather than
+// one column per sweep.  Rows are visited in alternating order (Gauss-
 // Seidel down, then up); a row whose cells changed marks its neighbour rows
 // for another solve, and a tile is done after a sweep without changes.
 
@@ -35,24 +35,78 @@ __device__ inline void clamp_after(float& lo, float& hi, float plo, float phi) {
     hi = nhi;
 }
 
-// Inclusive scan towards higher lanes: lane x ends with f_x o ... o f_0.
-__device__ inline void scan_clamp_up(float& lo, float& hi) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const float plo = __shfl_up(lo, d, 64), phi = __shfl_up(hi, d, 64);
-        if (lane >= d) clamp_after(lo, hi, plo, phi);
-    }
+// ---- lane movement on the VALU (DPP) -----------------------------------------
+// GFX9 data-parallel-primitive controls: row shifts within 16-lane rows,
+// whole-wave shifts by one lane, broadcasts of lane 15 / 31 into the next
+// rows.  A lane whose source is outside the pattern, or whose row is masked
+// off, keeps `old`.  These replace LDS permutes (ds_bpermute) in the row
+// solves: no LDS traffic and a few cycles of latency per step.
+constexpr int DPP_ROW_SHL = 0x100;  // + d: lane x reads x + d within its row
+constexpr int DPP_ROW_SHR = 0x110;  // + d: lane x reads x - d within its row
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROWS = 0xf>
+__device__ inline int dpp_i(int old, int src) {
+    return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ inline float dpp_f(float old, float src) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, ROWS, 0xf, false));
+}
+// Value of lane x - 1 (lane 0 keeps its own) and of lane x + 1 (lane 63 keeps
+// its own), as __shfl_up / __shfl_down by one.
+__device__ inline float lane_prev(float v) { return dpp_f<DPP_WAVE_SHR1>(v, v); }
+__device__ inline float lane_next(float v) { return dpp_f<DPP_WAVE_SHL1>(v, v); }
+__device__ inline int lane_prev(int v) { return dpp_i<DPP_WAVE_SHR1>(v, v); }
+__device__ inline int lane_next(int v) { return dpp_i<DPP_WAVE_SHL1>(v, v); }
+__device__ inline float lane_read(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Inclusive scan towards lower lanes: lane x ends with f_x o ... o f_63.
+// Inclusive scans of an associative "self after prev" composition C(self,
+// prev) with identity element E, over the 64 lanes of a wave.
+//   up:   lane x ends with f_x o ... o f_0   (row shifts, then the row
+//         broadcasts carry rows 0 / 0-1 into the rows after them);
+//   down: lane x ends with f_x o ... o f_63  (row shifts, then the totals
+//         of the rows after each row, read from lanes 16, 32, 48).
+#define CAGEO_SCAN_UP(DPP, a, b, ea, eb, C)                                                    \
+    do {                                                                                       \
+        C(a, b, DPP<DPP_ROW_SHR + 1>(ea, a), DPP<DPP_ROW_SHR + 1>(eb, b));                     \
+        C(a, b, DPP<DPP_ROW_SHR + 2>(ea, a), DPP<DPP_ROW_SHR + 2>(eb, b));                     \
+        C(a, b, DPP<DPP_ROW_SHR + 4>(ea, a), DPP<DPP_ROW_SHR + 4>(eb, b));                     \
+        C(a, b, DPP<DPP_ROW_SHR + 8>(ea, a), DPP<DPP_ROW_SHR + 8>(eb, b));                     \
+        C(a, b, (DPP<DPP_ROW_BCAST15, 0xa>(ea, a)), (DPP<DPP_ROW_BCAST15, 0xa>(eb, b)));       \
+        C(a, b, (DPP<DPP_ROW_BCAST31, 0xc>(ea, a)), (DPP<DPP_ROW_BCAST31, 0xc>(eb, b)));       \
+    } while (0)
+#define CAGEO_SCAN_DOWN(DPP, READ, T, a, b, ea, eb, C)                                         \
+    do {                                                                                       \
+        C(a, b, DPP<DPP_ROW_SHL + 1>(ea, a), DPP<DPP_ROW_SHL + 1>(eb, b));                     \
+        C(a, b, DPP<DPP_ROW_SHL + 2>(ea, a), DPP<DPP_ROW_SHL + 2>(eb, b));                     \
+        C(a, b, DPP<DPP_ROW_SHL + 4>(ea, a), DPP<DPP_ROW_SHL + 4>(eb, b));                     \
+        C(a, b, DPP<DPP_ROW_SHL + 8>(ea, a), DPP<DPP_ROW_SHL + 8>(eb, b));                     \
+        T s3a_ = READ(a, 48), s3b_ = READ(b, 48);                                              \
+        T s2a_ = READ(a, 32), s2b_ = READ(b, 32);                                              \
+        T s1a_ = READ(a, 16), s1b_ = READ(b, 16);                                              \
+        C(s2a_, s2b_, s3a_, s3b_);                                                             \
+        C(s1a_, s1b_, s2a_, s2b_);                                                             \
+        const int row_ = (threadIdx.x & 63) >> 4;                                              \
+        C(a, b, row_ == 0 ? s1a_ : row_ == 1 ? s2a_ : row_ == 2 ? s3a_ : (ea),                 \
+          row_ == 0 ? s1b_ : row_ == 1 ? s2b_ : row_ == 2 ? s3b_ : (eb));                      \
+    } while (0)
+__device__ inline int lane_read(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Inclusive clamp-map scans towards higher / lower lanes.
+__device__ inline void scan_clamp_up(float& lo, float& hi) {
+    const float NI = -__builtin_huge_valf(), PI = __builtin_huge_valf();
+    CAGEO_SCAN_UP(dpp_f, lo, hi, NI, PI, clamp_after);
+}
 __device__ inline void scan_clamp_down(float& lo, float& hi) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const float plo = __shfl_down(lo, d, 64), phi = __shfl_down(hi, d, 64);
-        if (lane + d < 64) clamp_after(lo, hi, plo, phi);
-    }
+    const float NI = -__builtin_huge_valf(), PI = __builtin_huge_valf();
+    CAGEO_SCAN_DOWN(dpp_f, lane_read, float, lo, hi, NI, PI, clamp_after);
 }
 
 // Row flags of the sweeps: rows ly-1 and ly+1, and ly itself when `self`,
@@ -82,14 +136,15 @@ __device__ inline uint32_t border_bits(int lx, int ly) {
 }
 
 // Called by one thread per workgroup.
-__device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty,
+// `gbase` is the global number of the raster's first block.
+__device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty, int gbase,
                                       int* __restrict__ work_out, uint32_t* count_out,
                                       uint32_t* __restrict__ mark, uint32_t epoch) {
     for (int d = 0; d < 8; ++d) {
         if (!(bits & (1u << d))) continue;
         int nx = tx + TDX[d], ny = ty + TDY[d];
         if (nx < 0 || ny < 0 || nx >= ntx || ny >= nty) continue;
-        int t = ny * ntx + nx;
+        int t = gbase + ny * ntx + nx;
         if (atomicExch(&mark[t], epoch) != epoch) {
             uint32_t slot = atomicAdd(count_out, 1u);
             work_out[slot] = t;

@@ -186,10 +186,15 @@ int cageo_batch_run(cageo_batch* batch, float* const* d_mc, int n, uint32_t firs
                     int mode);
 /* Autotuner: per tile, times `reps` realizations (accumulated into the
  * scratch counters d_scratch_mc) for candidate workgroup counts of the
- * persistent kernels and row-block heights of the 2-D convolution, and keeps
- * the fastest.  out_config (optional, 2 ints per tile) receives them.
- * Results do not depend on these settings. */
+ * tile's own persistent kernels (PER_TILE mode) and row-block heights of the
+ * 2-D convolution, and keeps the fastest; then the workgroup count of the
+ * all-tile persistent launches (STREAM and GRAPH modes).  out_config
+ * (optional, 2 ints per tile) receives the per-tile choices.  Results do not
+ * depend on these settings. */
 int cageo_batch_tune(cageo_batch* batch, float* const* d_scratch_mc, int reps, int* out_config);
+/* out[4]: 64x64 blocks over all tiles, and the workgroup counts of the
+ * all-tile pit-fill, flow-routing and flow-tracing launches. */
+int cageo_batch_info(cageo_batch* batch, int* out);
 /* Set one tile's settings by hand: persistent_workgroups (0 = full chip) and
  * conv_rows in {0 (auto), 4, 8, 16}. */
 int cageo_batch_configure_tile(cageo_batch* batch, int tile, int persistent_workgroups,

@@ -101,3 +101,36 @@ def test_batch_tuned_settings_do_not_change_results(torch_cuda, tiles):
         b.run(other, 2, first=0, mode=ca.BATCH_STREAM)
         for a, c in zip(base, other):
             assert torch.equal(a, c)
+
+
+def test_batch_tile_outside_domain(torch_cuda, tiles):
+    """A tile whose elevation span exceeds the reference's bucket range
+    (2^24 mm, PitFillingAlgorithmCPUOpenMP.h:111-118) fails with
+    CAGEO_ERR_RANGE, alone: in the all-tile launches the other tile's counter
+    raster still equals its single-tile result."""
+    import cageo_amd as ca
+    torch = torch_cuda
+    announce("batch tile outside the pit-fill domain")
+    dem0, st0, _, dst0 = tiles[0]
+    bad = dem0.copy()
+    bad[10, 10] = 17000.0  # span > 16777.216 m
+    dbad = torch.from_numpy(bad).cuda()
+    _, _, ddem2, dst2 = tiles[2]
+    shapes = [SHAPES[0], SHAPES[2]]
+    (m0, pr0, s0), (m2, pr2, s2) = PARAMS[0], PARAMS[2]
+    with ca.Batch(shapes) as b:
+        b.setup_tile(0, dbad, dst0, float(NODATA), float(NODATA), 2, m0, pr0, 10.0, 0.0, s0, 7)
+        b.setup_tile(1, ddem2, dst2, float(NODATA), float(NODATA), 2, m2, pr2, 10.0, 0.0, s2, 8)
+        for mode in (ca.BATCH_STREAM, ca.BATCH_GRAPH):
+            mcs = [torch.zeros(s, dtype=torch.float32, device="cuda") for s in shapes]
+            with pytest.raises(ca.CageoError) as err:
+                b.run(mcs, 2, first=0, mode=mode)
+            assert err.value.status == -6
+            h, w = SHAPES[2]
+            r = ca.filter_radius(pr2, 10.0)
+            with ca.Context(w, h) as ctx:
+                ctx.set_filter(m2, r, ca.create_filter(m2, pr2, 10.0, r))
+                ctx.mc_setup(ddem2, dst2, float(NODATA), float(NODATA), 2, 0.0, s2)
+                single = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+                ctx.run_realizations(single, 2, 8, 0)
+            assert torch.equal(mcs[1], single), mode
