@@ -46,6 +46,7 @@ sys.path.insert(0, ROOT)
 METRIC = "realizations/s + Mcells/s at 4096² DEM; surrogate vs MC time @ L1≤0.02"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 NODATA = -9999.0
 CALIB_BYTES = 512 << 20  # the calibration kernel reads and writes this much
 
@@ -84,6 +85,8 @@ def parse():
     p.add_argument("--batch-realizations", type=int, default=100)
     p.add_argument("--surrogate-realizations", type=int, default=1000,
                    help="Monte Carlo size the surrogate is judged against (0: skip)")
+    p.add_argument("--traffic-out", default=None,
+                   help="keep the counter passes' CSV files in this directory")
     p.add_argument("--traffic", action="store_true",
                    help="run the two rocprofv3 counter passes for roofline.traffic (off by "
                         "default: the profiler can crash in its own exit path, so the passes "
@@ -169,7 +172,9 @@ def measure_traffic(args, kernel):
     reads and writes CALIB_BYTES).  Returns (result or None, reason)."""
     if not shutil.which("rocprofv3"):
         return None, "rocprofv3 not found"
-    outdir = tempfile.mkdtemp(prefix="cageo_pmc_")
+    outdir = args.traffic_out or tempfile.mkdtemp(prefix="cageo_pmc_")
+    if args.traffic_out:
+        os.makedirs(outdir, exist_ok=True)
     try:
         res = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -189,7 +194,8 @@ def measure_traffic(args, kernel):
     except (OSError, ValueError, KeyError) as e:
         return None, f"counter parsing failed: {e}"
     finally:
-        shutil.rmtree(outdir, ignore_errors=True)
+        if not args.traffic_out:
+            shutil.rmtree(outdir, ignore_errors=True)
 
 
 def batched_leg(torch, ca, args):
@@ -222,6 +228,7 @@ def batched_leg(torch, ca, args):
         torch.cuda.synchronize()
         out["tune_s"] = round(time.perf_counter() - t0, 2)
         out["tuned_workgroups"] = sorted(set(g for g, _ in cfg))
+        out["all_tile_launch"] = b.info()
         res = {}
         for name, mode in [("graph", ca.BATCH_GRAPH), ("per_tile", ca.BATCH_PER_TILE)]:
             mcs = [torch.zeros(sh, dtype=torch.float32, device="cuda") for sh in shapes]
@@ -262,6 +269,19 @@ def surrogate_leg(torch, ca, dem, stream, args):
     q, info = sg.checked_prediction(torch, net, dem, stream, 1.8, 60.0, args.seed + 7,
                                     fallback_realizations=n)
     sur_s = info["predict_s"] + info["check_s"] + info.get("fallback_s", 0.0)
+    # the network alone: forward passes timed with events on its stream
+    h, w = dem.shape
+    x = sg.device_inputs(torch, dem, stream, 1.8, 60.0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    net.predict(x)
+    e0.record()
+    for _ in range(reps):
+        net.predict(x)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    tflops = sg.forward_flops(h, w) / (fwd_ms * 1e-3) / 1e12
     dev = (q - p_mc).abs()
     border = p_mc > 0
     return {
@@ -273,6 +293,11 @@ def surrogate_leg(torch, ca, dem, stream, args):
         "max_dev": round(float(dev.max()), 4),
         "check": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in info.items()},
         "weights": os.path.relpath(sg.DEFAULT_WEIGHTS, ROOT),
+        "network_forward_ms": round(fwd_ms, 3),
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 5),
+                     "flops_per_forward": sg.forward_flops(h, w),
+                     "note": "BF16 MFMA convolutions of one 4096x4096 forward pass"},
     }
 
 

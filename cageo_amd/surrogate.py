@@ -88,6 +88,24 @@ def layer_spec(base=32):
     ]
 
 
+# Resolution level (0 = full, k = 1/2^k) of each convolution layer.
+LAYER_LEVEL = {"enc1a": 0, "enc1b": 0, "enc2a": 1, "enc2b": 1, "enc3a": 2, "enc3b": 2,
+               "mida": 3, "midb": 3, "dec3a": 2, "dec3b": 2, "dec2a": 1, "dec2b": 1,
+               "dec1a": 0, "dec1b": 0}
+
+
+def forward_flops(h, w, base=32, in_channels=IN_PAD):
+    """Multiply-add FLOPs (2 per MAC) of one forward pass over an h x w input,
+    as the kernels execute it (the first layer reads all `in_channels`)."""
+    total = 0
+    for name, c1, _, c2, co in layer_spec(base):
+        if name == "enc1a":
+            c1 = in_channels
+        lv = LAYER_LEVEL[name]
+        total += 2 * 9 * (c1 + c2) * co * (h >> lv) * (w >> lv)
+    return total + 2 * base * h * w  # the 1x1 head
+
+
 def init_params(base=32, seed=0):
     """He-normal weights ([9][out][in], the padded input channels zero) and
     zero biases, as float32 numpy arrays keyed by layer name."""
