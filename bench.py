@@ -324,11 +324,13 @@ def main():
     tst = torch.from_numpy(stream).cuda()
     stream_h = torch.cuda.current_stream()
 
-    def run(model, pr, steps, warmup, stage_times):
+    def run(model, pr, steps, warmup, stage_times, conv_mode=ca.CONV_EXACT):
         r = ca.filter_radius(pr, 10.0)
         taps = ca.create_filter(model, pr, 10.0, r)
         ctx = ca.Context(size, size, stream_h)
         ctx.set_filter(model, r, taps)
+        ctx.set_conv_mode(conv_mode)
+        run.terms = ctx.conv_terms()
         ctx.mc_setup(tdem, tst, NODATA, NODATA, 2, 0.0, 1.8)
         mc = torch.zeros((size, size), dtype=torch.float32, device="cuda")
         ctx.stage_times(stage_times)
@@ -406,7 +408,23 @@ def main():
             line["exponential_pr200"]["convolution_roofline"] = {
                 "bound": "fp32-vector", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
-                "note": "k_conv_2d plus its output copy; no FMA (exact reference order)"}
+                "note": "k_conv_2d; no FMA (exact reference order)"}
+        dtf, st_f, _ = run(ca.EXPONENTIAL, 200.0, args.exp_steps, 1, not args.no_stage_times,
+                           ca.CONV_FAST)
+        fast = {"realizations_per_s": round(args.exp_steps / dtf, 3),
+                "mcells_per_s": round(cells * args.exp_steps / dtf / 1e6, 2),
+                "terms": run.terms, "stage_ms": st_f,
+                "note": "low-rank separable form of the filter with fused multiply-adds, within "
+                        "1e-5 of the exact convolution (not bit-exact)"}
+        if st_f:
+            K, L = run.terms, 2 * ca.filter_radius(200.0, 10.0) + 1
+            flops = 2.0 * 2 * K * L * cells  # column and row passes, one FMA = 2 FLOP
+            tf = flops / st_f["convolution"] / 1e9
+            fast["convolution_roofline"] = {
+                "bound": "fp32-vector", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
+                "note": "k_lr_cols + k_lr_rows, 2 x K x (2r+1) FMA per cell"}
+        line["exponential_pr200"]["fast_mode"] = fast
     if args.batch_tiles > 0:
         line["batched"] = batched_leg(torch, ca, args)
     if args.surrogate_realizations > 0:

@@ -24,7 +24,8 @@ import numpy as np
 
 __all__ = [
     "CageoError", "Context", "lib", "build", "create_filter", "filter_radius", "synth_dem",
-    "EXPONENTIAL", "GAUSSIAN", "OUTSIDE", "LAKE", "LIB_PATH",
+    "EXPONENTIAL", "GAUSSIAN", "OUTSIDE", "LAKE", "LIB_PATH", "CONV_EXACT", "CONV_FAST",
+    "filter_low_rank",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -92,6 +93,9 @@ def lib():
         "cageo_write_jpeg_gray": ([ctypes.c_char_p, vp, i, i, i], i),
         "cageo_iteration_counts": ([vp, vp], i),
         "cageo_context_configure": ([vp, i, i], i),
+        "cageo_context_set_conv_mode": ([vp, i], i),
+        "cageo_context_conv_terms": ([vp, vp], i),
+        "cageo_filter_low_rank": ([vp, i, i, ctypes.c_double, vp, vp, vp, vp], i),
         "cageo_batch_create": ([ctypes.POINTER(vp), i, vp, vp, vp], i),
         "cageo_batch_destroy": ([vp], i),
         "cageo_batch_context": ([vp, i, ctypes.POINTER(vp)], i),
@@ -151,6 +155,21 @@ def synth_dem(width: int, height: int, seed: int, cell_size: float = 10.0,
 
 
 BATCH_PER_TILE, BATCH_STREAM, BATCH_GRAPH = 0, 1, 2
+CONV_EXACT, CONV_FAST = 0, 1
+
+
+def filter_low_rank(taps, radius, kmax=32, tol=1e-6):
+    """(values[K], vectors[K][2r+1], residual) of the symmetric filter's
+    low-rank separable form (cageo_filter_low_rank, host only)."""
+    n = 2 * radius + 1
+    f = np.ascontiguousarray(taps, np.float32).reshape(n, n)
+    vec = np.zeros((kmax, n), np.float32)
+    val = np.zeros(kmax, np.float32)
+    k, res = ctypes.c_int(), ctypes.c_double()
+    _check(lib().cageo_filter_low_rank(f.ctypes.data, radius, kmax, tol, vec.ctypes.data,
+                                       val.ctypes.data, ctypes.byref(k), ctypes.byref(res)),
+           "filter_low_rank")
+    return val[:k.value].copy(), vec[:k.value].copy(), res.value
 
 
 class Batch:
@@ -324,6 +343,16 @@ class Context:
         out = np.zeros(4, np.int32)
         _check(lib().cageo_iteration_counts(self._h, out.ctypes.data), "iteration_counts")
         return tuple(int(v) for v in out)
+
+    def set_conv_mode(self, mode):
+        """CONV_EXACT (bit-exact reference order) or CONV_FAST (low-rank form of
+        the exponential filter, within 1e-5)."""
+        _check(lib().cageo_context_set_conv_mode(self._h, mode), "set_conv_mode")
+
+    def conv_terms(self):
+        out = ctypes.c_int()
+        _check(lib().cageo_context_conv_terms(self._h, ctypes.byref(out)), "conv_terms")
+        return out.value
 
     def configure(self, persistent_workgroups=0, conv_rows=0):
         _check(lib().cageo_context_configure(self._h, persistent_workgroups, conv_rows),

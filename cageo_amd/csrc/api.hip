@@ -55,7 +55,8 @@ int alloc(T** p, size_t count) {
 void free_all(Context* c) {
     void* ptrs[] = {c->d_work[0], c->d_work[1], c->d_mark, c->d_topen, c->d_mark_pf, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_filter,
                     c->d_sflags, c->d_slabel, c->d_off, c->d_factor, c->d_a, c->d_b, c->d_z,
-                    c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr, c->d_ptr2};
+                    c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr, c->d_ptr2,
+                    c->d_lr_col, c->d_lr_row, c->d_lr_planes};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -135,6 +136,44 @@ int check_status(Context* c) {
     CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
     st = c->h_scal[SC_NSLOTS - 1];
     return st ? -(int)st : CAGEO_OK;
+}
+
+// Low-rank separable form of the exponential filter for the fast mode:
+// terms until the remainder is below 1e-6 of the filter (at most 32),
+// column taps padded to whole register-ring turns, row taps lambda_k v_k.
+int prepare_low_rank(Context* c) {
+    if (c->model != CAGEO_FILTER_EXPONENTIAL || c->h_taps.empty()) return CAGEO_OK;
+    const int r = c->radius, L = 2 * r + 1, KMAX = 32;
+    int LC, LP, S;
+    size_t lc, lr;
+    if (lr_cols_geometry(r, &LC, &lc) != CAGEO_OK || lr_rows_geometry(r, &LP, &S, &lr) != CAGEO_OK)
+        return CAGEO_ERR_ARG;
+    std::vector<float> vec((size_t)KMAX * L), val(KMAX);
+    int K = 0;
+    double resid = 0.0;
+    CAGEO_TRY(cageo_filter_low_rank(c->h_taps.data(), r, KMAX, 1e-6, vec.data(), val.data(), &K,
+                                    &resid));
+    std::vector<float> col((size_t)K * LC, 0.0f), row((size_t)K * LP, 0.0f);
+    for (int k = 0; k < K; ++k)
+        for (int i = 0; i < L; ++i) {
+            col[(size_t)k * LC + i] = vec[(size_t)k * L + i];
+            row[(size_t)k * LP + i] = val[k] * vec[(size_t)k * L + i];
+        }
+    if (c->d_lr_col) (void)hipFree(c->d_lr_col);
+    if (c->d_lr_row) (void)hipFree(c->d_lr_row);
+    if (c->d_lr_planes) (void)hipFree(c->d_lr_planes);
+    c->d_lr_col = c->d_lr_row = c->d_lr_planes = nullptr;
+    c->lr_k = 0;
+    CAGEO_TRY(alloc(&c->d_lr_col, col.size()));
+    CAGEO_TRY(alloc(&c->d_lr_row, row.size()));
+    CAGEO_TRY(alloc(&c->d_lr_planes, (size_t)K * c->n));
+    CAGEO_HIP_TRY(hipMemcpyAsync(c->d_lr_col, col.data(), col.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+    CAGEO_HIP_TRY(hipMemcpyAsync(c->d_lr_row, row.data(), row.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->lr_k = K;
+    return CAGEO_OK;
 }
 
 uint32_t next_epoch(Context* c) {
@@ -273,6 +312,31 @@ int cageo_set_filter(cageo_context* ctx, int model, int radius, const float* tap
     c->model = model;
     c->radius = radius;
     c->filter_len = (int)host.size();
+    if (model == CAGEO_FILTER_EXPONENTIAL) c->h_taps.assign(taps, taps + (size_t)L * L);
+    else c->h_taps.clear();
+    if (c->conv_fast) CAGEO_TRY(cageo::prepare_low_rank(c));
+    return CAGEO_OK;
+}
+
+int cageo_context_set_conv_mode(cageo_context* ctx, int mode) {
+    if (!ctx || (mode != CAGEO_CONV_EXACT && mode != CAGEO_CONV_FAST)) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    const int old = c->conv_fast;
+    c->conv_fast = mode == CAGEO_CONV_FAST;
+    if (c->conv_fast && c->model == CAGEO_FILTER_EXPONENTIAL) {
+        const int st = cageo::prepare_low_rank(c);
+        if (st != CAGEO_OK) {
+            c->conv_fast = old;
+            return st;
+        }
+    }
+    return CAGEO_OK;
+}
+
+int cageo_context_conv_terms(cageo_context* ctx, int* terms) {
+    if (!ctx || !terms) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    *terms = (c->conv_fast && c->model == CAGEO_FILTER_EXPONENTIAL) ? c->lr_k : 0;
     return CAGEO_OK;
 }
 

@@ -60,6 +60,13 @@ struct Context {
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
     uint32_t epoch = 0;
     int conv_ty = 0;  // tuned row-block height of the 2-D convolution (0 = largest that fits)
+    // Fast mode of the 2-D convolution: the filter's low-rank separable form.
+    int conv_fast = 0;
+    int lr_k = 0;                     // terms
+    float* d_lr_col = nullptr;        // K x LC column taps
+    float* d_lr_row = nullptr;        // K x LP row taps (lambda_k v_k)
+    float* d_lr_planes = nullptr;     // K column-pass planes of W x H
+    std::vector<float> h_taps;        // the 2-D filter (host copy)
 
     // scalars
     uint32_t* d_scal = nullptr;
@@ -250,6 +257,11 @@ int launch_convolve(Context* c, float* field, float* buf);
 // Convolution whose result may stay in `buf` (exponential model): returns in
 // *result the buffer that holds it, saving the copy back into `field`.
 int launch_convolve_to(Context* c, float* field, float* buf, float** result);
+int launch_convolve_fast(Context* c, const float* field, float* out);
+int lr_cols_geometry(int r, int* LC, size_t* lds);
+int lr_rows_geometry(int r, int* LP, int* S, size_t* lds);
+int prepare_low_rank(Context* c);
+constexpr size_t LDS_LIMIT = 160 * 1024;
 int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, int count);
 int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes);
 int launch_realistic(Context* c, const float* dem, float* field, float nodata);

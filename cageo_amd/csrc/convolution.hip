@@ -190,7 +190,6 @@ __global__ __launch_bounds__(16 * TY) void k_conv_2d(float* __restrict__ out,
 }  // namespace
 
 // Largest LDS allocation a workgroup may request on gfx950.
-constexpr size_t LDS_LIMIT = 160 * 1024;
 
 int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes) {
     // S: LDS row stride, covering 64 + 2r columns plus one float4 of slack
@@ -223,6 +222,11 @@ int launch_convolve_to(Context* c, float* field, float* buf, float** result) {
                            c->d_filter, r, W, H);
         CAGEO_HIP_TRY(hipGetLastError());
         *result = field;
+        return CAGEO_OK;
+    }
+    if (c->conv_fast) {  // low-rank separable form (within 1e-5 of the exact order)
+        CAGEO_TRY(launch_convolve_fast(c, field, buf));
+        *result = buf;
         return CAGEO_OK;
     }
     int ty, S;

@@ -193,10 +193,11 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
             const int ly = strip * STRIP + k;
             if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
             S.rowdirty[ly] = 0u;
+            const bool isflat = (flat >> k) & 1u;
+            if (__ballot(isflat) == 0) continue;  // no flat cell: nothing in the row can change
             const int li = (ly + 1) * HALO_W + lx + 1;
             const int2 c = S.dz[li];
             const float zc = __int_as_float(c.y);
-            const bool isflat = (flat >> k) & 1u;
             const float zl = lane_prev(zc), zrt = lane_next(zc);
             int a, kl = D_INF, kr = D_INF;
             if (isflat) {
@@ -246,7 +247,7 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     for (int off = 32; off > 0; off >>= 1) bits |= (uint32_t)__shfl_xor((int)bits, off);
     if ((threadIdx.x & 63) == 0 && bits) atomicOr(&S.bits, bits);
     __syncthreads();
-    if (threadIdx.x == 0 && S.bits)
+    if (threadIdx.x < 64 && S.bits)  // uniform within the first wave
         activate_tiles(S.bits, tx, ty, a.ntx, a.nty, b.gbase, work_out, count_out, args.mark,
                        epoch);
     __syncthreads();

@@ -230,12 +230,10 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
         if (nfixed) atomicAdd(cert_ctr, nfixed);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        a.topen[b.gbase + b.lb] = S.any;
-        if (S.bits)
-            activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
-                           work_out, count_out, a.mark, epoch);
-    }
+    if (threadIdx.x == 0) a.topen[b.gbase + b.lb] = S.any;
+    if (threadIdx.x < 64 && S.bits)  // uniform within the first wave
+        activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
+                       work_out, count_out, a.mark, epoch);
     __syncthreads();
 }
 
@@ -293,9 +291,10 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             const int ly = strip * STRIP + k;
             if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
             S.rowdirty[ly] = 0u;
-            const int li = (ly + 1) * HALO_W + lx + 1;
             const uint32_t bit = 1u << k;
             const bool op = (open & bit) != 0;
+            if (__ballot(op) == 0) continue;  // no OPEN cell: nothing in the row can change
+            const int li = (ly + 1) * HALO_W + lx + 1;
             const float z = S.zt[ly * TILE + lx];
             const float wcur = S.rel[li].x;
             const bool zlive = op && pf_live(z, mn);
@@ -403,9 +402,10 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             const int ly = strip * STRIP + k;
             if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
             S.rowdirty[ly] = 0u;
-            const int li = (ly + 1) * HALO_W + lx + 1;
             const uint32_t bit = 1u << k;
             const bool op = (open & bit) != 0;
+            if (__ballot(op) == 0) continue;  // no OPEN cell: nothing in the row can change
+            const int li = (ly + 1) * HALO_W + lx + 1;
             const float w = S.rel[li].x;
             const bool live = op && pf_live(w, mn);
             const bool seed = op && min8y(S.rel, li) <= w;
@@ -501,16 +501,17 @@ __device__ inline int build_open_list(const PfArgs& a, int b, int slot) {
     }
     int* out = a.work + (size_t)l * a.total;
     int* fin = a.work + (size_t)3 * a.total;
-    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x) {
-        if (!a.topen[g]) continue;
-        if (slot >= 0) {
+    // whole waves iterate (the appends are wave-aggregated)
+    const int n = (a.total + 63) & ~63;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+        const bool open = g < a.total && a.topen[g] != 0u;
+        bool done = false;
+        if (open && slot >= 0) {
             const int r = MULTI ? a.bmap[g].x : 0;
-            if (ld(&a.cert[3 * r + slot]) == 0) {
-                fin[atomicAdd(&a.ctrl[CTL_FINAL], 1u)] = g;
-                continue;
-            }
+            done = ld(&a.cert[3 * r + slot]) == 0u;
         }
-        out[atomicAdd(&a.ctrl[CTL_COUNT + l], 1u)] = g;
+        wave_append(open && done, g, fin, &a.ctrl[CTL_FINAL]);
+        wave_append(open && !done, g, out, &a.ctrl[CTL_COUNT + l]);
     }
     return l;
 }
@@ -524,11 +525,15 @@ __device__ inline void open_cells_set(const PfArgs& a, int g, bool to_z) {
     const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
     const int gx = x0 + (threadIdx.x & 63);
     if (gx >= R.W) return;
+    const int gy0 = y0 + (threadIdx.x >> 6) * STRIP;
+    uint8_t sv[STRIP];
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) sv[k] = gy0 + k < R.H ? R.st[(size_t)(gy0 + k) * R.W + gx] : 0;
+#pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        const int gy = y0 + (threadIdx.x >> 6) * STRIP + k;
-        if (gy >= R.H) break;
-        const size_t gi = (size_t)gy * R.W + gx;
-        if (R.st[gi] == PF_OPEN) R.w[gi] = to_z ? R.zq[gi] : PF_INF;
+        if (gy0 + k >= R.H || sv[k] != PF_OPEN) continue;
+        const size_t gi = (size_t)(gy0 + k) * R.W + gx;
+        R.w[gi] = to_z ? R.zq[gi] : PF_INF;
     }
 }
 

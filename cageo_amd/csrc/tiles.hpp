@@ -135,21 +135,41 @@ __device__ inline uint32_t border_bits(int lx, int ly) {
     return b;
 }
 
-// Called by one thread per workgroup.
-// `gbase` is the global number of the raster's first block.
+// Lists the neighbour tiles of tile (tx, ty) named by `bits` for the next
+// pass, each at most once per epoch.  Called by all 64 lanes of one wave:
+// lanes 0..7 take one direction each (their mark exchanges are in flight
+// together), and one atomic reserves the list slots of the whole wave.
 __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty, int gbase,
                                       int* __restrict__ work_out, uint32_t* count_out,
                                       uint32_t* __restrict__ mark, uint32_t epoch) {
-    for (int d = 0; d < 8; ++d) {
-        if (!(bits & (1u << d))) continue;
-        int nx = tx + TDX[d], ny = ty + TDY[d];
-        if (nx < 0 || ny < 0 || nx >= ntx || ny >= nty) continue;
-        int t = gbase + ny * ntx + nx;
-        if (atomicExch(&mark[t], epoch) != epoch) {
-            uint32_t slot = atomicAdd(count_out, 1u);
-            work_out[slot] = t;
+    const int lane = threadIdx.x & 63;
+    bool push = false;
+    int t = 0;
+    if (lane < 8 && (bits & (1u << lane))) {
+        const int nx = tx + TDX[lane], ny = ty + TDY[lane];
+        if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) {
+            t = gbase + ny * ntx + nx;
+            push = atomicExch(&mark[t], epoch) != epoch;
         }
     }
+    const unsigned long long m = __ballot(push);
+    if (m == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count_out, (uint32_t)__popcll(m));
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    if (push) work_out[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+}
+
+// Appends `item` for the lanes with `take` to a list: one atomic per wave.
+__device__ inline void wave_append(bool take, int item, int* __restrict__ out, uint32_t* count) {
+    const unsigned long long m = __ballot(take);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, first, 64);
+    if (take) out[base + __popcll(m & ((1ull << lane) - 1ull))] = item;
 }
 
 // Block-wide OR of per-thread border bits; returns the result in all threads.

@@ -70,6 +70,17 @@ int cageo_filter_radius(float practical_range, float cell_size);
 int cageo_create_filter(int model, float practical_range, float cell_size, int radius,
                         float* taps_out);
 
+/* Fast mode of the 2-D (exponential) convolution, no reference counterpart:
+ * the symmetric filter's low-rank separable form f = sum_k lambda_k v_k v_k^T
+ * (cyclic Jacobi eigen-decomposition in double), terms kept in decreasing
+ * |lambda| until the Frobenius norm of the remainder is at most tol * |f|,
+ * at most kmax.  vectors: kmax x (2r+1), values: kmax.  Returns
+ * CAGEO_ERR_RANGE when kmax terms do not reach tol, CAGEO_ERR_ARG for a
+ * filter whose asymmetry exceeds 1e-6 of its largest tap (below that, its
+ * symmetric part is decomposed). */
+int cageo_filter_low_rank(const float* taps, int radius, int kmax, double tol, float* vectors,
+                          float* values, int* k_out, double* residual_out);
+
 /* ConvolutionAlgorithm<T>::setFilterModel/setFilterRadius + the per-call
  * H2D filter copy of ConvolutionAlgorithmGPUCUDA.cu:32-33, done once. */
 int cageo_set_filter(cageo_context* ctx, int model, int radius, const float* host_taps);
@@ -170,6 +181,19 @@ int cageo_stage_times(cageo_context* ctx, float* out, int enable);
 #define CAGEO_BATCH_PER_TILE 0
 #define CAGEO_BATCH_STREAM 1
 #define CAGEO_BATCH_GRAPH 2
+/* Convolution order (no reference counterpart).  CAGEO_CONV_EXACT (the
+ * default): the reference CPU loops' order, bit-exact.  CAGEO_CONV_FAST:
+ * the exponential filter's low-rank separable form with fused multiply-adds
+ * (terms until the remainder is below 1e-6 of the filter, at most 32; within
+ * 1e-5 of the exact convolution relative to its largest magnitude); labels
+ * downstream then follow that field, not the exact one.  The Gaussian
+ * filter is separable already and runs exactly in both modes. */
+#define CAGEO_CONV_EXACT 0
+#define CAGEO_CONV_FAST 1
+int cageo_context_set_conv_mode(cageo_context* ctx, int mode);
+/* Terms of the low-rank form in use (0: the exact 2-D or separable path). */
+int cageo_context_conv_terms(cageo_context* ctx, int* terms);
+
 typedef struct cageo_batch cageo_batch;
 int cageo_batch_create(cageo_batch** out, int ntiles, const int* widths, const int* heights,
                        void* hip_stream);

@@ -70,6 +70,66 @@ def test_convolution_bit_exact(ca, torch_cuda, shape, pr):
             assert np.array_equal(got, ref), f"max diff {np.abs(got - ref).max()}"
 
 
+@pytest.mark.parametrize("shape,pr", [((256, 256), 200), ((96, 130), 200), ((200, 300), 60),
+                                      ((1024, 1024), 200), ((17, 17), 30)])
+def test_convolution_fast_mode(ca, torch_cuda, shape, pr):
+    """Fast mode (low-rank separable form of the exponential filter, fused
+    multiply-adds): within 1e-5 of the exact convolution relative to its
+    largest magnitude; the Gaussian filter, separable already, stays
+    bit-exact.  The exact GPU convolution is the reference (it equals the
+    oracle bit for bit, test above)."""
+    torch = torch_cuda
+    h, w = shape
+    noise = oracle.random_field(shape, 99, 1, 0, 0.0, 1.0)
+    for model in (oracle.EXPONENTIAL, oracle.GAUSSIAN):
+        r = oracle.filter_radius(pr, 10.0)
+        if r > min(h, w):
+            continue
+        announce("convolution fast mode", shape, pr, model)
+        f = ca.create_filter(model, pr, 10.0, r)
+        with ca.Context(w, h) as ctx:
+            ctx.set_filter(model, r, f)
+            t = dev(torch, noise)
+            ctx.convolve(t)
+            exact = host(t)
+            ctx.set_conv_mode(ca.CONV_FAST)
+            terms = ctx.conv_terms()
+            t = dev(torch, noise)
+            ctx.convolve(t)
+            fast = host(t)
+        if model == oracle.GAUSSIAN:
+            assert terms == 0 and np.array_equal(fast, exact)
+        else:
+            assert 1 <= terms <= 32
+            err = np.abs(fast.astype(np.float64) - exact).max() / np.abs(exact).max()
+            assert err <= 1e-5, err
+
+
+def test_run_realizations_fast_mode(ca, torch_cuda):
+    """A Monte Carlo run in fast mode (exponential PR 200 m, 512^2): the
+    counters stay within [0, n], and their total within 1 % of the exact
+    mode's (the fields differ by ~1e-6 of their range, so only cells at a
+    quantisation or tie boundary can change)."""
+    torch = torch_cuda
+    announce("run_realizations fast mode 512^2")
+    size, n = 512, 4
+    dem, st = synth_case(size, seed=11)
+    r = ca.filter_radius(200.0, 10.0)
+    f = ca.create_filter(ca.EXPONENTIAL, 200.0, 10.0, r)
+    sums = {}
+    for mode in (ca.CONV_EXACT, ca.CONV_FAST):
+        with ca.Context(size, size) as ctx:
+            ctx.set_filter(ca.EXPONENTIAL, r, f)
+            ctx.set_conv_mode(mode)
+            ctx.mc_setup(dev(torch, dem), dev(torch, st), float(NODATA), float(NODATA), 2, 0.0, 2.5)
+            mc = torch.zeros((size, size), dtype=torch.float32, device="cuda")
+            ctx.run_realizations(mc, n, 77, 0)
+            m = host(mc)
+            assert m.min() >= 0 and m.max() <= n
+            sums[mode] = float(m.sum())
+    assert abs(sums[ca.CONV_FAST] - sums[ca.CONV_EXACT]) <= 0.01 * sums[ca.CONV_EXACT], sums
+
+
 # ------------------------------------------------------------- realistic / burn
 def test_realistic_dem_bit_exact(ca, torch_cuda):
     announce("realistic_dem")

@@ -68,3 +68,33 @@ def test_context_requires_device_or_fails_cleanly():
         pytest.skip("device present; covered by the GPU tests")
     with pytest.raises(ca.CageoError):
         ca.Context(16, 16)
+
+
+@pytest.mark.parametrize("pr", [30.0, 60.0, 200.0])
+def test_filter_low_rank_against_eigh(pr):
+    """The fast convolution's low-rank form (host Jacobi eigen-decomposition)
+    against numpy's symmetric eigen-solver: the same leading eigenvalues,
+    orthonormal vectors, and the reconstruction within the tolerance."""
+    r = ca.filter_radius(pr, 10.0)
+    f = ca.create_filter(ca.EXPONENTIAL, pr, 10.0, r).astype(np.float64)
+    lam, vec, res = ca.filter_low_rank(f, r, kmax=32, tol=1e-6)
+    k = len(lam)
+    assert 1 <= k <= 32 and res <= 1e-6
+    w = np.linalg.eigvalsh(0.5 * (f + f.T))
+    w = w[np.argsort(-np.abs(w))][:k]
+    assert np.allclose(lam, w, rtol=1e-5, atol=1e-7 * np.abs(w[0]))
+    v = vec.astype(np.float64)
+    assert np.allclose(v @ v.T, np.eye(k), atol=1e-5)
+    rec = (v.T * lam) @ v
+    assert np.linalg.norm(rec - f) <= 2e-6 * np.linalg.norm(f)
+
+
+def test_filter_low_rank_refuses_asymmetric_and_short_budgets():
+    r = 8
+    f = ca.create_filter(ca.EXPONENTIAL, 30.0, 10.0, r).astype(np.float32)
+    g = f.copy()
+    g[0, 1] += 1e-3 * f.max()
+    with pytest.raises(ca.CageoError):
+        ca.filter_low_rank(g, r)
+    with pytest.raises(ca.CageoError):  # two terms cannot reach 1e-6
+        ca.filter_low_rank(f, r, kmax=2)
