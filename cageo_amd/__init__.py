@@ -25,7 +25,7 @@ import numpy as np
 __all__ = [
     "CageoError", "Context", "lib", "build", "create_filter", "filter_radius", "synth_dem",
     "EXPONENTIAL", "GAUSSIAN", "OUTSIDE", "LAKE", "LIB_PATH", "CONV_EXACT", "CONV_FAST",
-    "filter_low_rank",
+    "filter_low_rank", "Surrogate",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -104,6 +104,13 @@ def lib():
         "cageo_batch_tune": ([vp, vp, i, vp], i),
         "cageo_batch_configure_tile": ([vp, i, i, i], i),
         "cageo_batch_info": ([vp, vp], i),
+        "cageo_surrogate_create": ([ctypes.POINTER(vp), ctypes.c_char_p, vp], i),
+        "cageo_surrogate_destroy": ([vp], i),
+        "cageo_surrogate_forward": ([vp, vp, i, i, vp], i),
+        "cageo_surrogate_predict": ([vp, vp, vp, i, i, f, f, f, f, vp], i),
+        "cageo_surrogate_check": ([vp, vp, vp, i, u64, ctypes.c_double, vp,
+                                   ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double)], i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -234,6 +241,56 @@ class Batch:
     def configure_tile(self, t, persistent_workgroups=0, conv_rows=0):
         _check(lib().cageo_batch_configure_tile(self._h, t, persistent_workgroups, conv_rows),
                "batch_configure_tile")
+
+
+class Surrogate:
+    """The border-probability surrogate behind the C ABI (cageo_surrogate):
+    weights from a `.bin` file, inference and the Monte Carlo check on the
+    device."""
+
+    def __init__(self, weights_path, stream=None):
+        h = ctypes.c_void_p()
+        sp = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream")
+                                          else stream)
+        _check(lib().cageo_surrogate_create(ctypes.byref(h), os.fsencode(weights_path), sp),
+               "surrogate_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cageo_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def forward(self, x_bf16, h, w, out):
+        """Prepared bf16 inputs [h][w][32] (h, w multiples of 8) -> out [h][w]."""
+        _check(lib().cageo_surrogate_forward(self._h, _ptr(x_bf16), h, w, _ptr(out)),
+               "surrogate_forward")
+
+    def predict(self, dem, stream, nodata_dem, nodata_stream, rmse, practical_range, out):
+        h, w = dem.shape
+        _check(lib().cageo_surrogate_predict(self._h, _ptr(dem), _ptr(stream), w, h, nodata_dem,
+                                             nodata_stream, rmse, practical_range, _ptr(out)),
+               "surrogate_predict")
+
+    def check(self, ctx, prob, k, seed, tolerance, scratch):
+        """(accepted, l1, noise_l1) of a k-realization Monte Carlo check."""
+        acc, l1, noise = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().cageo_surrogate_check(self._h, ctx._h, _ptr(prob), k, seed, tolerance,
+                                           _ptr(scratch), ctypes.byref(acc), ctypes.byref(l1),
+                                           ctypes.byref(noise)), "surrogate_check")
+        return bool(acc.value), l1.value, noise.value
 
 
 def read_header(path: str) -> dict:

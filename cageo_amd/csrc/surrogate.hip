@@ -1,0 +1,441 @@
+// Border-probability surrogate: inference and its Monte Carlo check behind
+// the C ABI, on the library's BF16 MFMA kernels (nn.hip).
+//
+// No reference counterpart: the reference only runs the Monte Carlo.  The
+// network (a 4-level U-Net, base b channels) maps the z-scored DEM, the
+// stream mask, the error RMSE / 3 m and the practical range / 200 m to the
+// per-cell probability that a realization puts a catchment border there.
+// Layers (3x3 convolutions, ReLU, 2x2 average pooling, nearest 2x
+// upsampling folded into the decoder's convolution, skip concatenation):
+//   enc1 32->b->b, enc2 b->2b->2b, enc3 2b->4b->4b, mid 4b->8b->8b,
+//   dec3 [8b up | 4b]->4b->4b, dec2 [4b up | 2b]->2b->2b,
+//   dec1 [2b up | b]->b->b, head 1x1 b->1 with a sigmoid.
+// The same network is trained by cageo_amd/surrogate.py on these kernels;
+// weights come from its `.bin` files.
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+extern "C" {
+int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const uint16_t* in2,
+                     int c2, const uint16_t* w, const float* bias, uint16_t* out, float* outf,
+                     int n, int h, int wdt, int cout, int relu);
+int cageo_nn_avgpool2(void* stream, const uint16_t* in, uint16_t* out, int n, int h, int w, int c);
+int cageo_nn_head(void* stream, const uint16_t* act, const float* w, const float* b, float* logit,
+                  long long p, int c, int sigmoid);
+}
+
+namespace cageo {
+
+namespace {
+
+constexpr int IN_PAD = 32;  // input channels the first convolution reads (4 used)
+
+__device__ inline uint16_t to_bf16(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Partial sums (count, sum, sum of squares) of the valid DEM cells, one
+// triple per workgroup, in double.
+__global__ void k_dem_partials(const float* __restrict__ dem, long long n, float nodata,
+                               double* __restrict__ part) {
+    __shared__ double s[3][256];
+    double c = 0.0, a = 0.0, q = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float v = dem[i];
+        if (v != nodata) {
+            c += 1.0;
+            a += v;
+            q += (double)v * v;
+        }
+    }
+    s[0][threadIdx.x] = c;
+    s[1][threadIdx.x] = a;
+    s[2][threadIdx.x] = q;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[(size_t)blockIdx.x * 3 + k] = s[k][0];
+}
+
+// Mean and (population) standard deviation from the partials -> stats[0..1].
+__global__ void k_dem_stats(const double* __restrict__ part, int nparts, float* __restrict__ stats) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double c = 0.0, a = 0.0, q = 0.0;
+    for (int i = 0; i < nparts; ++i) {
+        c += part[(size_t)i * 3];
+        a += part[(size_t)i * 3 + 1];
+        q += part[(size_t)i * 3 + 2];
+    }
+    double mu = c > 0 ? a / c : 0.0;
+    double var = c > 0 ? q / c - mu * mu : 1.0;
+    double sd = var > 0 ? sqrt(var) : 0.0;
+    stats[0] = (float)mu;
+    stats[1] = (float)(sd > 1e-6 ? sd : 1e-6);
+}
+
+// Input planes, NHWC bf16 with IN_PAD channels over the padded hp x wp grid
+// (zero outside the W x H raster): z-scored DEM (nodata 0), stream mask,
+// rmse / 3, practical range / 200.
+__global__ void k_make_inputs(const float* __restrict__ dem, const float* __restrict__ stream,
+                              int W, int H, int wp, int hp, float nodata_dem, float nodata_stream,
+                              float c2, float c3, const float* __restrict__ stats,
+                              uint16_t* __restrict__ x) {
+    const float mu = stats[0], sd = stats[1];
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < (long long)wp * hp;
+         p += (long long)gridDim.x * blockDim.x) {
+        const int yy = (int)(p / wp), xx = (int)(p - (long long)yy * wp);
+        uint4 v0 = make_uint4(0, 0, 0, 0);
+        if (yy < H && xx < W) {
+            const size_t i = (size_t)yy * W + xx;
+            const float d = dem[i];
+            const float z = d != nodata_dem ? (d - mu) / sd : 0.0f;
+            const float s = stream[i] != nodata_stream ? 1.0f : 0.0f;
+            v0.x = (uint32_t)to_bf16(z) | ((uint32_t)to_bf16(s) << 16);
+            v0.y = (uint32_t)to_bf16(c2) | ((uint32_t)to_bf16(c3) << 16);
+        }
+        uint4* o = reinterpret_cast<uint4*>(x + (size_t)p * IN_PAD);
+        o[0] = v0;
+        o[1] = make_uint4(0, 0, 0, 0);
+        o[2] = make_uint4(0, 0, 0, 0);
+        o[3] = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// Crop of the padded probability map.
+__global__ void k_crop(const float* __restrict__ in, int wp, float* __restrict__ out, int W, int H) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)W * H;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i - (long long)y * W);
+        out[i] = in[(size_t)y * wp + x];
+    }
+}
+
+// Check statistics, one pair per workgroup: sum |q - c/k| and the expected
+// sum E|B/k - q| with B ~ Binomial(k, q) (the distance sampling noise alone
+// would give if q were exact).
+__global__ void k_check_partials(const float* __restrict__ q, const float* __restrict__ cnt,
+                                 long long n, int k, double* __restrict__ part) {
+    __shared__ double s[2][256];
+    double l1 = 0.0, ex = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double qi = q[i];
+        l1 += fabs(qi - (double)cnt[i] / k);
+        // sum_j C(k, j) q^j (1-q)^(k-j) |j/k - q|
+        double e = 0.0, comb = 1.0;
+        for (int j = 0; j <= k; ++j) {
+            e += comb * pow(qi, j) * pow(1.0 - qi, k - j) * fabs((double)j / k - qi);
+            comb = comb * (k - j) / (j + 1);
+        }
+        ex += e;
+    }
+    s[0][threadIdx.x] = l1;
+    s[1][threadIdx.x] = ex;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + off];
+            s[1][threadIdx.x] += s[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[(size_t)blockIdx.x * 2] = s[0][0];
+        part[(size_t)blockIdx.x * 2 + 1] = s[1][0];
+    }
+}
+
+constexpr int RED_BLOCKS = 1024;
+
+struct Layer {
+    const char* name;
+    int level;  // resolution 1/2^level
+    int up;     // first input is the coarser decoder tensor (upsampled)
+};
+// enc / mid / dec layers in execution order (names of the weights file)
+constexpr Layer LAYERS[] = {
+    {"enc1a", 0, 0}, {"enc1b", 0, 0}, {"enc2a", 1, 0}, {"enc2b", 1, 0}, {"enc3a", 2, 0},
+    {"enc3b", 2, 0}, {"mida", 3, 0},  {"midb", 3, 0},  {"dec3a", 2, 1}, {"dec3b", 2, 0},
+    {"dec2a", 1, 1}, {"dec2b", 1, 0}, {"dec1a", 0, 1}, {"dec1b", 0, 0}};
+
+}  // namespace
+
+}  // namespace cageo
+
+struct cageo_surrogate {
+    hipStream_t stream = nullptr;
+    int base = 0;
+    std::map<std::string, void*> dev;           // weights on the device
+    std::map<std::string, std::vector<int>> shape;
+    int hp = 0, wp = 0;                         // workspace for this padded size
+    std::vector<uint16_t*> act;                 // activations, see forward()
+    float* prob_pad = nullptr;
+    double* part = nullptr;
+    float* stats = nullptr;
+};
+
+namespace cageo {
+
+namespace {
+
+void free_workspace(cageo_surrogate* s) {
+    for (uint16_t* p : s->act)
+        if (p) (void)hipFree(p);
+    s->act.clear();
+    if (s->prob_pad) (void)hipFree(s->prob_pad);
+    s->prob_pad = nullptr;
+    s->hp = s->wp = 0;
+}
+
+// Activation buffers for an hp x wp input (multiples of 8): index 0 is the
+// input x; per level 4 buffers (a temporary, the skip tensor, the decoder
+// output, the pooled input of the next level) sized for the level's widest
+// tensor.
+int ensure_workspace(cageo_surrogate* s, int hp, int wp) {
+    if (s->hp == hp && s->wp == wp) return CAGEO_OK;
+    free_workspace(s);
+    const int b = s->base;
+    const size_t px0 = (size_t)hp * wp;
+    auto A = [&](size_t elems) -> int {
+        uint16_t* p = nullptr;
+        if (hipMalloc(reinterpret_cast<void**>(&p), elems * sizeof(uint16_t)) != hipSuccess)
+            return CAGEO_ERR_NOMEM;
+        s->act.push_back(p);
+        return CAGEO_OK;
+    };
+    CAGEO_TRY(A(px0 * IN_PAD));  // x
+    for (int lv = 0; lv < 4; ++lv) {
+        const size_t px = px0 >> (2 * lv);
+        const int c = b << lv;
+        CAGEO_TRY(A(px * c));  // temporary (first conv of the level)
+        CAGEO_TRY(A(px * c));  // skip / encoder output
+        CAGEO_TRY(A(px * c));  // decoder output
+        CAGEO_TRY(A(px * c));  // pooled input of the next level (c channels at 1/4 the pixels)
+    }
+    if (hipMalloc(reinterpret_cast<void**>(&s->prob_pad), px0 * sizeof(float)) != hipSuccess)
+        return CAGEO_ERR_NOMEM;
+    s->hp = hp;
+    s->wp = wp;
+    return CAGEO_OK;
+}
+
+int conv(cageo_surrogate* s, const char* name, const uint16_t* in1, int c1, int up1,
+         const uint16_t* in2, int c2, uint16_t* out, int h, int w) {
+    const std::string n(name);
+    auto wi = s->dev.find(n + ".w"), bi = s->dev.find(n + ".b");
+    if (wi == s->dev.end() || bi == s->dev.end()) return CAGEO_ERR_STATE;
+    const int cout = s->shape[n + ".b"][0];
+    return cageo_nn_conv3x3(s->stream, in1, c1, up1, in2, c2,
+                            static_cast<const uint16_t*>(wi->second),
+                            static_cast<const float*>(bi->second), out, nullptr, 1, h, w, cout, 1);
+}
+
+// Forward pass of x (bf16 [hp][wp][IN_PAD] = act[0]) into prob_pad.
+int forward(cageo_surrogate* s) {
+    const int b = s->base, hp = s->hp, wp = s->wp;
+    auto T = [&](int lv) { return s->act[1 + 4 * lv + 0]; };  // temporary
+    auto E = [&](int lv) { return s->act[1 + 4 * lv + 1]; };  // encoder output (skip)
+    auto D = [&](int lv) { return s->act[1 + 4 * lv + 2]; };  // decoder output
+    auto P = [&](int lv) { return s->act[1 + 4 * lv + 3]; };  // pooled input of level lv + 1
+    auto H = [&](int lv) { return hp >> lv; };
+    auto Wd = [&](int lv) { return wp >> lv; };
+    const uint16_t* x = s->act[0];
+    // encoder
+    CAGEO_TRY(conv(s, "enc1a", x, IN_PAD, 0, nullptr, 0, T(0), H(0), Wd(0)));
+    CAGEO_TRY(conv(s, "enc1b", T(0), b, 0, nullptr, 0, E(0), H(0), Wd(0)));
+    CAGEO_TRY(cageo_nn_avgpool2(s->stream, E(0), P(0), 1, H(0), Wd(0), b));
+    CAGEO_TRY(conv(s, "enc2a", P(0), b, 0, nullptr, 0, T(1), H(1), Wd(1)));
+    CAGEO_TRY(conv(s, "enc2b", T(1), 2 * b, 0, nullptr, 0, E(1), H(1), Wd(1)));
+    CAGEO_TRY(cageo_nn_avgpool2(s->stream, E(1), P(1), 1, H(1), Wd(1), 2 * b));
+    CAGEO_TRY(conv(s, "enc3a", P(1), 2 * b, 0, nullptr, 0, T(2), H(2), Wd(2)));
+    CAGEO_TRY(conv(s, "enc3b", T(2), 4 * b, 0, nullptr, 0, E(2), H(2), Wd(2)));
+    CAGEO_TRY(cageo_nn_avgpool2(s->stream, E(2), P(2), 1, H(2), Wd(2), 4 * b));
+    CAGEO_TRY(conv(s, "mida", P(2), 4 * b, 0, nullptr, 0, T(3), H(3), Wd(3)));
+    CAGEO_TRY(conv(s, "midb", T(3), 8 * b, 0, nullptr, 0, E(3), H(3), Wd(3)));
+    // decoder: the coarser tensor is upsampled inside the convolution's loads
+    CAGEO_TRY(conv(s, "dec3a", E(3), 8 * b, 1, E(2), 4 * b, T(2), H(2), Wd(2)));
+    CAGEO_TRY(conv(s, "dec3b", T(2), 4 * b, 0, nullptr, 0, D(2), H(2), Wd(2)));
+    CAGEO_TRY(conv(s, "dec2a", D(2), 4 * b, 1, E(1), 2 * b, T(1), H(1), Wd(1)));
+    CAGEO_TRY(conv(s, "dec2b", T(1), 2 * b, 0, nullptr, 0, D(1), H(1), Wd(1)));
+    CAGEO_TRY(conv(s, "dec1a", D(1), 2 * b, 1, E(0), b, T(0), H(0), Wd(0)));
+    CAGEO_TRY(conv(s, "dec1b", T(0), b, 0, nullptr, 0, D(0), H(0), Wd(0)));
+    return cageo_nn_head(s->stream, D(0), static_cast<const float*>(s->dev["head.w"]),
+                         static_cast<const float*>(s->dev["head.b"]), s->prob_pad,
+                         (long long)hp * wp, b, 1);
+}
+
+bool read_exact(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
+
+}  // namespace
+
+}  // namespace cageo
+
+using cageo::Context;
+
+extern "C" {
+
+int cageo_surrogate_destroy(cageo_surrogate* s) {
+    if (!s) return CAGEO_ERR_ARG;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    cageo::free_workspace(s);
+    for (auto& kv : s->dev) (void)hipFree(kv.second);
+    if (s->part) (void)hipFree(s->part);
+    if (s->stats) (void)hipFree(s->stats);
+    delete s;
+    return CAGEO_OK;
+}
+
+int cageo_surrogate_create(cageo_surrogate** out, const char* weights_path, void* stream) {
+    if (!out || !weights_path) return CAGEO_ERR_ARG;
+    FILE* f = std::fopen(weights_path, "rb");
+    if (!f) return CAGEO_ERR_ARG;
+    cageo_surrogate* s = new cageo_surrogate();
+    s->stream = reinterpret_cast<hipStream_t>(stream);
+    int st = CAGEO_OK;
+    char magic[8];
+    int32_t hdr[2];
+    if (!cageo::read_exact(f, magic, 8) || std::memcmp(magic, "CGSURR01", 8) != 0 ||
+        !cageo::read_exact(f, hdr, sizeof(hdr)) || hdr[0] <= 0 || hdr[0] % 32 || hdr[1] <= 0 ||
+        hdr[1] > 256)
+        st = CAGEO_ERR_ARG;
+    if (st == CAGEO_OK) s->base = hdr[0];
+    for (int t = 0; st == CAGEO_OK && t < hdr[1]; ++t) {
+        int32_t ln = 0, meta[2];
+        if (!cageo::read_exact(f, &ln, 4) || ln <= 0 || ln > 64) {
+            st = CAGEO_ERR_ARG;
+            break;
+        }
+        std::string name(ln, '\0');
+        if (!cageo::read_exact(f, &name[0], ln) || !cageo::read_exact(f, meta, 8) || meta[1] < 1 ||
+            meta[1] > 4 || (meta[0] != 0 && meta[0] != 1)) {
+            st = CAGEO_ERR_ARG;
+            break;
+        }
+        std::vector<int> dims(meta[1]);
+        size_t nel = 1;
+        for (int d = 0; d < meta[1]; ++d) {
+            int32_t v = 0;
+            if (!cageo::read_exact(f, &v, 4) || v <= 0 || v > (1 << 20)) {
+                st = CAGEO_ERR_ARG;
+                break;
+            }
+            dims[d] = v;
+            nel *= (size_t)v;
+        }
+        if (st != CAGEO_OK) break;
+        const size_t bytes = nel * (meta[0] == 1 ? 2 : 4);
+        std::vector<char> host(bytes);
+        void* d = nullptr;
+        if (!cageo::read_exact(f, host.data(), bytes)) st = CAGEO_ERR_ARG;
+        else if (hipMalloc(&d, bytes) != hipSuccess) st = CAGEO_ERR_NOMEM;
+        else if (hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) st = CAGEO_ERR_HIP;
+        if (d) s->dev[name] = d;
+        s->shape[name] = dims;
+    }
+    std::fclose(f);
+    // every layer of the architecture, with the shapes it implies
+    const int b = s->base;
+    for (const auto& L : cageo::LAYERS) {
+        if (st != CAGEO_OK) break;
+        const std::string n(L.name);
+        auto w = s->shape.find(n + ".w");
+        auto bb = s->shape.find(n + ".b");
+        if (w == s->shape.end() || bb == s->shape.end() || w->second.size() != 3 ||
+            w->second[0] != 9 || bb->second[0] != w->second[1] || w->second[1] % 32 ||
+            w->second[2] % 32)
+            st = CAGEO_ERR_ARG;
+    }
+    if (st == CAGEO_OK && (!s->shape.count("head.w") || s->shape["head.w"][0] != b ||
+                           !s->shape.count("head.b")))
+        st = CAGEO_ERR_ARG;
+    if (st == CAGEO_OK &&
+        (hipMalloc(reinterpret_cast<void**>(&s->part), cageo::RED_BLOCKS * 3 * sizeof(double)) !=
+             hipSuccess ||
+         hipMalloc(reinterpret_cast<void**>(&s->stats), 2 * sizeof(float)) != hipSuccess))
+        st = CAGEO_ERR_NOMEM;
+    if (st != CAGEO_OK) {
+        cageo_surrogate_destroy(s);
+        return st;
+    }
+    *out = s;
+    return CAGEO_OK;
+}
+
+int cageo_surrogate_forward(cageo_surrogate* s, const uint16_t* d_x, int h, int w, float* d_prob) {
+    if (!s || !d_x || !d_prob || h <= 0 || w <= 0 || h % 8 || w % 8) return CAGEO_ERR_ARG;
+    CAGEO_TRY(cageo::ensure_workspace(s, h, w));
+    CAGEO_HIP_TRY(hipMemcpyAsync(s->act[0], d_x, (size_t)h * w * cageo::IN_PAD * sizeof(uint16_t),
+                                 hipMemcpyDeviceToDevice, s->stream));
+    CAGEO_TRY(cageo::forward(s));
+    CAGEO_HIP_TRY(hipMemcpyAsync(d_prob, s->prob_pad, (size_t)h * w * sizeof(float),
+                                 hipMemcpyDeviceToDevice, s->stream));
+    CAGEO_HIP_TRY(hipStreamSynchronize(s->stream));
+    return CAGEO_OK;
+}
+
+int cageo_surrogate_predict(cageo_surrogate* s, const float* d_dem, const float* d_stream, int W,
+                            int H, float nodata_dem, float nodata_stream, float rmse,
+                            float practical_range, float* d_prob) {
+    if (!s || !d_dem || !d_stream || !d_prob || W <= 0 || H <= 0) return CAGEO_ERR_ARG;
+    const int hp = (H + 7) & ~7, wp = (W + 7) & ~7;  // the network halves three times
+    CAGEO_TRY(cageo::ensure_workspace(s, hp, wp));
+    const long long n = (long long)W * H;
+    hipLaunchKernelGGL(cageo::k_dem_partials, dim3(cageo::RED_BLOCKS), dim3(256), 0, s->stream,
+                       d_dem, n, nodata_dem, s->part);
+    CAGEO_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(cageo::k_dem_stats, dim3(1), dim3(64), 0, s->stream, s->part,
+                       cageo::RED_BLOCKS, s->stats);
+    CAGEO_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(cageo::k_make_inputs, dim3(2048), dim3(256), 0, s->stream, d_dem, d_stream,
+                       W, H, wp, hp, nodata_dem, nodata_stream, rmse / 3.0f,
+                       practical_range / 200.0f, s->stats, s->act[0]);
+    CAGEO_HIP_TRY(hipGetLastError());
+    CAGEO_TRY(cageo::forward(s));
+    hipLaunchKernelGGL(cageo::k_crop, dim3(2048), dim3(256), 0, s->stream, s->prob_pad, wp,
+                       d_prob, W, H);
+    CAGEO_HIP_TRY(hipGetLastError());
+    CAGEO_HIP_TRY(hipStreamSynchronize(s->stream));
+    return CAGEO_OK;
+}
+
+int cageo_surrogate_check(cageo_surrogate* s, cageo_context* ctx, const float* d_prob, int k,
+                          uint64_t seed, double tolerance, float* d_scratch, int* accepted,
+                          double* l1, double* noise_l1) {
+    if (!s || !ctx || !d_prob || !d_scratch || k < 1 || k > 1000) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    if (!c->mc_ready) return CAGEO_ERR_STATE;
+    CAGEO_HIP_TRY(hipMemsetAsync(d_scratch, 0, c->n * sizeof(float), c->stream));
+    CAGEO_TRY(cageo_run_realizations(ctx, d_scratch, k, seed, 0));
+    hipLaunchKernelGGL(cageo::k_check_partials, dim3(cageo::RED_BLOCKS), dim3(256), 0, c->stream,
+                       d_prob, d_scratch, (long long)c->n, k, s->part);
+    CAGEO_HIP_TRY(hipGetLastError());
+    std::vector<double> part((size_t)cageo::RED_BLOCKS * 2);
+    CAGEO_HIP_TRY(hipMemcpyAsync(part.data(), s->part, part.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
+    double a = 0.0, e = 0.0;
+    for (int i = 0; i < cageo::RED_BLOCKS; ++i) {
+        a += part[(size_t)i * 2];
+        e += part[(size_t)i * 2 + 1];
+    }
+    a /= (double)c->n;
+    e /= (double)c->n;
+    if (l1) *l1 = a;
+    if (noise_l1) *noise_l1 = e;
+    if (accepted) *accepted = a <= e + tolerance;
+    return CAGEO_OK;
+}
+
+}  // extern "C"

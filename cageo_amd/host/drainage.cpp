@@ -68,6 +68,10 @@ struct Options {
     int synthW = 0, synthH = 0;
     uint64_t synthSeed = 0;
     int accThreshold = 2000;
+    bool fastConvolution = false;
+    std::string surrogate;
+    int surrogateCheck = 4;
+    double surrogateTolerance = 0.01;
 };
 
 const char* kHelp =
@@ -95,7 +99,15 @@ const char* kHelp =
     "  --checkpoint arg                    Resume-point file to write\n"
     "  --checkpoint-every arg (=0)         Write the resume point every K iterations\n"
     "  --resume arg                        Continue from a resume-point file\n"
-    "  --synth W,H,SEED                    Generate synthetic inputs under --dem-path\n";
+    "  --synth W,H,SEED                    Generate synthetic inputs under --dem-path\n"
+    "  --fast-convolution                  Low-rank separable exponential convolution\n"
+    "                                      (within 1e-5 of the exact order, not bit-exact)\n"
+    "  --surrogate arg                     Border-probability network weights (.bin):\n"
+    "                                      predict, check with a short Monte Carlo, and\n"
+    "                                      write probability x iterations; the full Monte\n"
+    "                                      Carlo runs if the check fails\n"
+    "  --surrogate-check arg (=4)          Realizations of the check\n"
+    "  --surrogate-tolerance arg (=0.01)   Allowed L1 above the check's sampling noise\n";
 
 [[noreturn]] void fail(const std::string& msg) {
     std::cout << msg << std::endl;
@@ -134,7 +146,8 @@ Options parseArgs(int argc, char** argv) {
     bool haveOutputFile = false, haveOutputOption = false, haveFilter = false;
     const std::map<std::string, std::string> shortNames = {{"i", "iterations"}, {"o", "output-file"}};
     const std::map<std::string, bool> implicitBool = {{"gpu", true}, {"cpu", true}, {"staged", true},
-                                                      {"timing", true}, {"help", true}};
+                                                      {"timing", true}, {"help", true},
+                                                      {"fast-convolution", true}};
     for (int k = 1; k < argc; ++k) {
         std::string a = argv[k], name, value;
         bool hasValue = false;
@@ -160,7 +173,8 @@ Options parseArgs(int argc, char** argv) {
         static const char* const known[] = {
             "help", "dem", "stream", "dem-path", "stream-path", "filter-type", "practical-range",
             "error-mean", "error-standard-deviation", "iterations", "gpu", "cpu", "staged", "timing",
-            "output-file", "output-option", "seed", "checkpoint", "checkpoint-every", "resume", "synth"};
+            "output-file", "output-option", "seed", "checkpoint", "checkpoint-every", "resume", "synth",
+            "fast-convolution", "surrogate", "surrogate-check", "surrogate-tolerance"};
         bool isKnown = false;
         for (const char* kname : known) isKnown = isKnown || name == kname;
         if (!isKnown) fail("unrecognised option '--" + name + "'");
@@ -191,6 +205,10 @@ Options parseArgs(int argc, char** argv) {
         else if (name == "checkpoint") o.checkpoint = value;
         else if (name == "checkpoint-every") o.checkpointEvery = parseNum<int>(name, value);
         else if (name == "resume") o.resume = value;
+        else if (name == "fast-convolution") o.fastConvolution = hasValue ? parseBool(value) : true;
+        else if (name == "surrogate") o.surrogate = value;
+        else if (name == "surrogate-check") o.surrogateCheck = parseNum<int>(name, value);
+        else if (name == "surrogate-tolerance") o.surrogateTolerance = parseNum<double>(name, value);
         else if (name == "synth") {
             int w = 0, h = 0;
             unsigned long long s = 0;
@@ -226,6 +244,8 @@ Options parseArgs(int argc, char** argv) {
     if (o.useCPU) o.useGPU = false;
     if (o.iterations < 0) fail("The number of iterations must not be negative");
     if (o.checkpointEvery < 0) fail("--checkpoint-every must not be negative");
+    if (o.surrogateCheck < 1 || o.surrogateCheck > 1000) fail("--surrogate-check must be in 1..1000");
+    if (!o.surrogate.empty() && o.staged) fail("--surrogate runs on the fused path, not with --staged");
     return o;
 }
 
@@ -299,6 +319,49 @@ void writeOutput(const Options& o, int iter, CatchmentBorder_t* mc,
     } else {
         writeRaster(name.str() + ".bin", mc->getDataConst(), (size_t)mc->size);
     }
+}
+
+// Surrogate prediction with its Monte Carlo check (cageo_surrogate_*).  On
+// acceptance `mc` holds probability x iterations (the expected border count
+// of the run) and true is returned; otherwise the caller runs the Monte
+// Carlo.
+bool runSurrogate(const Options& o, cageo_context* ctx, int W, int H, RasterDEM_t& dem,
+                  StreamData_t& streams, CatchmentBorder_t* mc) {
+    const auto t0 = std::chrono::steady_clock::now();
+    cageo_surrogate* s = nullptr;
+    check(cageo_surrogate_create(&s, o.surrogate.c_str(), nullptr), "surrogate weights");
+    if (o.filterModel != GAUSSIAN)
+        std::cout << "Surrogate: note, the packaged weights are trained for the Gaussian filter"
+                  << std::endl;
+    float *dProb = nullptr, *dScratch = nullptr;
+    hip_check(hipMalloc(&dProb, (size_t)W * H * sizeof(float)), "hipMalloc");
+    hip_check(hipMalloc(&dScratch, (size_t)W * H * sizeof(float)), "hipMalloc");
+    check(cageo_surrogate_predict(s, dem.getDeviceDataConst(), streams.getDeviceDataConst(), W, H,
+                                  (float)G::noDataValueDEM, (float)G::noDataValueStream, o.errorStd,
+                                  o.practicalRange, dProb),
+          "surrogate prediction");
+    int accepted = 0;
+    double l1 = 0.0, noise = 0.0;
+    check(cageo_surrogate_check(s, ctx, dProb, o.surrogateCheck, (uint64_t)o.seed ^ 0x5eedull,
+                                o.surrogateTolerance, dScratch, &accepted, &l1, &noise),
+          "surrogate check");
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::cout << "Surrogate: " << o.surrogateCheck << "-realization check L1 " << std::setprecision(5)
+              << l1 << " vs sampling noise " << noise << " + " << o.surrogateTolerance << " -> "
+              << (accepted ? "accepted" : "rejected, running the Monte Carlo") << " ("
+              << std::setprecision(1) << std::fixed << ms << " ms)" << std::endl;
+    std::cout.unsetf(std::ios::fixed);
+    if (accepted) {
+        std::vector<float> p((size_t)W * H);
+        hip_check(hipMemcpy(p.data(), dProb, p.size() * sizeof(float), hipMemcpyDeviceToHost),
+                  "hipMemcpy");
+        float* out = mc->getData();
+        for (size_t i = 0; i < p.size(); ++i) out[i] = p[i] * (float)o.iterations;
+    }
+    (void)hipFree(dProb);
+    (void)hipFree(dScratch);
+    cageo_surrogate_destroy(s);
+    return accepted != 0;
 }
 
 void synthesize(const Options& o) {
@@ -482,7 +545,14 @@ int run(int argc, char** argv) {
                              (float)G::noDataValueDEM, (float)G::noDataValueStream, o.bevelRadius,
                              o.errorMean, o.errorStd),
               "Monte Carlo set-up");
+        if (o.fastConvolution)
+            check(cageo_context_set_conv_mode(e.ctx, CAGEO_CONV_FAST), "fast convolution");
         check(cageo_stage_times(e.ctx, nullptr, o.timing ? 1 : 0), "stage timing");
+        if (!o.surrogate.empty() && first < o.iterations && runSurrogate(o, e.ctx, W, H, dem, streams,
+                                                                          &monteCarloBorder)) {
+            writeOutput(o, o.iterations - 1, &monteCarloBorder, toBytes.get(), &byteTimer, &jpegTimer);
+            first = o.iterations;  // no Monte Carlo left to run
+        }
         Timer* stageTimers[7] = {&rfTimer, &convTimer, &realTimer, &pfTimer, &frTimer, &ftTimer, &cbTimer};
         realTimer.setTimingTarget("Realistic DEM + stream burning");
         int iter = first;

@@ -36,7 +36,7 @@ IN_PAD = 32  # the first convolution reads 32 channels (4 used)
 
 # Weights trained by `python -m cageo_amd.train_surrogate` (64 synthetic DEMs
 # of 2048^2, 200-realization Monte Carlo labels, 4000 Adam steps).
-DEFAULT_WEIGHTS = os.path.join(os.path.dirname(__file__), "weights", "surrogate_b32.npz")
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(__file__), "weights", "surrogate_b32.bin")
 
 
 def _p(t):
@@ -420,11 +420,39 @@ def train(torch, x4, y, steps=1500, batch=8, lr=2e-3, seed=0, base=32, log_every
     return net, hist
 
 
-def save_params(net, path):
-    """Weights -> .npz: convolution weights as bfloat16 bit patterns (the
-    kernels read bf16, so inference is unchanged), biases and head fp32."""
+WEIGHTS_MAGIC = b"CGSURR01"
+
+
+def save_params(net_or_params, path):
+    """Weights -> file.  `.bin`: the library's format (read by
+    cageo_surrogate_create): magic "CGSURR01", int32 base, int32 count, then
+    per tensor int32 name length, name, int32 dtype (0 float32, 1 bfloat16
+    bit patterns), int32 ndim, int32 dims, little-endian data.  `.npz`:
+    the same tensors for numpy.  Convolution weights are stored as bfloat16
+    (the kernels read bf16, so inference is unchanged), biases and the head
+    as float32."""
+    import struct
+    params = net_or_params.params_numpy() if hasattr(net_or_params, "params_numpy") else net_or_params
+    base = int(params["enc1b.b"].size)
+    names = [n + sfx for n, *_ in layer_spec(base) for sfx in (".w", ".b")] + ["head.w", "head.b"]
+    if str(path).endswith(".bin"):
+        with open(path, "wb") as f:
+            f.write(WEIGHTS_MAGIC)
+            f.write(struct.pack("<ii", base, len(names)))
+            for k in names:
+                a = np.asarray(params[k], np.float32)
+                bf = k.endswith(".w") and not k.startswith("head")
+                data = ((bf16_round(a).view(np.uint32) >> 16).astype("<u2") if bf
+                        else a.astype("<f4"))
+                nb = k.encode()
+                f.write(struct.pack("<i", len(nb)) + nb)
+                f.write(struct.pack("<ii", 1 if bf else 0, a.ndim))
+                f.write(struct.pack("<%di" % a.ndim, *a.shape))
+                f.write(data.tobytes())
+        return
     out = {}
-    for k, v in net.params_numpy().items():
+    for k in names:
+        v = params[k]
         if k.endswith(".w") and not k.startswith("head"):
             out[k.replace(".", "__") + "__bf16"] = (bf16_round(v).view(np.uint32) >> 16).astype(np.uint16)
         else:
@@ -433,8 +461,27 @@ def save_params(net, path):
 
 
 def load_params(path):
-    z = np.load(path)
+    """Weights from a `.bin` (library format) or `.npz` file, as float32."""
+    import struct
     params = {}
+    if str(path).endswith(".bin"):
+        with open(path, "rb") as f:
+            if f.read(8) != WEIGHTS_MAGIC:
+                raise ValueError(f"{path}: not a surrogate weights file")
+            _, count = struct.unpack("<ii", f.read(8))
+            for _ in range(count):
+                (ln,) = struct.unpack("<i", f.read(4))
+                name = f.read(ln).decode()
+                dtype, ndim = struct.unpack("<ii", f.read(8))
+                shape = struct.unpack("<%di" % ndim, f.read(4 * ndim))
+                nel = int(np.prod(shape))
+                if dtype == 1:
+                    raw = np.frombuffer(f.read(2 * nel), "<u2").astype(np.uint32) << 16
+                    params[name] = raw.view(np.float32).reshape(shape)
+                else:
+                    params[name] = np.frombuffer(f.read(4 * nel), "<f4").reshape(shape).copy()
+        return params
+    z = np.load(path)
     for k in z.files:
         if k.endswith("__bf16"):
             params[k[:-6].replace("__", ".")] = (z[k].astype(np.uint32) << 16).view(np.float32)

@@ -293,6 +293,36 @@ int cageo_synth_dem(int width, int height, uint64_t seed, float cell_size,
                     int acc_threshold, float nodata_stream, float* dem_out,
                     float* stream_out);
 
+/* ---- border-probability surrogate: inference and check ----------------
+ * No reference counterpart (the reference only runs the Monte Carlo).  A
+ * U-Net on the cageo_nn_* kernels maps the z-scored DEM, the stream mask,
+ * RMSE / 3 m and practical range / 200 m to the per-cell probability that
+ * a realization puts a catchment border there (the Monte Carlo's count /
+ * realizations).  Weights: the `.bin` files cageo_amd/surrogate.py writes
+ * (magic "CGSURR01"; convolution weights bf16, biases float32).  Calls on
+ * one surrogate must not overlap (it owns one workspace). */
+typedef struct cageo_surrogate cageo_surrogate;
+int cageo_surrogate_create(cageo_surrogate** out, const char* weights_path, void* hip_stream);
+int cageo_surrogate_destroy(cageo_surrogate* s);
+/* Forward pass of prepared inputs: bf16 NHWC [h][w][32] (h, w multiples of
+ * 8) -> probabilities [h][w] (float). */
+int cageo_surrogate_forward(cageo_surrogate* s, const uint16_t* d_x, int h, int w, float* d_prob);
+/* DEM + stream raster (W x H, any size) + error model -> probabilities
+ * [H][W]: input planes built on the device (mean and population standard
+ * deviation of the valid DEM cells), padded to multiples of 8, cropped. */
+int cageo_surrogate_predict(cageo_surrogate* s, const float* d_dem, const float* d_stream, int W,
+                            int H, float nodata_dem, float nodata_stream, float rmse,
+                            float practical_range, float* d_prob);
+/* Monte Carlo check of a probability map: k realizations of `ctx` (set up
+ * with cageo_mc_setup on the same DEM; counts into d_scratch, W x H) give
+ * l1 = mean |p - count/k| and noise_l1 = mean E|B/k - p|, B ~ Binomial(k,
+ * p) (the distance sampling alone would give if p were exact).  accepted =
+ * l1 <= noise_l1 + tolerance; on rejection the caller runs the Monte Carlo
+ * instead. */
+int cageo_surrogate_check(cageo_surrogate* s, cageo_context* ctx, const float* d_prob, int k,
+                          uint64_t seed, double tolerance, float* d_scratch, int* accepted,
+                          double* l1, double* noise_l1);
+
 #ifdef __cplusplus
 }
 #endif

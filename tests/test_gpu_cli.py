@@ -144,3 +144,47 @@ def test_to_byte_array(torch_cuda, case):
         ctx.to_byte_array(out, torch.from_numpy(x).cuda())
         got = out.cpu().numpy()
     assert np.array_equal(got, reference_bytes(x))
+
+
+def test_cli_surrogate_accept_and_fallback(inputs, torch_cuda):
+    """--surrogate: an accepted prediction is written as probability x
+    iterations (the C-ABI prediction of the same inputs); a rejected one
+    leaves the output to the Monte Carlo, equal to a run without the
+    surrogate.  Acceptance and rejection are forced by the tolerance."""
+    import cageo_amd as ca
+    from cageo_amd import surrogate as sg
+    torch = torch_cuda
+    d = str(inputs)
+    announce("cli surrogate accept / fallback")
+    common = ["--dem-path", ".", "-i", "6", "--seed", "9", "--error-standard-deviation", "1.5",
+              "--surrogate", sg.DEFAULT_WEIGHTS]
+    r = run_cli(d, *common, "--surrogate-tolerance", "10", "-o", "sur_ok.bin")
+    assert "accepted" in r.stdout, r.stdout
+    got = read_bin(os.path.join(d, "sur_ok.bin"))
+    dem = torch.from_numpy(read_bin(os.path.join(d, "dem10.bin"))).cuda()
+    st = torch.from_numpy(read_bin(os.path.join(d, "streams.bin"))).cuda()
+    p = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    with ca.Surrogate(sg.DEFAULT_WEIGHTS) as s:
+        s.predict(dem, st, -9999.0, -9999.0, 1.5, 60.0, p)
+    torch.cuda.synchronize()
+    assert np.array_equal(got, p.cpu().numpy() * np.float32(6))
+    r = run_cli(d, *common, "--surrogate-tolerance", "-1", "-o", "sur_no.bin")
+    assert "rejected" in r.stdout, r.stdout
+    run_cli(d, "--dem-path", ".", "-i", "6", "--seed", "9", "--error-standard-deviation", "1.5",
+            "-o", "plain.bin")
+    assert np.array_equal(read_bin(os.path.join(d, "sur_no.bin")),
+                          read_bin(os.path.join(d, "plain.bin")))
+
+
+def test_cli_fast_convolution(inputs, torch_cuda):
+    """--fast-convolution with the exponential filter runs and gives counts
+    in [0, iterations] whose total is within 1 % of the exact order's."""
+    d = str(inputs)
+    announce("cli fast convolution")
+    base = ["--dem-path", ".", "-i", "3", "--seed", "4", "--filter-type", "Exponential",
+            "--practical-range", "200", "--error-standard-deviation", "2.5"]
+    run_cli(d, *base, "-o", "exact.bin")
+    run_cli(d, *base, "--fast-convolution", "-o", "fast.bin")
+    a, b = read_bin(os.path.join(d, "exact.bin")), read_bin(os.path.join(d, "fast.bin"))
+    assert b.min() >= 0 and b.max() <= 3
+    assert abs(float(b.sum()) - float(a.sum())) <= 0.01 * float(a.sum())

@@ -229,3 +229,41 @@ def test_surrogate_fallback(env):
     assert not info["accepted"] and "fallback_s" in info, info
     ref = sg.mc_frequency(torch, [dem], [st], [1.5], [80.0], 64, 41, on_device=True)[0]
     assert torch.equal(m, ref)
+
+
+def test_native_surrogate_matches_python(env):
+    """The C-ABI surrogate (cageo_surrogate_*, C++ composition of the same
+    kernels) against the Python composition with the packaged weights:
+    bit-identical forward pass on the same bf16 input; predictions from the
+    DEM within the rounding of the input planes (mean / deviation in double
+    there, float32 here); the Monte Carlo check statistics equal to the
+    Python check's for the same seed (float vs double sums)."""
+    import cageo_amd as ca
+    torch, sg, L = env
+    announce("native surrogate vs python 256x200")
+    dem, st = ca.synth_dem(200, 256, 5, 10.0, 300, -9999.0)
+    net = sg.load_net(torch)
+    x = sg.device_inputs(torch, dem, st, 1.8, 60.0)  # [1][256][200][32] float32
+    xb = net._to_bf16(x)
+    ref = net.predict(x)[0]
+    out = torch.empty((256, 200), dtype=torch.float32, device="cuda")
+    with ca.Surrogate(sg.DEFAULT_WEIGHTS, torch.cuda.current_stream()) as s:
+        s.forward(xb, 256, 200, out)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+        ddem, dst = torch.from_numpy(dem).cuda(), torch.from_numpy(st).cuda()
+        p = torch.empty((256, 200), dtype=torch.float32, device="cuda")
+        s.predict(ddem, dst, -9999.0, -9999.0, 1.8, 60.0, p)
+        torch.cuda.synchronize()
+        assert float((p - ref).abs().max()) <= 0.02 and float((p - ref).abs().mean()) <= 1e-3
+        r = ca.filter_radius(60.0, 10.0)
+        with ca.Context(200, 256, torch.cuda.current_stream()) as ctx:
+            ctx.set_filter(ca.GAUSSIAN, r, ca.create_filter(ca.GAUSSIAN, 60.0, 10.0, r))
+            ctx.mc_setup(ddem, dst, -9999.0, -9999.0, 2, 0.0, 1.8)
+            scratch = torch.empty((256, 200), dtype=torch.float32, device="cuda")
+            acc, l1, noise = s.check(ctx, p, 4, 21, 0.01, scratch)
+        chk = sg.mc_frequency(torch, [dem], [st], [1.8], [60.0], 4, 21, on_device=True)[0]
+        l1_py = float((p - chk).abs().mean())
+        noise_py = sg.expected_l1_of_sample(p.double(), 4)
+        assert abs(l1 - l1_py) <= 1e-6 and abs(noise - noise_py) <= 1e-6, (l1, l1_py, noise, noise_py)
+        assert acc == (l1 <= noise + 0.01)
