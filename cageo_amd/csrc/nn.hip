@@ -53,11 +53,13 @@ __device__ inline uint4 load_in8(const ConvIn& ci, int n, int y, int x, int H, i
 
 __device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-// Forward 3x3 convolution (stride 1, zero padding 1).  One wave: 16 pixels x
-// 16*NT output channels; a workgroup of 4 waves covers 64 pixels.
+// Forward 3x3 convolution (stride 1, zero padding 1).  One wave: 16*MT
+// consecutive pixels x 16*NT output channels (MT x NT accumulator tiles, so
+// each operand fragment loaded feeds NT or MT MFMAs); a workgroup of 4 waves
+// covers 64*MT pixels.
 // w: [9][Cout][Cin] bf16 (Cin = C1 + C2), bias: [Cout] fp32 or null.
 // out: [N][H][W][Cout] bf16 (ReLU applied when relu), and/or outf fp32.
-template <int NT>
+template <int NT, int MT>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvIn ci, const uint16_t* __restrict__ w,
                                                  const float* __restrict__ bias,
                                                  uint16_t* __restrict__ out,
@@ -65,51 +67,67 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn ci, const uint16_t* __re
                                                  int Cout, int relu) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long M = (long long)N * H * W;
-    const long long p0 = ((long long)blockIdx.x * 4 + wave) * 16;
+    const long long p0 = ((long long)blockIdx.x * 4 + wave) * 16 * MT;
     const int co0 = blockIdx.y * 16 * NT;
     if (p0 >= M) return;  // whole wave out of range (uniform)
     const int Cin = ci.C1 + ci.C2;
     const int row = lane & 15, kb = lane >> 4;
-    const long long p = p0 + row;
-    int n = 0, y = 0, x = 0;
-    const bool pvalid = p < M;
-    if (pvalid) {
-        n = (int)(p / ((long long)H * W));
-        const int r = (int)(p - (long long)n * H * W);
-        y = r / W;
-        x = r - y * W;
-    }
-    f32x4 acc[NT];
+    int n[MT], y[MT], x[MT];
+    bool pvalid[MT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; ++m) {
+        const long long p = p0 + 16 * m + row;
+        pvalid[m] = p < M;
+        n[m] = y[m] = x[m] = 0;
+        if (pvalid[m]) {
+            n[m] = (int)(p / ((long long)H * W));
+            const int r = (int)(p - (long long)n[m] * H * W);
+            y[m] = r / W;
+            x[m] = r - y[m] * W;
+        }
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int tap = 0; tap < 9; ++tap) {
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
         const uint16_t* wt = w + (size_t)tap * Cout * Cin;
         for (int c = 0; c < Cin; c += 32) {
             const int cc = c + 8 * kb;
-            const uint4 av = pvalid ? load_in8(ci, n, y + dy, x + dx, H, W, cc) : make_uint4(0, 0, 0, 0);
-            const bf16x8 a = as_bf16x8(av);
+            bf16x8 a[MT], b[NT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                a[m] = as_bf16x8(pvalid[m] ? load_in8(ci, n[m], y[m] + dy, x[m] + dx, H, W, cc)
+                                           : make_uint4(0, 0, 0, 0));
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const int co = co0 + t * 16 + row;
-                const uint4 bv = *reinterpret_cast<const uint4*>(wt + (size_t)co * Cin + cc);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, as_bf16x8(bv), acc[t], 0, 0, 0);
+                b[t] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + (size_t)co * Cin + cc));
             }
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
         }
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int co = co0 + t * 16 + row;  // accumulator column = lane % 16
-        const float b = bias ? bias[co] : 0.0f;
+        const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const long long pp = p0 + 4 * kb + r;  // accumulator row
-            if (pp >= M) continue;
-            float v = acc[t][r] + b;
-            if (relu) v = fmaxf(v, 0.0f);
-            if (out) out[pp * Cout + co] = f2bf(v);
-            if (outf) outf[pp * Cout + co] = v;
-        }
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long pp = p0 + 16 * m + 4 * kb + r;  // accumulator row
+                if (pp >= M) continue;
+                float v = acc[m][t][r] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                if (out) out[pp * Cout + co] = f2bf(v);
+                if (outf) outf[pp * Cout + co] = v;
+            }
     }
 }
 
@@ -386,15 +404,21 @@ int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const u
     if (c1 <= 0 || c1 % 32 || c2 < 0 || c2 % 32 || (c2 && !in2) || cout % 16) return CAGEO_ERR_ARG;
     ConvIn ci{in1, in2, c1, c2, up1};
     const long long M = (long long)n * h * wdt;
-    const int nt = cout % 64 == 0 ? 4 : 1;
-    dim3 grid((unsigned)((M + 63) / 64), (unsigned)(cout / (16 * nt)));
+    // output channel tile: 64 (NT 4), 32 (NT 2) or 16 (NT 1); 4 pixel tiles
+    // per wave (256 pixels per workgroup; 8 lowered occupancy and time)
+    constexpr int MT = 4;
+    const int nt = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+    dim3 grid((unsigned)((M + 64 * MT - 1) / (64 * MT)), (unsigned)(cout / (16 * nt)));
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (nt == 4)
-        hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, ci, w, bias, out, outf, n, h, wdt,
-                           cout, relu);
+        hipLaunchKernelGGL((k_conv3x3<4, MT>), grid, dim3(256), 0, s, ci, w, bias, out, outf, n, h,
+                           wdt, cout, relu);
+    else if (nt == 2)
+        hipLaunchKernelGGL((k_conv3x3<2, MT>), grid, dim3(256), 0, s, ci, w, bias, out, outf, n, h,
+                           wdt, cout, relu);
     else
-        hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, s, ci, w, bias, out, outf, n, h, wdt,
-                           cout, relu);
+        hipLaunchKernelGGL((k_conv3x3<1, MT>), grid, dim3(256), 0, s, ci, w, bias, out, outf, n, h,
+                           wdt, cout, relu);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
