@@ -33,6 +33,7 @@ import argparse
 import csv
 import glob
 import json
+import math
 import os
 import shutil
 import subprocess
@@ -83,6 +84,13 @@ def parse():
     p.add_argument("--batch-tiles", type=int, default=64,
                    help="tiles of the batched workload (0: skip)")
     p.add_argument("--batch-realizations", type=int, default=100)
+    p.add_argument("--large", action="store_true",
+                   help="also measure the large-tile configuration (16384^2, exponential PR "
+                        "200 m, RMSE 2.5 m): Monte Carlo vs surrogate + refinement (minutes)")
+    p.add_argument("--large-realizations", type=int, default=1000)
+    p.add_argument("--large-weights",
+                   default=os.path.join(ROOT, "cageo_amd", "weights", "surrogate_exp_b32.bin"),
+                   help="surrogate weights for the exponential filter (.bin)")
     p.add_argument("--surrogate-realizations", type=int, default=1000,
                    help="Monte Carlo size the surrogate is judged against (0: skip)")
     p.add_argument("--traffic-out", default=None,
@@ -96,8 +104,29 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(size, sample, seed):
-    """Time the CPU oracle on `sample` realizations (1 thread) in a child."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Threads for the all-core figure: the CPUs this process may run on,
+    at most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(size, sample, seed, threads=1):
+    """Time the CPU oracle on `sample` realizations with `threads` OpenMP
+    threads in a child."""
     code = f"""
 import os, sys, time, json
 sys.path.insert(0, {ROOT!r})
@@ -114,7 +143,7 @@ for k in range({sample}):
 dt = time.perf_counter() - t
 print(json.dumps({{"seconds": dt, "n": {sample}}}))
 """
-    env = dict(os.environ, OMP_NUM_THREADS="1")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                          check=True)
     res = json.loads(out.stdout.strip().splitlines()[-1])
@@ -246,6 +275,109 @@ def batched_leg(torch, ca, args):
                          "mcells_per_s": round(cells * args.batch_realizations / dt / 1e6, 1)}
         out["identical"] = all(torch.equal(a, c) for a, c in zip(res["graph"], res["per_tile"]))
         out["graph_speedup_vs_per_tile"] = round(out["per_tile"]["s"] / out["graph"]["s"], 3)
+    return out
+
+
+def large_leg(torch, ca, args):
+    """Large tile, both growth patterns: a 16384^2 DEM (the generator's
+    8192^2 DEM, seed --seed, mirrored across x and y: continuous heights and
+    streams at the seams), exponential filter PR 200 m, RMSE 2.5 m.
+    - Monte Carlo: --large-realizations realizations with the fast
+      convolution (within 1e-6 of the exact field), run in chunks of 50 (one
+      progress line each); its frequency map is the reference.  The exact
+      order is timed on 10 realizations beside.
+    - Surrogate (exponential weights) + check on 1024^2 windows with 4
+      realizations; windows whose L1 exceeds the sampling noise + 0.01 are
+      refined by a 200-realization Monte Carlo of the whole DEM.
+    Times are wall times with the device synchronised."""
+    import numpy as np
+    log = lambda m: print(f"large: {m}", file=sys.stderr, flush=True)
+    half = 8192
+    t0 = time.perf_counter()
+    d0, s0 = ca.synth_dem(half, half, args.seed, 10.0, 2000, NODATA)
+    log(f"synthetic {half}^2 in {time.perf_counter() - t0:.1f} s")
+    dem = np.block([[d0, d0[:, ::-1]], [d0[::-1, :], d0[::-1, ::-1]]])
+    st = np.block([[s0, s0[:, ::-1]], [s0[::-1, :], s0[::-1, ::-1]]])
+    del d0, s0
+    H, W = dem.shape
+    ddem, dst = torch.from_numpy(np.ascontiguousarray(dem)).cuda(), torch.from_numpy(np.ascontiguousarray(st)).cuda()
+    r = ca.filter_radius(200.0, 10.0)
+    taps = ca.create_filter(ca.EXPONENTIAL, 200.0, 10.0, r)
+    out = {"dem": f"{W}x{H} (8192^2 generator DEM mirrored)", "filter": "exponential PR 200 m",
+           "error_std_m": 2.5}
+    seed = args.seed + 11
+    n = args.large_realizations
+    with ca.Context(W, H, torch.cuda.current_stream()) as ctx:
+        ctx.set_filter(ca.EXPONENTIAL, r, taps)
+        ctx.mc_setup(ddem, dst, NODATA, NODATA, 2, 0.0, 2.5)
+        scratch = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+        ctx.run_realizations(scratch, 1, seed, 0)  # exact order, warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ctx.run_realizations(scratch, 10, seed, 1)
+        torch.cuda.synchronize()
+        exact_per = (time.perf_counter() - t) / 10
+        out["mc_exact_order"] = {"realizations_per_s": round(1 / exact_per, 3),
+                                 "est_1000_s": round(1000 * exact_per, 1), "timed": 10}
+        log(f"exact order {1 / exact_per:.2f} realizations/s")
+        ctx.set_conv_mode(ca.CONV_FAST)
+        mc = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+        ctx.run_realizations(scratch, 1, seed, 0)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(0, n, 50):
+            ctx.run_realizations(mc, min(50, n - k), seed, k)
+            torch.cuda.synchronize()
+            log(f"Monte Carlo {min(k + 50, n)}/{n} ({time.perf_counter() - t:.1f} s)")
+        mc_s = time.perf_counter() - t
+        ref = mc / float(n)
+        out["mc"] = {"realizations": n, "convolution": f"fast ({ctx.conv_terms()} terms)",
+                     "s": round(mc_s, 2), "realizations_per_s": round(n / mc_s, 3),
+                     "mcells_per_s": round(W * H * n / mc_s / 1e6, 1)}
+        if args.large_weights and os.path.exists(args.large_weights):
+            K, win, tol = 4, 1024, 0.01
+            with ca.Surrogate(args.large_weights, torch.cuda.current_stream()) as s:
+                p = torch.empty((H, W), dtype=torch.float32, device="cuda")
+                s.predict(ddem[:1024, :1024].contiguous(), dst[:1024, :1024].contiguous(), NODATA,
+                          NODATA, 2.5, 200.0, p[:1024, :1024].contiguous())  # warm-up
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                s.predict(ddem, dst, NODATA, NODATA, 2.5, 200.0, p)
+                torch.cuda.synchronize()
+                pred_s = time.perf_counter() - t
+                cnt = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+                ctx.run_realizations(cnt, K, seed + 1, 0)
+                f = cnt / K
+                qd = p.double()
+                noise = sum(math.comb(K, j) * qd ** j * (1 - qd) ** (K - j) * (j / K - qd).abs()
+                            for j in range(K + 1))
+                l1w = (p - f).abs().reshape(H // win, win, W // win, win).mean(dim=(1, 3))
+                nw = noise.reshape(H // win, win, W // win, win).mean(dim=(1, 3))
+                bad = (l1w > nw + tol)
+                torch.cuda.synchronize()
+                check_s = time.perf_counter() - t - pred_s
+                nbad = int(bad.sum())
+                final = p.clone()
+                refine_s = 0.0
+                if nbad:
+                    t2 = time.perf_counter()
+                    c200 = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+                    ctx.run_realizations(c200, 200, seed + 2, 0)
+                    m = bad.repeat_interleave(win, 0).repeat_interleave(win, 1)
+                    final = torch.where(m, c200 / 200.0, p)
+                    torch.cuda.synchronize()
+                    refine_s = time.perf_counter() - t2
+                sur_s = pred_s + check_s + refine_s
+                log(f"surrogate {sur_s:.2f} s, {nbad} of {bad.numel()} windows refined")
+                out["surrogate"] = {
+                    "weights": os.path.relpath(os.path.abspath(args.large_weights), ROOT),
+                    "predict_s": round(pred_s, 3), "check_s": round(check_s, 3),
+                    "windows": int(bad.numel()), "windows_refined": nbad,
+                    "refine_s": round(refine_s, 2), "total_s": round(sur_s, 2),
+                    "speedup_vs_mc": round(mc_s / sur_s, 1),
+                    "l1_prediction": round(float((p - ref).abs().mean()), 5),
+                    "l1_final": round(float((final - ref).abs().mean()), 5),
+                    "max_dev_final": round(float((final - ref).abs().max()), 4)}
     return out
 
 
@@ -429,6 +561,8 @@ def main():
         line["batched"] = batched_leg(torch, ca, args)
     if args.surrogate_realizations > 0:
         line["surrogate"] = surrogate_leg(torch, ca, dem, stream, args)
+    if args.large:
+        line["large_tile"] = large_leg(torch, ca, args)
     if args.cpu_sample > 0 and oracle_ok:
         cps, secs = cpu_baseline(size, args.cpu_sample, args.seed)
         line["cpu_baseline"] = {
@@ -437,6 +571,15 @@ def main():
                       f"workload on the CPU oracle, OMP_NUM_THREADS=1 ({secs:.1f} s)",
         }
         line["speedup_vs_cpu_1core"] = round(real_s / cps, 1)
+        nt = cpu_threads()
+        cpa, secs_a = cpu_baseline(size, args.cpu_sample, args.seed, nt)
+        line["cpu_all_cores"] = {
+            "value": round(cpa, 5), "unit": "realizations/s", "cores": nt,
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"{args.cpu_sample} realizations, OMP_NUM_THREADS={nt} ({secs_a:.1f} s); "
+                      "the oracle's OpenMP loops follow the reference's (pit fill and tracing "
+                      "are sequential there too)",
+            "gpu_speedup": round(real_s / cpa, 1)}
     print(json.dumps(line), flush=True)
 
 

@@ -219,10 +219,10 @@ int ensure_workspace(cageo_surrogate* s, int hp, int wp) {
     for (int lv = 0; lv < 4; ++lv) {
         const size_t px = px0 >> (2 * lv);
         const int c = b << lv;
-        CAGEO_TRY(A(px * c));  // temporary (first conv of the level)
-        CAGEO_TRY(A(px * c));  // skip / encoder output
-        CAGEO_TRY(A(px * c));  // decoder output
-        CAGEO_TRY(A(px * c));  // pooled input of the next level (c channels at 1/4 the pixels)
+        CAGEO_TRY(A(px * c));      // temporary (first conv of the level)
+        CAGEO_TRY(A(px * c));      // skip / encoder output
+        CAGEO_TRY(A(px * c));      // decoder output
+        CAGEO_TRY(A(px / 4 * c));  // pooled input of the next level (c channels at 1/4 the pixels)
     }
     if (hipMalloc(reinterpret_cast<void**>(&s->prob_pad), px0 * sizeof(float)) != hipSuccess)
         return CAGEO_ERR_NOMEM;

@@ -341,18 +341,20 @@ class UNet:
 # ---- Monte Carlo labels, training, checked prediction -------------------------
 
 def mc_frequency(torch, dems, streams, rmses, ranges, realizations, seed, nodata=-9999.0,
-                 on_device=False):
+                 on_device=False, model=None):
     """Border frequency (count / realizations) of each DEM by the batched Monte
-    Carlo (Gaussian error field with the given RMSE and practical range),
-    as float32 numpy arrays (device tensors with on_device)."""
+    Carlo (error field with the given RMSE and practical range, Gaussian
+    filter unless `model`), as float32 numpy arrays (device tensors with
+    on_device)."""
     from . import Batch, BATCH_GRAPH, GAUSSIAN
+    model = GAUSSIAN if model is None else model
     shapes = [d.shape for d in dems]
     keep = []
     with Batch(shapes) as b:
         for t, (d, s, e, pr) in enumerate(zip(dems, streams, rmses, ranges)):
             dd, ss = torch.from_numpy(d).cuda(), torch.from_numpy(s).cuda()
             keep.append((dd, ss))
-            b.setup_tile(t, dd, ss, nodata, nodata, 2, GAUSSIAN, float(pr), 10.0, 0.0, float(e),
+            b.setup_tile(t, dd, ss, nodata, nodata, 2, model, float(pr), 10.0, 0.0, float(e),
                          seed + t)
         mcs = [torch.zeros(sh, dtype=torch.float32, device="cuda") for sh in shapes]
         b.run(mcs, realizations, 0, BATCH_GRAPH)
@@ -363,7 +365,8 @@ def mc_frequency(torch, dems, streams, rmses, ranges, realizations, seed, nodata
 
 
 def training_set(torch, n_dems=16, size=2048, crop=512, realizations=200, seed=1000,
-                 acc_threshold=2000, nodata=-9999.0):
+                 acc_threshold=2000, nodata=-9999.0, model=None, rmse_range=(0.5, 3.0),
+                 pr_range=(30.0, 200.0), log=None, group=16):
     """Inputs [N][crop][crop][4] and labels [N][crop][crop] (float32 numpy):
     synthetic DEMs of the evaluation generator, each with its own RMSE ~ U(0.5,
     3) m and practical range ~ U(30, 200) m, labelled by `realizations`
@@ -375,9 +378,14 @@ def training_set(torch, n_dems=16, size=2048, crop=512, realizations=200, seed=1
         d, s = synth_dem(size, size, seed + k, 10.0, acc_threshold, nodata)
         dems.append(d)
         streams.append(s)
-        rmses.append(float(rng.uniform(0.5, 3.0)))
-        ranges.append(float(rng.uniform(30.0, 200.0)))
-    labels = mc_frequency(torch, dems, streams, rmses, ranges, realizations, seed, nodata)
+        rmses.append(float(rng.uniform(*rmse_range)))
+        ranges.append(float(rng.uniform(*pr_range)))
+    labels = []
+    for g in range(0, n_dems, group):  # groups of DEMs, one progress line each
+        labels += mc_frequency(torch, dems[g:g + group], streams[g:g + group], rmses[g:g + group],
+                               ranges[g:g + group], realizations, seed + g, nodata, model=model)
+        if log:
+            log(f"labels: {min(g + group, n_dems)}/{n_dems} DEMs")
     xs, ys = [], []
     for d, s, e, pr, lab in zip(dems, streams, rmses, ranges, labels):
         x = make_inputs(d, s != nodata, e, pr, nodata, channels=IN_CHANNELS)
@@ -526,7 +534,7 @@ def device_inputs(torch, dem, stream, rmse, practical_range, nodata=-9999.0):
 
 def checked_prediction(torch, net, dem, stream, rmse, practical_range, seed,
                        check_realizations=4, fallback_realizations=1000, tolerance=0.01,
-                       nodata=-9999.0):
+                       nodata=-9999.0, model=None):
     """Surrogate map with its Monte Carlo check.  A `check_realizations` Monte
     Carlo map is drawn; if its L1 distance to the prediction exceeds the
     distance expected from sampling noise alone (were the prediction exact)
@@ -548,7 +556,7 @@ def checked_prediction(torch, net, dem, stream, rmse, practical_range, seed,
     T.cuda.synchronize()
     t1 = time.perf_counter()
     chk = mc_frequency(torch, [dem], [stream], [rmse], [practical_range], check_realizations,
-                       seed, nodata, on_device=True)[0]
+                       seed, nodata, on_device=True, model=model)[0]
     l1 = float((q - chk).abs().mean())
     noise = expected_l1_of_sample(q.double(), check_realizations)
     t2 = time.perf_counter()
@@ -558,7 +566,7 @@ def checked_prediction(torch, net, dem, stream, rmse, practical_range, seed,
     if ok:
         return q, info
     full = mc_frequency(torch, [dem], [stream], [rmse], [practical_range], fallback_realizations,
-                        seed + 1, nodata, on_device=True)[0]
+                        seed + 1, nodata, on_device=True, model=model)[0]
     T.cuda.synchronize()
     info["fallback_s"] = time.perf_counter() - t2
     return full, info

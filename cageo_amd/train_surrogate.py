@@ -50,6 +50,10 @@ def main(argv=None):
     ap.add_argument("--eval-range", type=float, default=60.0)
     ap.add_argument("--eval-realizations", type=int, default=1000)
     ap.add_argument("--check-realizations", type=int, default=4)
+    ap.add_argument("--filter", choices=["Gaussian", "Exponential"], default="Gaussian",
+                    help="error-field filter of the Monte Carlo labels and the judgement")
+    ap.add_argument("--pr-max", type=float, default=200.0,
+                    help="training practical ranges are drawn U(30, pr-max) m")
     a = ap.parse_args(argv)
 
     import torch
@@ -59,6 +63,8 @@ def main(argv=None):
     def log(msg):
         print(msg, flush=True)
 
+    model = ca.GAUSSIAN if a.filter == "Gaussian" else ca.EXPONENTIAL
+
     if a.load:
         net = sg.load_net(torch, None if a.load == "default" else a.load)
         hist, train_s = [], 0.0
@@ -67,7 +73,8 @@ def main(argv=None):
         if not a.out:
             ap.error("--out is required when training")
         t0 = time.perf_counter()
-        x4, y, meta = sg.training_set(torch, a.n_dems, a.size, a.crop, a.realizations, a.seed)
+        x4, y, meta = sg.training_set(torch, a.n_dems, a.size, a.crop, a.realizations, a.seed,
+                                      model=model, pr_range=(30.0, a.pr_max), log=log)
         log(f"training set: {x4.shape[0]} tiles of {a.crop}^2 from {a.n_dems} DEMs of "
             f"{a.size}^2, {a.realizations} realizations each ({time.perf_counter() - t0:.1f} s)")
         t1 = time.perf_counter()
@@ -83,15 +90,15 @@ def main(argv=None):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     p_mc = sg.mc_frequency(torch, [dem], [st], [a.eval_rmse], [a.eval_range],
-                           a.eval_realizations, a.eval_seed, on_device=True)[0]
+                           a.eval_realizations, a.eval_seed, on_device=True, model=model)[0]
     torch.cuda.synchronize()
     mc_s = time.perf_counter() - t2
     log(f"judge: {a.eval_realizations}-realization Monte Carlo in {mc_s:.2f} s")
     sg.checked_prediction(torch, net, dem[:512, :512], st[:512, :512], a.eval_rmse, a.eval_range,
-                          1, check_realizations=2, fallback_realizations=2)  # warm-up
+                          1, check_realizations=2, fallback_realizations=2, model=model)  # warm-up
     q, info = sg.checked_prediction(torch, net, dem, st, a.eval_rmse, a.eval_range,
                                     a.eval_seed + 7, check_realizations=a.check_realizations,
-                                    fallback_realizations=a.eval_realizations)
+                                    fallback_realizations=a.eval_realizations, model=model)
     surrogate_s = info["predict_s"] + info["check_s"] + info.get("fallback_s", 0.0)
     dev = (q - p_mc).abs().cpu().numpy()
     p_mc = p_mc.cpu().numpy()
@@ -101,7 +108,7 @@ def main(argv=None):
         "train_realizations": a.realizations, "train_steps": len(hist), "train_s": round(train_s, 2),
         "final_loss": float(np.mean(hist[-100:])) if hist else None,
         "eval_dem": f"{a.eval_size}x{a.eval_size} seed {a.eval_seed}",
-        "eval_rmse_m": a.eval_rmse, "eval_practical_range_m": a.eval_range,
+        "eval_rmse_m": a.eval_rmse, "eval_practical_range_m": a.eval_range, "filter": a.filter,
         "mc_realizations": a.eval_realizations, "mc_s": round(mc_s, 3),
         "l1": float(dev.mean()), "max_dev": float(dev.max()),
         "p999_dev": float(np.quantile(dev, 0.999)),
