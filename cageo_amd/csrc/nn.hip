@@ -312,7 +312,18 @@ __global__ void k_head(const uint16_t* __restrict__ act, const float* __restrict
                        int sigmoid) {
     GRID_STRIDE(p, P) {
         float s = b[0];
-        for (int c = 0; c < C; ++c) s += w[c] * bf2f(act[p * C + c]);
+        // C is a multiple of 32: the pixel's channels as 16-byte loads, summed
+        // in channel order
+        const uint4* a = reinterpret_cast<const uint4*>(act + p * C);
+        for (int c8 = 0; c8 < C / 8; ++c8) {
+            const uint4 v = a[c8];
+            const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                s += w[8 * c8 + 2 * k] * bf2f((uint16_t)(u[k] & 0xffffu));
+                s += w[8 * c8 + 2 * k + 1] * bf2f((uint16_t)(u[k] >> 16));
+            }
+        }
         logit[p] = sigmoid ? 1.0f / (1.0f + expf(-s)) : s;
     }
 }
@@ -501,6 +512,7 @@ int cageo_nn_bias_grad(void* stream, const uint16_t* g, float* db, long long p, 
 
 int cageo_nn_head(void* stream, const uint16_t* act, const float* w, const float* b, float* logit,
                   long long p, int c, int sigmoid) {
+    if (!act || !w || !b || !logit || p < 0 || c <= 0 || c % 32) return CAGEO_ERR_ARG;
     hipLaunchKernelGGL(k_head, dim3(grid_for(p)), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), act, w, b, logit, p, c, sigmoid);
     CAGEO_HIP_TRY(hipGetLastError());
