@@ -244,7 +244,7 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
             bits |= border_bits(lx, ly);
         }
     }
-    for (int off = 32; off > 0; off >>= 1) bits |= (uint32_t)__shfl_xor((int)bits, off);
+    bits = wave_or8(bits);
     if ((threadIdx.x & 63) == 0 && bits) atomicOr(&S.bits, bits);
     __syncthreads();
     if (threadIdx.x < 64 && S.bits)  // uniform within the first wave

@@ -219,11 +219,10 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0)
 __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& b, uint32_t bits,
                                    uint32_t open_left, uint32_t nfixed, uint32_t* cert_ctr,
                                    int* work_out, uint32_t* count_out, uint32_t epoch) {
-    for (int off = 32; off > 0; off >>= 1) {
-        bits |= (uint32_t)__shfl_xor((int)bits, off);
-        open_left |= (uint32_t)__shfl_xor((int)open_left, off);
-        nfixed += (uint32_t)__shfl_xor((int)nfixed, off);
-    }
+    // per-lane: bits < 256, nfixed <= STRIP = 16
+    bits = wave_or8(bits);
+    open_left = __any(open_left != 0u) ? 1u : 0u;
+    nfixed = wave_sum5(nfixed);
     if ((threadIdx.x & 63) == 0) {
         if (bits) atomicOr(&S.bits, bits);
         if (open_left) atomicOr(&S.any, 1u);
@@ -331,8 +330,8 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             scan_clamp_up(llo, lhi);
             scan_clamp_down(rlo, rhi);
             if (open & bit) {
-                const float L = fminf(fmaxf(S.rel[(ly + 1) * HALO_W + hl].x, llo), lhi);
-                const float R = fminf(fmaxf(S.rel[(ly + 1) * HALO_W + hr].x, rlo), rhi);
+                const float L = clamp3(S.rel[(ly + 1) * HALO_W + hl].x, llo, lhi);
+                const float R = clamp3(S.rel[(ly + 1) * HALO_W + hr].x, rlo, rhi);
                 const float nw = fminf(L, R);
                 if (nw < wcur) {
                     S.rel[li].x = nw;

@@ -26,11 +26,18 @@ __device__ constexpr int TDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 // Seidel down, then up); a row whose cells changed marks its neighbour rows
 // for another solve, and a tile is done after a sweep without changes.
 
+// min(max(v, lo), hi) for lo <= hi and no NaN: the median of the three (one
+// exact selection, v_med3_f32).
+__device__ inline float clamp3(float v, float lo, float hi) {
+    return __builtin_amdgcn_fmed3f(v, lo, hi);
+}
+
 // Clamp map v -> min(max(v, lo), hi) with lo <= hi.  Replaces (lo, hi) by
-// the composition "self after prev".  Compositions of clamps are clamps.
+// the composition "self after prev".  Compositions of clamps are clamps
+// (and keep lo <= hi).
 __device__ inline void clamp_after(float& lo, float& hi, float plo, float phi) {
-    const float nlo = fminf(fmaxf(plo, lo), hi);
-    const float nhi = fminf(fmaxf(phi, lo), hi);
+    const float nlo = clamp3(plo, lo, hi);
+    const float nhi = clamp3(phi, lo, hi);
     lo = nlo;
     hi = nhi;
 }
@@ -133,6 +140,22 @@ __device__ inline uint32_t border_bits(int lx, int ly) {
     if (w) b |= 1u << 6;
     if (n && w) b |= 1u << 7;
     return b;
+}
+
+// Wave-wide OR of 8-bit masks and sum of values below 32, by ballots (no
+// LDS permutes): the result in every lane.
+__device__ inline uint32_t wave_or8(uint32_t bits) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d)
+        if (__ballot((bits >> d) & 1u)) r |= 1u << d;
+    return r;
+}
+__device__ inline uint32_t wave_sum5(uint32_t v) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) r += (uint32_t)__popcll(__ballot((v >> b) & 1u)) << b;
+    return r;
 }
 
 // Lists the neighbour tiles of tile (tx, ty) named by `bits` for the next
