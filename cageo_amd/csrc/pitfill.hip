@@ -247,7 +247,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
-    uint32_t open = 0;
+    uint32_t open = 0, zlm = 0;  // OPEN cells; OPEN cells whose Z is live
 #pragma unroll
     for (int k0 = 0; k0 < STRIP; k0 += STRIP / 2) {  // two batches of independent loads
         uint8_t sv[STRIP / 2];
@@ -265,7 +265,10 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
 #pragma unroll
         for (int j = 0; j < STRIP / 2; ++j) {
-            if (sv[j] == PF_OPEN) open |= 1u << (k0 + j);
+            if (sv[j] == PF_OPEN) {
+                open |= 1u << (k0 + j);
+                if (pf_live(zv[j], mn)) zlm |= 1u << (k0 + j);
+            }
             S.zt[(strip * STRIP + k0 + j) * TILE + lx] = zv[j];
         }
     }
@@ -296,7 +299,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             const int li = (ly + 1) * HALO_W + lx + 1;
             const float z = S.zt[ly * TILE + lx];
             const float wcur = S.rel[li].x;
-            const bool zlive = op && pf_live(z, mn);
+            const bool zlive = op && ((zlm >> k) & 1u);
             const bool seed = op && min8y(S.rel, li) <= z;
             const float zl = lane_prev(z), zrt = lane_next(z);
             const int livel = lane_prev((int)zlive), liver = lane_next((int)zlive);
@@ -381,11 +384,17 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         if (gx < a.W && gy < a.H) sv[k] = a.st[(size_t)gy * a.W + gx];
     }
     __syncthreads();  // the relays of load_tile are in LDS
+    uint32_t lm = 0;  // OPEN cells whose (converged, fixed here) value is live
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         if (sv[k] == PF_OPEN) {
-            if (S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1].x < PF_INF) open |= 1u << k;
-            else open_inf |= 1u << k;
+            const float wk = S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1].x;
+            if (wk < PF_INF) {
+                open |= 1u << k;
+                if (pf_live(wk, mn)) lm |= 1u << k;
+            } else {
+                open_inf |= 1u << k;
+            }
         }
     }
 
@@ -406,7 +415,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             if (__ballot(op) == 0) continue;  // no OPEN cell: nothing in the row can change
             const int li = (ly + 1) * HALO_W + lx + 1;
             const float w = S.rel[li].x;
-            const bool live = op && pf_live(w, mn);
+            const bool live = op && ((lm >> k) & 1u);
             const bool seed = op && min8y(S.rel, li) <= w;
             // from the left neighbour x-1 (open, live) into x, and from x+1
             const float wl = lane_prev(w), wrt = lane_next(w);
