@@ -22,20 +22,20 @@ __global__ __launch_bounds__(256) void k_border(float* __restrict__ mc,
     if (x >= W || y >= H) return;
     const size_t i = (size_t)y * W + x;
     const int32_t c = lab[i];
-    float b = 0.0f;
-    if (c != CAGEO_FLOWTRACE_LAKE) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int nx = x + BDX[k], ny = y + BDY[k];
-            if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
-            const int32_t v = lab[(size_t)ny * W + nx];
-            if (c < v && v != CAGEO_FLOWTRACE_LAKE) {
-                b = 1.0f;
-                break;
-            }
-        }
-    }
     const float m = mc[i];
+    // all neighbour labels are loaded before any is tested (no dependent
+    // loads); an off-grid neighbour reads as LAKE, which never qualifies
+    int32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int nx = x + BDX[k], ny = y + BDY[k];
+        v[k] = (nx < 0 || ny < 0 || nx >= W || ny >= H) ? CAGEO_FLOWTRACE_LAKE
+                                                         : lab[(size_t)ny * W + nx];
+    }
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) any |= c < v[k] && v[k] != CAGEO_FLOWTRACE_LAKE;
+    const float b = (c != CAGEO_FLOWTRACE_LAKE && any) ? 1.0f : 0.0f;
     if (m != -1.0f) mc[i] = m + b;
 }
 

@@ -22,7 +22,7 @@
 #include <list>
 #include <vector>
 
-#include "common.hpp"
+#include "tiles.hpp"
 
 namespace cageo {
 
@@ -74,20 +74,35 @@ __device__ inline void burn_tile(int W, int H, const uint8_t* __restrict__ sf,
                                  int gx_out[4], int gy_out[4]) {
     __shared__ float zt[BL_W * BL_H];
     __shared__ uint8_t ft[BL_W * BL_H];
+    constexpr int ITER = (BL_W * BL_H + 255) / 256;  // loads per thread at the largest radius
     const int x0 = blockIdx.x * BT_X, y0 = blockIdx.y * BT_Y;
     const int lw = BT_X + 2 * R, lh = BT_Y + 2 * R;
-    for (int k = threadIdx.x; k < lw * lh; k += 256) {
-        const int ly = k / lw, lx = k - ly * lw;
-        const int gx = x0 - R + lx, gy = y0 - R + ly;
-        float z = 0.0f;
-        uint8_t f = 0;
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            const size_t gi = (size_t)gy * W + gx;
-            z = zr(gi);
-            f = sf[gi];
+    // all of a thread's loads are issued before its LDS stores
+    float zv[ITER];
+    uint8_t fv[ITER];
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int k = threadIdx.x + it * 256;
+        zv[it] = 0.0f;
+        fv[it] = 0;
+        if (k < lw * lh) {
+            const int ly = k / lw, lx = k - ly * lw;
+            const int gx = x0 - R + lx, gy = y0 - R + ly;
+            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+                const size_t gi = (size_t)gy * W + gx;
+                zv[it] = zr(gi);
+                fv[it] = sf[gi];
+            }
         }
-        zt[ly * BL_W + lx] = z;
-        ft[ly * BL_W + lx] = f;
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < lw * lh) {
+            const int ly = k / lw, lx = k - ly * lw;
+            zt[ly * BL_W + lx] = zv[it];
+            ft[ly * BL_W + lx] = fv[it];
+        }
     }
     __syncthreads();
     const int lx = threadIdx.x & 63;
@@ -146,31 +161,40 @@ __device__ inline float quantize_mm(float v, float nodata, uint32_t* bad) {
     return pf_to_elev(q, 0.0f);
 }
 
-// Block-wide min/max of ordered uints, then one atomic per block.
+// (min, max) of ordered uints: "self after prev" of the wave scan.
+__device__ inline void minmax_after(int& a, int& b, int pa, int pb) {
+    a = (int)min((uint32_t)a, (uint32_t)pa);
+    b = (int)max((uint32_t)b, (uint32_t)pb);
+}
+
+// Block-wide min/max of ordered uints (wave scans on DPP, then the four
+// waves' results through LDS), then at most one atomic per word per block:
+// a block whose value cannot improve the global word (which only moves
+// one way) skips it, so the atomics do not queue on the three words.
 __device__ inline void block_minmax(uint32_t mn, uint32_t mx, uint32_t bad, uint32_t* scal) {
-    __shared__ uint32_t s_mn, s_mx, s_bad;
-    if (threadIdx.x == 0) {
-        s_mn = 0xffffffffu;
-        s_mx = 0u;
-        s_bad = 0u;
-    }
-    __syncthreads();
-    // wave-level reduction first
-    for (int off = 32; off > 0; off >>= 1) {
-        mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
-        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-        bad |= (uint32_t)__shfl_xor((int)bad, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&s_mn, mn);
-        atomicMax(&s_mx, mx);
-        atomicOr(&s_bad, bad);
+    __shared__ uint32_t s_mn[4], s_mx[4], s_bad[4];
+    int a = (int)mn, b = (int)mx;
+    CAGEO_SCAN_UP(dpp_i, a, b, (int)0xffffffffu, 0, minmax_after);  // lane 63 holds the wave's
+    const int w = threadIdx.x >> 6;
+    const uint32_t wb = __any(bad != 0u) ? 1u : 0u;
+    if ((threadIdx.x & 63) == 63) {
+        s_mn[w] = (uint32_t)a;
+        s_mx[w] = (uint32_t)b;
+        s_bad[w] = wb;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicMin(&scal[SC_MIN], s_mn);
-        atomicMax(&scal[SC_MAX], s_mx);
-        if (s_bad) atomicOr(&scal[SC_BAD], s_bad);
+        uint32_t m0 = s_mn[0], m1 = s_mx[0], bb = s_bad[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            m0 = min(m0, s_mn[k]);
+            m1 = max(m1, s_mx[k]);
+            bb |= s_bad[k];
+        }
+        if (m0 < __hip_atomic_load(&scal[SC_MIN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&scal[SC_MIN], m0);
+        if (m1 > __hip_atomic_load(&scal[SC_MAX], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&scal[SC_MAX], m1);
+        if (bb) atomicOr(&scal[SC_BAD], bb);
     }
 }
 
