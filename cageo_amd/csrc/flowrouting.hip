@@ -157,15 +157,34 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     const FrRaster& a = b.R;
     const int tx = b.lb % a.ntx, ty = b.lb / a.ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
-    for (int k = threadIdx.x; k < HALO_W * HALO_W; k += TILE_THREADS) {
-        const int hy = k / HALO_W, hx = k - hy * HALO_W;
-        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-        int2 v = make_int2(D_OUT, 0);
-        if (gx >= 0 && gy >= 0 && gx < a.W && gy < a.H) {
-            const size_t gi = (size_t)gy * a.W + gx;
-            v = make_int2(a.dist[gi], __float_as_int(a.z[gi]));
+    // halo tile in two batches of independent loads (each thread's loads of
+    // a batch are in flight together)
+    {
+        constexpr int N = HALO_W * HALO_W;
+        constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
+        constexpr int CHUNK = (ITER + 1) / 2;
+#pragma unroll
+        for (int c0 = 0; c0 < ITER; c0 += CHUNK) {
+            int2 v[CHUNK];
+#pragma unroll
+            for (int j = 0; j < CHUNK; ++j) {
+                const int k = threadIdx.x + (c0 + j) * TILE_THREADS;
+                v[j] = make_int2(D_OUT, 0);
+                if (c0 + j < ITER && k < N) {
+                    const int hy = k / HALO_W, hx = k - hy * HALO_W;
+                    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+                    if (gx >= 0 && gy >= 0 && gx < a.W && gy < a.H) {
+                        const size_t gi = (size_t)gy * a.W + gx;
+                        v[j] = make_int2(a.dist[gi], __float_as_int(a.z[gi]));
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < CHUNK; ++j) {
+                const int k = threadIdx.x + (c0 + j) * TILE_THREADS;
+                if (c0 + j < ITER && k < N) S.dz[k] = v[j];
+            }
         }
-        S.dz[k] = v;
     }
     if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 1u;
     if (threadIdx.x == 0) S.bits = 0;
@@ -272,14 +291,17 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
         const int x = x0 + (threadIdx.x & 63);
         int has_flat = 0;
-        if (x < R.W)
-            for (int k = 0; k < STRIP; ++k) {
+        if (x < R.W) {
+#pragma unroll
+            for (int k = 0; k < STRIP; ++k) {  // no early exit: the rows' loads overlap
                 const int y = y0 + (threadIdx.x >> 6) * STRIP + k;
-                if (y >= R.H) break;
-                const int32_t d = d8_cell(R.z, R.fd, x, y, R.W, R.H);
-                R.dist[(size_t)y * R.W + x] = d;
-                has_flat |= d == D_INF;
+                if (y < R.H) {
+                    const int32_t d = d8_cell(R.z, R.fd, x, y, R.W, R.H);
+                    R.dist[(size_t)y * R.W + x] = d;
+                    has_flat |= d == D_INF;
+                }
             }
+        }
         if (__syncthreads_or(has_flat) && threadIdx.x == 0)
             a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
     }
