@@ -101,7 +101,6 @@ struct Context {
     int32_t* d_dist = nullptr;
     int32_t* d_lab = nullptr;
     uint64_t* d_ptr = nullptr;
-    uint64_t* d_ptr2 = nullptr;
 
     // timing
     bool timing = false;

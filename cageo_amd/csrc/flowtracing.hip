@@ -11,9 +11,14 @@
 //      part of its path (12 doubling rounds cover 4096 cells) and ends as a
 //      terminal label, "stuck" (its path cycles inside the tile, e.g. a cell
 //      without a direction), or an exit: the first cell outside the tile;
-//   2. over the grid: exits jump to the entries of their exit cells
-//      (ping-pong buffers) until no exit is left; a chain still moving after
+//   2. over the grid: exits jump to the entries of their exit cells, in
+//      place, until no exit is left; a chain still moving after
 //      ceil(log2 n)+1 rounds is a cycle through several tiles (stuck).
+//      In place is safe because every entry of a cell, old or rewritten,
+//      names a cell on that cell's own path with the step count to it, so a
+//      hop through either is correct, and a rewritten one is further along:
+//      each round advances every exit at least as far as a round over a
+//      copy would, and only the exits are written.
 // Entries are 64 bits: kind (2 bits) | steps (30 bits) | value (32 bits), so
 // the step count to the terminal is known and cells whose terminal is more
 // than 100 001 steps away (where the reference's result depends on its
@@ -129,14 +134,21 @@ __device__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
 __device__ inline uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Entries read and rewritten by other waves during the exit rounds move as
+// whole 64-bit words.
+__device__ inline uint64_t ld64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // One raster of a launch.
 struct TrRaster {
     const int8_t* fd;
     const uint8_t* sf;
     const int32_t* slabel;
-    uint64_t* ent0;  // entries, ping-pong pair
-    uint64_t* ent1;
+    uint64_t* ent;  // one 64-bit entry per cell
     int32_t* lab;
     int W, H, ntx;
     uint32_t* scal;
@@ -209,46 +221,42 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         const TrBlock b = tr_block<MULTI>(a, g);
         const TrRaster& R = b.R;
         if (b.lb == 0 && threadIdx.x == 0) R.scal[SC_LONGPATH] = 0;
-        trace_tile(L, R.fd, R.sf, R.slabel, R.ent0, R.W, R.H, R.ntx, b.lb);
+        trace_tile(L, R.fd, R.sf, R.slabel, R.ent, R.W, R.H, R.ntx, b.lb);
     }
     grid_sync(a.bar);
 
     const long long units = (long long)a.total * UNITS_PER_BLOCK;
-    int cur = 0;
     int rounds = 0;
     for (int r = 0; r < a.max_rounds; ++r) {
         rounds = r + 1;
         uint32_t pending = 0;
         for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
             const TrUnit t = tr_unit<MULTI>(a, u);
-            const uint64_t* src = cur ? t.R.ent1 : t.R.ent0;
-            uint64_t* dst = cur ? t.R.ent0 : t.R.ent1;
+            uint64_t* ent = t.R.ent;
             uint64_t e[UNIT_ROWS], f[UNIT_ROWS];
 #pragma unroll
-            for (int k = 0; k < UNIT_ROWS; ++k)
-                e[k] = k < t.rows ? src[(size_t)(t.y0 + k) * t.R.W + t.x] : pack(K_STUCK, 0, 0);
+            for (int k = 0; k < UNIT_ROWS; ++k)  // only this lane writes its own cells
+                e[k] = k < t.rows ? ent[(size_t)(t.y0 + k) * t.R.W + t.x] : pack(K_STUCK, 0, 0);
 #pragma unroll
-            for (int k = 0; k < UNIT_ROWS; ++k) f[k] = kind_of(e[k]) == K_EXIT ? src[value_of(e[k])] : 0;
+            for (int k = 0; k < UNIT_ROWS; ++k)
+                f[k] = kind_of(e[k]) == K_EXIT ? ld64(&ent[value_of(e[k])]) : 0;
 #pragma unroll
             for (int k = 0; k < UNIT_ROWS; ++k) {
-                if (k >= t.rows) continue;
-                if (kind_of(e[k]) == K_EXIT) {
-                    e[k] = hop(e[k], f[k]);
-                    pending |= kind_of(e[k]) == K_EXIT;
-                }
-                dst[(size_t)(t.y0 + k) * t.R.W + t.x] = e[k];
+                if (k >= t.rows || kind_of(e[k]) != K_EXIT) continue;
+                e[k] = hop(e[k], f[k]);
+                pending |= kind_of(e[k]) == K_EXIT;
+                st64(&ent[(size_t)(t.y0 + k) * t.R.W + t.x], e[k]);
             }
         }
         if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&a.ctrl[CTL_FLAG + r % 3], 1u);
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_FLAG + (r + 1) % 3] = 0;
         grid_sync(a.bar);
-        cur ^= 1;
         if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
     }
 
     for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
         const TrUnit t = tr_unit<MULTI>(a, u);
-        const uint64_t* fin = cur ? t.R.ent1 : t.R.ent0;
+        const uint64_t* fin = t.R.ent;
         uint64_t e[UNIT_ROWS];
 #pragma unroll
         for (int k = 0; k < UNIT_ROWS; ++k)
@@ -300,8 +308,7 @@ TrRaster tr_raster(Context* c, const int8_t* fd, const uint8_t* sflags, const in
     R.fd = fd;
     R.sf = sflags;
     R.slabel = slabel;
-    R.ent0 = c->d_ptr;
-    R.ent1 = c->d_ptr2;
+    R.ent = c->d_ptr;
     R.lab = lab;
     R.W = c->W;
     R.H = c->H;
@@ -312,7 +319,7 @@ TrRaster tr_raster(Context* c, const int8_t* fd, const uint8_t* sflags, const in
 
 }  // namespace
 
-// Needs two 64-bit rasters: c->d_ptr and c->d_ptr2.  The long-path count is
+// Needs one 64-bit raster, c->d_ptr.  The long-path count is
 // left in the scalar block.
 // Co-resident workgroups of the persistent kernel on this device.
 int flow_tracing_grid_cap(Context* c, int* cap) {
