@@ -54,6 +54,7 @@ struct Context {
     uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
     uint32_t* d_topen = nullptr; // per-tile flag: tile holds OPEN pit-fill cells
     uint32_t* d_mark_pf = nullptr;  // epoch stamps of the pit-fill kernel
+    uint32_t* d_rows = nullptr;     // per tile: dirty-row masks, 2 pass parities x 64 bits
     int* d_work3 = nullptr;         // 3 worklists for the persistent kernels
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
     uint32_t* d_bar = nullptr;      // grid barrier words (BAR_WORDS), see grid_sync
@@ -234,6 +235,7 @@ struct MultiState {
     uint32_t* d_mark_pf = nullptr;    // per-block epoch stamps (pit fill)
     uint32_t* d_mark_fr = nullptr;    // per-block epoch stamps (flats)
     uint32_t* d_topen = nullptr;      // per-block OPEN flag (pit fill)
+    uint32_t* d_rows = nullptr;       // per block: dirty-row masks, 2 pass parities x 64 bits
     uint32_t* d_ctrl = nullptr;       // CTL_WORDS
     uint32_t* d_cert = nullptr;       // 3 rotating counters per raster
     uint32_t* d_bar = nullptr;        // grid barrier words

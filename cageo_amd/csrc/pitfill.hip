@@ -50,7 +50,8 @@ constexpr float PF_INF = __builtin_huge_valf();
 struct PfShared {
     float2 rel[HALO_W * HALO_W];  // x: relay (OPEN or live FIXED), y: live FIXED only
     float zt[TILE * TILE];        // quantized elevation Z of the tile's OPEN cells
-    uint32_t rowdirty[TILE];      // rows due for another exact row solve
+    uint32_t rowdirty[TILE];      // strip edge rows marked by the neighbouring strips
+    uint32_t wcol[TILE_THREADS / 64], ecol[TILE_THREADS / 64];  // changed rows of columns 0, TILE-1
     uint32_t bits;
     uint32_t any;
 };
@@ -121,6 +122,7 @@ struct PfArgs {
     int* work;            // 4 lists of `total` entries: 3 rotating, 1 final
     uint32_t* mark;       // per-block epoch stamps
     uint32_t* topen;      // per-block: block holds OPEN cells
+    uint32_t* rows;       // per-block dirty-row masks: 2 pass parities x 64 bits
     uint32_t* ctrl;       // control block (CTL_*)
     uint32_t* cert;       // per raster: 3 rotating per-round fixed-cell counters
     uint32_t* bar;        // grid barrier words
@@ -208,32 +210,98 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0)
             }
         }
     }
-    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 1u;
+    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
     if (threadIdx.x == 0) {
         S.bits = 0;
         S.any = 0;
     }
 }
 
-// Ends a tile: OR of border bits, tile-open flag, neighbour activation.
+// Ends a tile: OR of border bits, tile-open flag, neighbour activation with
+// the rows each neighbour must solve again in pass `pass` + 1.  `chg` holds
+// the thread's changed cells that its neighbours see (bit k = row k of its
+// strip).
 __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& b, uint32_t bits,
-                                   uint32_t open_left, uint32_t nfixed, uint32_t* cert_ctr,
-                                   int* work_out, uint32_t* count_out, uint32_t epoch) {
+                                   uint32_t open_left, uint32_t nfixed, uint32_t chg,
+                                   uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
+                                   uint32_t epoch, int pass) {
     // per-lane: bits < 256, nfixed <= STRIP = 16
     bits = wave_or8(bits);
     open_left = __any(open_left != 0u) ? 1u : 0u;
     nfixed = wave_sum5(nfixed);
+    const uint32_t wc = __builtin_amdgcn_readlane(chg, 0);
+    const uint32_t ec = __builtin_amdgcn_readlane(chg, TILE - 1);
     if ((threadIdx.x & 63) == 0) {
         if (bits) atomicOr(&S.bits, bits);
         if (open_left) atomicOr(&S.any, 1u);
         if (nfixed) atomicAdd(cert_ctr, nfixed);
+        S.wcol[threadIdx.x >> 6] = wc;
+        S.ecol[threadIdx.x >> 6] = ec;
     }
     __syncthreads();
     if (threadIdx.x == 0) a.topen[b.gbase + b.lb] = S.any;
-    if (threadIdx.x < 64 && S.bits)  // uniform within the first wave
+    if (threadIdx.x < 64 && S.bits) {  // uniform within the first wave
+        unsigned long long w64 = 0, e64 = 0;
+#pragma unroll
+        for (int s = 0; s < TILE_THREADS / 64; ++s) {
+            w64 |= (unsigned long long)S.wcol[s] << (STRIP * s);
+            e64 |= (unsigned long long)S.ecol[s] << (STRIP * s);
+        }
         activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
-                       work_out, count_out, a.mark, epoch);
+                       work_out, count_out, a.mark, epoch, a.rows, (pass + 1) & 1, w64, e64);
+    }
     __syncthreads();
+}
+
+// The rows of global block g due for a solve in pass `pass` of a phase:
+// every row in the phase's first pass; later, the rows its neighbours'
+// changes unsettled in the previous pass (every other row is still at the
+// fixed point of its last solve, its halo unchanged).  Zero: nothing to do.
+__device__ inline unsigned long long tile_dirty_rows(const PfArgs& a, int g, int pass) {
+    if (pass == 0) return ~0ull;
+    const uint32_t* rw = a.rows + 4 * (size_t)g + 2 * (pass & 1);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(rw[0]);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(rw[1]);
+    return ((unsigned long long)hi << 32) | lo;
+}
+// Clears what tile_dirty_rows read, once every thread has read it.
+__device__ inline void clear_dirty_rows(const PfArgs& a, int g, int pass) {
+    if (pass > 0 && threadIdx.x == 0) {
+        uint32_t* rw = a.rows + 4 * (size_t)g + 2 * (pass & 1);
+        rw[0] = 0u;
+        rw[1] = 0u;
+    }
+}
+
+// Rows due for a row solve.  Each wave keeps the rows of its own strip in a
+// wave-uniform mask (`rd`, bit k = row k of the strip); the first and last
+// rows of a strip are also marked by the neighbouring strips' waves, through
+// S.rowdirty.  A wave clears such a flag before it reads the row's
+// neighbours, and a marking wave writes its row before the flag (LDS
+// operations of a wave complete in order), so a change is either seen by
+// the solve or flagged for the next sweep.
+__device__ inline bool row_due(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
+    const uint32_t bit = 1u << k;
+    bool due = (rd & bit) != 0u;
+    rd &= ~bit;
+    if (k == 0 || k == STRIP - 1) {
+        if (__builtin_amdgcn_readfirstlane(rowdirty[ly]) != 0u) {
+            rowdirty[ly] = 0u;
+            due = true;
+        }
+    }
+    return due;
+}
+// Marks the rows next to row k of the strip (row ly of the tile).
+__device__ inline void row_mark(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
+    if (k > 0)
+        rd |= 1u << (k - 1);
+    else if (ly > 0 && (threadIdx.x & 63) == 0)
+        rowdirty[ly - 1] = 1u;
+    if (k < STRIP - 1)
+        rd |= 1u << (k + 1);
+    else if (ly < TILE - 1 && (threadIdx.x & 63) == 0)
+        rowdirty[ly + 1] = 1u;
 }
 
 // Fill relaxation of one tile (with immediate fixing at Z).  Cell state
@@ -241,8 +309,11 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
 // strip's cells only, so the row loop can run in either direction without
 // per-thread arrays.
 __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
-                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch) {
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
+                          int pass) {
     const PfRaster& a = b.R;
+    const unsigned long long dirty = tile_dirty_rows(args, b.gbase + b.lb, pass);
+    if (dirty == 0) return;  // uniform
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
@@ -273,6 +344,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
     }
     __syncthreads();
+    clear_dirty_rows(args, b.gbase + b.lb, pass);
 
     // Row step (row ly, all 64 columns at once):
     //  1. Z-rule: an OPEN cell with a live FIXED neighbour no higher than Z
@@ -286,21 +358,31 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     //     the map chains from the left and right halo cells.
     const int hl = 0, hr = HALO_W - 1;  // halo columns
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
+    uint32_t rd = (uint32_t)(dirty >> (strip * STRIP)) & ((1u << STRIP) - 1u);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
         for (int kk = 0; kk < STRIP; ++kk) {
             const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
-            if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
-            S.rowdirty[ly] = 0u;
+            if (!row_due(S.rowdirty, rd, k, ly)) continue;
             const uint32_t bit = 1u << k;
             const bool op = (open & bit) != 0;
             if (__ballot(op) == 0) continue;  // no OPEN cell: nothing in the row can change
             const int li = (ly + 1) * HALO_W + lx + 1;
+            // The step's reads at once: the cell, its neighbours in the rows
+            // above and below (relay and live value), the live values of its
+            // row neighbours and Z.
+            const float2 c = S.rel[li];
+            const float2 a0 = S.rel[li - HALO_W - 1], a1 = S.rel[li - HALO_W], a2 = S.rel[li - HALO_W + 1];
+            const float2 b0 = S.rel[li + HALO_W - 1], b1 = S.rel[li + HALO_W], b2 = S.rel[li + HALO_W + 1];
+            const float yl = S.rel[li - 1].y, yr = S.rel[li + 1].y;
             const float z = S.zt[ly * TILE + lx];
-            const float wcur = S.rel[li].x;
+            const float wcur = c.x;
+            const float m8y = fminf(fminf(fminf(a0.y, a1.y), fminf(a2.y, yl)),
+                                    fminf(fminf(yr, b0.y), fminf(b1.y, b2.y)));
+            const float m6x = fminf(fminf(fminf(a0.x, a1.x), fminf(a2.x, b0.x)), fminf(b1.x, b2.x));
             const bool zlive = op && ((zlm >> k) & 1u);
-            const bool seed = op && min8y(S.rel, li) <= z;
+            const bool seed = op && m8y <= z;
             const float zl = lane_prev(z), zrt = lane_next(z);
             const int livel = lane_prev((int)zlive), liver = lane_next((int)zlive);
             const unsigned long long sd = __ballot(seed);
@@ -325,9 +407,9 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
                 ch = true;
             } else if (op) {
                 lo = z;
-                hi = fmaxf(z, fminf(wcur, min6x(S.rel, li)));
+                hi = fmaxf(z, fminf(wcur, m6x));
             } else {
-                lo = hi = S.rel[li].y;  // live FIXED: its value; otherwise no relay
+                lo = hi = c.y;  // live FIXED: its value; otherwise no relay
             }
             float llo = lo, lhi = hi, rlo = lo, rhi = hi;
             scan_clamp_up(llo, lhi);
@@ -344,7 +426,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             }
             if (__ballot(ch)) {
                 changed = 1;
-                mark_rows(S.rowdirty, ly, false);
+                row_mark(S.rowdirty, rd, k, ly);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -364,13 +446,17 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
         if ((lowered | fixed) & (1u << k)) bits |= border_bits(lx, ly);
     }
-    finish_tile(S, args, b, bits, open, nfixed, cert_ctr, work_out, count_out, epoch);
+    finish_tile(S, args, b, bits, open, nfixed, lowered | fixed, cert_ctr, work_out, count_out,
+                epoch, pass);
 }
 
 // Certification of one tile after the fill has converged.
 __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
-                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch) {
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
+                          int pass) {
     const PfRaster& a = b.R;
+    const unsigned long long dirty = tile_dirty_rows(args, b.gbase + b.lb, pass);
+    if (dirty == 0) return;  // uniform
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     load_tile(S, a, x0, y0);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
@@ -384,6 +470,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         if (gx < a.W && gy < a.H) sv[k] = a.st[(size_t)gy * a.W + gx];
     }
     __syncthreads();  // the relays of load_tile are in LDS
+    clear_dirty_rows(args, b.gbase + b.lb, pass);
     uint32_t lm = 0;  // OPEN cells whose (converged, fixed here) value is live
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
@@ -403,13 +490,13 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     // to an OPEN neighbour whose value is not lower (segmented propagation
     // over the lane masks).
     uint32_t fixed = 0, fixed_live = 0;
+    uint32_t rd = (uint32_t)(dirty >> (strip * STRIP)) & ((1u << STRIP) - 1u);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
         for (int kk = 0; kk < STRIP; ++kk) {
             const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
-            if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
-            S.rowdirty[ly] = 0u;
+            if (!row_due(S.rowdirty, rd, k, ly)) continue;
             const uint32_t bit = 1u << k;
             const bool op = (open & bit) != 0;
             if (__ballot(op) == 0) continue;  // no OPEN cell: nothing in the row can change
@@ -437,7 +524,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             }
             if (__ballot(ch)) {
                 changed = 1;
-                mark_rows(S.rowdirty, ly, false);
+                row_mark(S.rowdirty, rd, k, ly);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -454,7 +541,8 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             if (fixed_live & (1u << k)) bits |= border_bits(lx, ly);
         }
     }
-    finish_tile(S, args, b, bits, open | open_inf, nfixed, cert_ctr, work_out, count_out, epoch);
+    finish_tile(S, args, b, bits, open | open_inf, nfixed, fixed_live, cert_ctr, work_out,
+                count_out, epoch, pass);
 }
 
 // Runs passes of `kind` (0 fill, 1 certify) from worklist `li` until a pass
@@ -480,9 +568,9 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int slot, uint32
             if (!b.ok) continue;  // uniform; such blocks never open
             uint32_t* cert_ctr = &a.cert[3 * b.r + slot];
             if (KIND == 0)
-                fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
+                fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass);
             else
-                cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch);
+                cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass);
         }
         grid_sync(a.bar);
         li = cout;
@@ -609,8 +697,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
     }
     // list 0: every block (those of rasters outside the domain are skipped)
-    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x)
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x) {
         a.work[g] = g;
+        reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
+    }
     for (int k = threadIdx.x; k < 3 * a.nr; k += blockDim.x)
         if (blockIdx.x == 0) a.cert[k] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -715,6 +805,7 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.work = c->d_work3;
     a.mark = c->d_mark_pf;
     a.topen = c->d_topen;
+    a.rows = c->d_rows;
     a.ctrl = c->d_ctrl;
     a.cert = c->d_ctrl + CTL_CERT;
     a.bar = c->d_bar;
@@ -751,6 +842,7 @@ int pit_fill_multi(MultiState* m, hipStream_t s) {
     a.work = m->d_work;
     a.mark = m->d_mark_pf;
     a.topen = m->d_topen;
+    a.rows = m->d_rows;
     a.ctrl = m->d_ctrl;
     a.cert = m->d_cert;
     a.bar = m->d_bar;

@@ -158,13 +158,32 @@ __device__ inline uint32_t wave_sum5(uint32_t v) {
     return r;
 }
 
+// Rows of the neighbour in direction d (TDX/TDY order) that a change of
+// this tile's border cells can unsettle: a halo row touches one row of the
+// neighbour (its last row for the northern three, its first for the
+// southern three); a halo column cell at row y touches rows y-1..y+1, with
+// w64 / e64 the changed rows of columns 0 / TILE-1 (bit = row).
+__device__ inline unsigned long long neighbour_rows(int d, unsigned long long w64,
+                                                   unsigned long long e64) {
+    if (d == 0 || d == 1 || d == 7) return 1ull << (TILE - 1);
+    if (d == 3 || d == 4 || d == 5) return 1ull;
+    const unsigned long long m = d == 2 ? e64 : w64;
+    return m | (m << 1) | (m >> 1);
+}
+
 // Lists the neighbour tiles of tile (tx, ty) named by `bits` for the next
 // pass, each at most once per epoch.  Called by all 64 lanes of one wave:
 // lanes 0..7 take one direction each (their mark exchanges are in flight
 // together), and one atomic reserves the list slots of the whole wave.
+// With `rows` set, each neighbour's dirty-row mask for
+// the next pass (two words at rows + 4 t + 2 rpar) also gets the rows the
+// change unsettles; those words are read and cleared by the neighbour in
+// that pass, and the other parity is the one being read in this pass.
 __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty, int gbase,
                                       int* __restrict__ work_out, uint32_t* count_out,
-                                      uint32_t* __restrict__ mark, uint32_t epoch) {
+                                      uint32_t* __restrict__ mark, uint32_t epoch,
+                                      uint32_t* rows = nullptr, int rpar = 0,
+                                      unsigned long long w64 = 0, unsigned long long e64 = 0) {
     const int lane = threadIdx.x & 63;
     bool push = false;
     int t = 0;
@@ -172,6 +191,12 @@ __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, in
         const int nx = tx + TDX[lane], ny = ty + TDY[lane];
         if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) {
             t = gbase + ny * ntx + nx;
+            if (rows) {
+                const unsigned long long m = neighbour_rows(lane, w64, e64);
+                uint32_t* p = rows + 4 * (size_t)t + 2 * rpar;
+                if ((uint32_t)m) atomicOr(p, (uint32_t)m);
+                if ((uint32_t)(m >> 32)) atomicOr(p + 1, (uint32_t)(m >> 32));
+            }
             push = atomicExch(&mark[t], epoch) != epoch;
         }
     }
