@@ -34,7 +34,8 @@ __device__ constexpr int FDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 
 struct FrShared {
     int2 dz[HALO_W * HALO_W];  // x: distance, y: bits of z
-    uint32_t rowdirty[TILE];   // rows due for another exact row solve
+    uint32_t rowdirty[TILE];   // strip edge rows marked by the neighbouring strips
+    uint32_t wcol[TILE_THREADS / 64], ecol[TILE_THREADS / 64];  // changed rows of columns 0, TILE-1
     uint32_t bits;
     uint32_t any;
 };
@@ -74,6 +75,7 @@ struct FrArgs {
     int nr, total;
     int* work;      // 3 lists of `total` entries
     uint32_t* mark; // per-block epoch stamps
+    uint32_t* rows; // per-block dirty-row masks: 2 pass parities x 64 bits
     uint32_t* ctrl;
     uint32_t* bar;  // grid barrier words
     int max_passes;
@@ -153,8 +155,10 @@ __device__ inline int32_t d8_cell(const float* __restrict__ z, int8_t* __restric
 }
 
 __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int* work_out,
-                         uint32_t* count_out, uint32_t epoch) {
+                         uint32_t* count_out, uint32_t epoch, int pass) {
     const FrRaster& a = b.R;
+    const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
+    if (dirty == 0) return;  // uniform
     const int tx = b.lb % a.ntx, ty = b.lb / a.ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
     // halo tile in two batches of independent loads (each thread's loads of
@@ -186,9 +190,10 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
             }
         }
     }
-    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 1u;
+    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
     if (threadIdx.x == 0) S.bits = 0;
     __syncthreads();
+    clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
 
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     uint32_t flat = 0;
@@ -204,18 +209,22 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     // cells with a direction relay their 0, stream and outside cells relay
     // nothing.  Scans from both sides give the row's exact fixed point.
     uint32_t changed_cells = 0;
+    uint32_t rd = (uint32_t)(dirty >> (strip * STRIP)) & ((1u << STRIP) - 1u);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
 #pragma unroll
         for (int kk = 0; kk < STRIP; ++kk) {
             const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
-            if (__builtin_amdgcn_readfirstlane(S.rowdirty[ly]) == 0u) continue;
-            S.rowdirty[ly] = 0u;
+            if (!row_due(S.rowdirty, rd, k, ly)) continue;
             const bool isflat = (flat >> k) & 1u;
             if (__ballot(isflat) == 0) continue;  // no flat cell: nothing in the row can change
             const int li = (ly + 1) * HALO_W + lx + 1;
+            // the step's reads at once: the cell and its eight neighbours
             const int2 c = S.dz[li];
+            int2 nb[8];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) nb[d] = S.dz[li + FDY[d] * HALO_W + FDX[d]];
             const float zc = __int_as_float(c.y);
             const float zl = lane_prev(zc), zrt = lane_next(zc);
             int a, kl = D_INF, kr = D_INF;
@@ -227,8 +236,7 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
                     const bool in_row = FDY[d] == 0;
                     if (in_row && !((lx == 0 && FDX[d] < 0) || (lx == TILE - 1 && FDX[d] > 0)))
                         continue;
-                    const int2 nv = S.dz[li + FDY[d] * HALO_W + FDX[d]];
-                    if (nv.x >= 0 && __int_as_float(nv.y) <= zc) a = min(a, nv.x + 1);
+                    if (nb[d].x >= 0 && __int_as_float(nb[d].y) <= zc) a = min(a, nb[d].x + 1);
                 }
                 if (lx > 0 && zl <= zc) kl = 1;
                 if (lx < TILE - 1 && zrt <= zc) kr = 1;
@@ -249,7 +257,7 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
             }
             if (__ballot(ch)) {
                 changed = 1;
-                mark_rows(S.rowdirty, ly, false);
+                row_mark(S.rowdirty, rd, k, ly);
             }
         }
         if (!__syncthreads_or(changed)) break;
@@ -264,11 +272,15 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
         }
     }
     bits = wave_or8(bits);
+    edge_columns_put(S.wcol, S.ecol, changed_cells);
     if ((threadIdx.x & 63) == 0 && bits) atomicOr(&S.bits, bits);
     __syncthreads();
-    if (threadIdx.x < 64 && S.bits)  // uniform within the first wave
+    if (threadIdx.x < 64 && S.bits) {  // uniform within the first wave
+        unsigned long long w64, e64;
+        edge_columns_get(S.wcol, S.ecol, w64, e64);
         activate_tiles(S.bits, tx, ty, a.ntx, a.nty, b.gbase, work_out, count_out, args.mark,
-                       epoch);
+                       epoch, args.rows, (pass + 1) & 1, w64, e64);
+    }
     __syncthreads();
 }
 
@@ -302,6 +314,7 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
                 }
             }
         }
+        if (threadIdx.x == 0) reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
         if (__syncthreads_or(has_flat) && threadIdx.x == 0)
             a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
     }
@@ -322,7 +335,7 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         const int* win = a.work + (size_t)cin * a.total;
         int* wout = a.work + (size_t)cout * a.total;
         for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
-            bfs_tile(S, a, fr_block<MULTI>(a, win[i]), wout, &a.ctrl[CTL_COUNT + cout], epoch);
+            bfs_tile(S, a, fr_block<MULTI>(a, win[i]), wout, &a.ctrl[CTL_COUNT + cout], epoch, pass);
         grid_sync(a.bar);
         li = cout;
     }
@@ -407,6 +420,7 @@ int flow_routing(Context* c, const float* z, int8_t* fd) {
     a.total = c->ntiles;
     a.work = c->d_work3;
     a.mark = c->d_mark;
+    a.rows = c->d_rows;
     a.ctrl = c->d_ctrl;
     a.bar = c->d_bar;
     a.max_passes = 1 << 20;
@@ -444,6 +458,7 @@ int flow_routing_multi(MultiState* m, hipStream_t s) {
     a.total = m->total;
     a.work = m->d_work;
     a.mark = m->d_mark_fr;
+    a.rows = m->d_rows;
     a.ctrl = m->d_ctrl;
     a.bar = m->d_bar;
     a.max_passes = 1 << 20;

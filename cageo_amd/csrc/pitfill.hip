@@ -229,79 +229,21 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
     bits = wave_or8(bits);
     open_left = __any(open_left != 0u) ? 1u : 0u;
     nfixed = wave_sum5(nfixed);
-    const uint32_t wc = __builtin_amdgcn_readlane(chg, 0);
-    const uint32_t ec = __builtin_amdgcn_readlane(chg, TILE - 1);
+    edge_columns_put(S.wcol, S.ecol, chg);
     if ((threadIdx.x & 63) == 0) {
         if (bits) atomicOr(&S.bits, bits);
         if (open_left) atomicOr(&S.any, 1u);
         if (nfixed) atomicAdd(cert_ctr, nfixed);
-        S.wcol[threadIdx.x >> 6] = wc;
-        S.ecol[threadIdx.x >> 6] = ec;
     }
     __syncthreads();
     if (threadIdx.x == 0) a.topen[b.gbase + b.lb] = S.any;
     if (threadIdx.x < 64 && S.bits) {  // uniform within the first wave
-        unsigned long long w64 = 0, e64 = 0;
-#pragma unroll
-        for (int s = 0; s < TILE_THREADS / 64; ++s) {
-            w64 |= (unsigned long long)S.wcol[s] << (STRIP * s);
-            e64 |= (unsigned long long)S.ecol[s] << (STRIP * s);
-        }
+        unsigned long long w64, e64;
+        edge_columns_get(S.wcol, S.ecol, w64, e64);
         activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
                        work_out, count_out, a.mark, epoch, a.rows, (pass + 1) & 1, w64, e64);
     }
     __syncthreads();
-}
-
-// The rows of global block g due for a solve in pass `pass` of a phase:
-// every row in the phase's first pass; later, the rows its neighbours'
-// changes unsettled in the previous pass (every other row is still at the
-// fixed point of its last solve, its halo unchanged).  Zero: nothing to do.
-__device__ inline unsigned long long tile_dirty_rows(const PfArgs& a, int g, int pass) {
-    if (pass == 0) return ~0ull;
-    const uint32_t* rw = a.rows + 4 * (size_t)g + 2 * (pass & 1);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(rw[0]);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(rw[1]);
-    return ((unsigned long long)hi << 32) | lo;
-}
-// Clears what tile_dirty_rows read, once every thread has read it.
-__device__ inline void clear_dirty_rows(const PfArgs& a, int g, int pass) {
-    if (pass > 0 && threadIdx.x == 0) {
-        uint32_t* rw = a.rows + 4 * (size_t)g + 2 * (pass & 1);
-        rw[0] = 0u;
-        rw[1] = 0u;
-    }
-}
-
-// Rows due for a row solve.  Each wave keeps the rows of its own strip in a
-// wave-uniform mask (`rd`, bit k = row k of the strip); the first and last
-// rows of a strip are also marked by the neighbouring strips' waves, through
-// S.rowdirty.  A wave clears such a flag before it reads the row's
-// neighbours, and a marking wave writes its row before the flag (LDS
-// operations of a wave complete in order), so a change is either seen by
-// the solve or flagged for the next sweep.
-__device__ inline bool row_due(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
-    const uint32_t bit = 1u << k;
-    bool due = (rd & bit) != 0u;
-    rd &= ~bit;
-    if (k == 0 || k == STRIP - 1) {
-        if (__builtin_amdgcn_readfirstlane(rowdirty[ly]) != 0u) {
-            rowdirty[ly] = 0u;
-            due = true;
-        }
-    }
-    return due;
-}
-// Marks the rows next to row k of the strip (row ly of the tile).
-__device__ inline void row_mark(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
-    if (k > 0)
-        rd |= 1u << (k - 1);
-    else if (ly > 0 && (threadIdx.x & 63) == 0)
-        rowdirty[ly - 1] = 1u;
-    if (k < STRIP - 1)
-        rd |= 1u << (k + 1);
-    else if (ly < TILE - 1 && (threadIdx.x & 63) == 0)
-        rowdirty[ly + 1] = 1u;
 }
 
 // Fill relaxation of one tile (with immediate fixing at Z).  Cell state
@@ -312,7 +254,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
                           uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
                           int pass) {
     const PfRaster& a = b.R;
-    const unsigned long long dirty = tile_dirty_rows(args, b.gbase + b.lb, pass);
+    const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
     if (dirty == 0) return;  // uniform
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     load_tile(S, a, x0, y0);
@@ -344,7 +286,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
     }
     __syncthreads();
-    clear_dirty_rows(args, b.gbase + b.lb, pass);
+    clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
 
     // Row step (row ly, all 64 columns at once):
     //  1. Z-rule: an OPEN cell with a live FIXED neighbour no higher than Z
@@ -455,7 +397,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
                           uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
                           int pass) {
     const PfRaster& a = b.R;
-    const unsigned long long dirty = tile_dirty_rows(args, b.gbase + b.lb, pass);
+    const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
     if (dirty == 0) return;  // uniform
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     load_tile(S, a, x0, y0);
@@ -470,7 +412,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         if (gx < a.W && gy < a.H) sv[k] = a.st[(size_t)gy * a.W + gx];
     }
     __syncthreads();  // the relays of load_tile are in LDS
-    clear_dirty_rows(args, b.gbase + b.lb, pass);
+    clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
     uint32_t lm = 0;  // OPEN cells whose (converged, fixed here) value is live
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {

@@ -127,6 +127,80 @@ __device__ inline void mark_rows(uint32_t* rd, int ly, bool self) {
     }
 }
 
+// Rows due for a row solve.  Each wave keeps the rows of its own strip in a
+// wave-uniform mask (`rd`, bit k = row k of the strip); the first and last
+// rows of a strip are also marked by the neighbouring strips' waves, through
+// S.rowdirty.  A wave clears such a flag before it reads the row's
+// neighbours, and a marking wave writes its row before the flag (LDS
+// operations of a wave complete in order), so a change is either seen by
+// the solve or flagged for the next sweep.
+__device__ inline bool row_due(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
+    const uint32_t bit = 1u << k;
+    bool due = (rd & bit) != 0u;
+    rd &= ~bit;
+    if (k == 0 || k == STRIP - 1) {
+        if (__builtin_amdgcn_readfirstlane(rowdirty[ly]) != 0u) {
+            rowdirty[ly] = 0u;
+            due = true;
+        }
+    }
+    return due;
+}
+// Marks the rows next to row k of the strip (row ly of the tile).
+__device__ inline void row_mark(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
+    if (k > 0)
+        rd |= 1u << (k - 1);
+    else if (ly > 0 && (threadIdx.x & 63) == 0)
+        rowdirty[ly - 1] = 1u;
+    if (k < STRIP - 1)
+        rd |= 1u << (k + 1);
+    else if (ly < TILE - 1 && (threadIdx.x & 63) == 0)
+        rowdirty[ly + 1] = 1u;
+}
+
+// Dirty-row masks of tiles solved in passes (two 32-bit words per pass
+// parity, four per tile, at `rows`).  The rows of global tile g due for a
+// solve in pass `pass` of a phase: every row in the phase's first pass;
+// later, the rows its neighbours' changes unsettled in the previous pass
+// (every other row is still at the fixed point of its last solve, its halo
+// unchanged).  Zero: nothing to do.  Wave-uniform.
+__device__ inline unsigned long long tile_dirty_rows(const uint32_t* rows, int g, int pass) {
+    if (pass == 0) return ~0ull;
+    const uint32_t* rw = rows + 4 * (size_t)g + 2 * (pass & 1);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(rw[0]);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(rw[1]);
+    return ((unsigned long long)hi << 32) | lo;
+}
+// Clears what tile_dirty_rows read; called once every thread has read it.
+__device__ inline void clear_dirty_rows(uint32_t* rows, int g, int pass) {
+    if (pass > 0 && threadIdx.x == 0) {
+        uint32_t* rw = rows + 4 * (size_t)g + 2 * (pass & 1);
+        rw[0] = 0u;
+        rw[1] = 0u;
+    }
+}
+// Changed rows of the tile's columns 0 and TILE-1, from each thread's mask
+// `chg` of changed cells (bit k = row k of its strip): put by every thread
+// before a __syncthreads, got as 64-bit row masks after it.
+__device__ inline void edge_columns_put(uint32_t* wcol, uint32_t* ecol, uint32_t chg) {
+    const uint32_t wc = __builtin_amdgcn_readlane(chg, 0);
+    const uint32_t ec = __builtin_amdgcn_readlane(chg, TILE - 1);
+    if ((threadIdx.x & 63) == 0) {
+        wcol[threadIdx.x >> 6] = wc;
+        ecol[threadIdx.x >> 6] = ec;
+    }
+}
+__device__ inline void edge_columns_get(const uint32_t* wcol, const uint32_t* ecol,
+                                        unsigned long long& w64, unsigned long long& e64) {
+    w64 = 0;
+    e64 = 0;
+#pragma unroll
+    for (int s = 0; s < TILE_THREADS / 64; ++s) {
+        w64 |= (unsigned long long)wcol[s] << (STRIP * s);
+        e64 |= (unsigned long long)ecol[s] << (STRIP * s);
+    }
+}
+
 // Bits of the neighbour tiles whose halo contains local cell (lx, ly).
 __device__ inline uint32_t border_bits(int lx, int ly) {
     uint32_t b = 0;
