@@ -209,12 +209,13 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     // cells with a direction relay their 0, stream and outside cells relay
     // nothing.  Scans from both sides give the row's exact fixed point.
     uint32_t changed_cells = 0;
-    uint32_t rd = (uint32_t)(dirty >> (strip * STRIP)) & ((1u << STRIP) - 1u);
+    uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
+        const bool upward = sweep_upward(S.rowdirty, rd, strip, sweep);
 #pragma unroll
         for (int kk = 0; kk < STRIP; ++kk) {
-            const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
+            const int k = upward ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
             if (!row_due(S.rowdirty, rd, k, ly)) continue;
             const bool isflat = (flat >> k) & 1u;

@@ -300,11 +300,12 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     //     the map chains from the left and right halo cells.
     const int hl = 0, hr = HALO_W - 1;  // halo columns
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
-    uint32_t rd = (uint32_t)(dirty >> (strip * STRIP)) & ((1u << STRIP) - 1u);
+    uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
+        const bool upward = sweep_upward(S.rowdirty, rd, strip, sweep);
         for (int kk = 0; kk < STRIP; ++kk) {
-            const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
+            const int k = upward ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
             if (!row_due(S.rowdirty, rd, k, ly)) continue;
             const uint32_t bit = 1u << k;
@@ -432,11 +433,12 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     // to an OPEN neighbour whose value is not lower (segmented propagation
     // over the lane masks).
     uint32_t fixed = 0, fixed_live = 0;
-    uint32_t rd = (uint32_t)(dirty >> (strip * STRIP)) & ((1u << STRIP) - 1u);
+    uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
+        const bool upward = sweep_upward(S.rowdirty, rd, strip, sweep);
         for (int kk = 0; kk < STRIP; ++kk) {
-            const int k = (sweep & 1) ? (STRIP - 1 - kk) : kk;
+            const int k = upward ? (STRIP - 1 - kk) : kk;
             const int ly = strip * STRIP + k;
             if (!row_due(S.rowdirty, rd, k, ly)) continue;
             const uint32_t bit = 1u << k;

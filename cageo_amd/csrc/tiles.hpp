@@ -127,17 +127,35 @@ __device__ inline void mark_rows(uint32_t* rd, int ly, bool self) {
     }
 }
 
-// Rows due for a row solve.  Each wave keeps the rows of its own strip in a
-// wave-uniform mask (`rd`, bit k = row k of the strip); the first and last
-// rows of a strip are also marked by the neighbouring strips' waves, through
-// S.rowdirty.  A wave clears such a flag before it reads the row's
-// neighbours, and a marking wave writes its row before the flag (LDS
-// operations of a wave complete in order), so a change is either seen by
-// the solve or flagged for the next sweep.
+// Rows due for a row solve.  Each wave keeps the rows of its own strip in
+// one wave-uniform word (bit k = row k of the strip): the low half holds
+// rows reached by a change below them (an upward sweep carries it on), the
+// high half rows reached by a change above them; a row due regardless of
+// direction is in both.  The first and last rows of a strip are also
+// marked by the neighbouring strips' waves, through S.rowdirty.  A wave
+// clears such a flag before it reads the row's neighbours, and a marking
+// wave writes its row before the flag (LDS operations of a wave complete in
+// order), so a change is either seen by the solve or flagged for the next
+// sweep.
+static_assert(STRIP == 16, "two strip masks share a word");
+__device__ inline uint32_t strip_rows(unsigned long long dirty, int strip) {
+    const uint32_t m = (uint32_t)(dirty >> (strip * STRIP)) & 0xffffu;
+    return m | (m << 16);
+}
+// Direction of the next sweep (true: upward): the one its pending rows ask
+// for, a strip edge flagged by the strip above asking for a downward sweep
+// and one flagged by the strip below for an upward one; alternating when
+// both or neither are pending.
+__device__ inline bool sweep_upward(const uint32_t* rowdirty, uint32_t rd, int strip, int sweep) {
+    const bool top = __builtin_amdgcn_readfirstlane(rowdirty[strip * STRIP]) != 0u;
+    const bool bottom = __builtin_amdgcn_readfirstlane(rowdirty[strip * STRIP + STRIP - 1]) != 0u;
+    const bool dn = (rd >> 16) != 0u || top, up = (rd & 0xffffu) != 0u || bottom;
+    return dn != up ? up : (sweep & 1) != 0;
+}
 __device__ inline bool row_due(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
-    const uint32_t bit = 1u << k;
-    bool due = (rd & bit) != 0u;
-    rd &= ~bit;
+    const uint32_t bits = 0x10001u << k;
+    bool due = (rd & bits) != 0u;
+    rd &= ~bits;
     if (k == 0 || k == STRIP - 1) {
         if (__builtin_amdgcn_readfirstlane(rowdirty[ly]) != 0u) {
             rowdirty[ly] = 0u;
@@ -153,7 +171,7 @@ __device__ inline void row_mark(uint32_t* rowdirty, uint32_t& rd, int k, int ly)
     else if (ly > 0 && (threadIdx.x & 63) == 0)
         rowdirty[ly - 1] = 1u;
     if (k < STRIP - 1)
-        rd |= 1u << (k + 1);
+        rd |= 1u << (k + 17);
     else if (ly < TILE - 1 && (threadIdx.x & 63) == 0)
         rowdirty[ly + 1] = 1u;
 }
@@ -232,6 +250,11 @@ __device__ inline uint32_t wave_sum5(uint32_t v) {
     return r;
 }
 
+// Set bits of the wave mask m in the lanes below this one.
+__device__ inline uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Rows of the neighbour in direction d (TDX/TDY order) that a change of
 // this tile's border cells can unsettle: a halo row touches one row of the
 // neighbour (its last row for the northern three, its first for the
@@ -279,7 +302,7 @@ __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, in
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(count_out, (uint32_t)__popcll(m));
     base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-    if (push) work_out[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+    if (push) work_out[base + lanes_below(m)] = t;
 }
 
 // Appends `item` for the lanes with `take` to a list: one atomic per wave.
@@ -291,7 +314,7 @@ __device__ inline void wave_append(bool take, int item, int* __restrict__ out, u
     uint32_t base = 0;
     if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
     base = (uint32_t)__shfl((int)base, first, 64);
-    if (take) out[base + __popcll(m & ((1ull << lane) - 1ull))] = item;
+    if (take) out[base + lanes_below(m)] = item;
 }
 
 // Block-wide OR of per-thread border bits; returns the result in all threads.
