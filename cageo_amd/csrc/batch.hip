@@ -213,10 +213,9 @@ int multi_setup(MultiState* m, Context* const* ctxs, int nr) {
     A(dev_alloc(&m->d_base, (size_t)nr + 1));
     A(dev_alloc(&m->d_bmap, (size_t)m->total));
     A(dev_alloc(&m->d_rinfo, (size_t)nr));
-    A(dev_alloc(&m->d_work, 4 * (size_t)m->total));
+    A(dev_alloc(&m->d_work, 6 * (size_t)m->total));
     A(dev_alloc(&m->d_mark_pf, (size_t)m->total));
     A(dev_alloc(&m->d_mark_fr, (size_t)m->total));
-    A(dev_alloc(&m->d_topen, (size_t)m->total));
     A(dev_alloc(&m->d_rows, 4 * (size_t)m->total));
     A(dev_alloc(&m->d_ctrl, (size_t)CTL_WORDS));
     A(dev_alloc(&m->d_cert, 3 * (size_t)nr));
@@ -231,7 +230,6 @@ int multi_setup(MultiState* m, Context* const* ctxs, int nr) {
              hipSuccess ||
          hipMemset(m->d_mark_pf, 0, (size_t)m->total * sizeof(uint32_t)) != hipSuccess ||
          hipMemset(m->d_mark_fr, 0, (size_t)m->total * sizeof(uint32_t)) != hipSuccess ||
-         hipMemset(m->d_topen, 0, (size_t)m->total * sizeof(uint32_t)) != hipSuccess ||
          hipMemset(m->d_ctrl, 0, CTL_WORDS * sizeof(uint32_t)) != hipSuccess ||
          hipMemset(m->d_cert, 0, 3 * (size_t)nr * sizeof(uint32_t)) != hipSuccess ||
          hipMemset(m->d_bar, 0, BAR_WORDS * sizeof(uint32_t)) != hipSuccess))
@@ -244,7 +242,7 @@ int multi_setup(MultiState* m, Context* const* ctxs, int nr) {
 }
 
 void multi_release(MultiState* m) {
-    void* ptrs[] = {m->d_base, m->d_bmap, m->d_rinfo, m->d_work, m->d_mark_pf, m->d_mark_fr, m->d_topen, m->d_rows, m->d_ctrl,
+    void* ptrs[] = {m->d_base, m->d_bmap, m->d_rinfo, m->d_work, m->d_mark_pf, m->d_mark_fr, m->d_rows, m->d_ctrl,
                     m->d_cert, m->d_bar, m->d_pf, m->d_fr, m->d_tr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

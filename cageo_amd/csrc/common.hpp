@@ -52,10 +52,9 @@ struct Context {
     int ntx = 0, nty = 0, ntiles = 0;
     int* d_work[2] = {nullptr, nullptr};
     uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
-    uint32_t* d_topen = nullptr; // per-tile flag: tile holds OPEN pit-fill cells
     uint32_t* d_mark_pf = nullptr;  // epoch stamps of the pit-fill kernel
     uint32_t* d_rows = nullptr;     // per tile: dirty-row masks, 2 pass parities x 64 bits
-    int* d_work3 = nullptr;         // 3 worklists for the persistent kernels
+    int* d_work3 = nullptr;         // 6 worklists of the persistent kernels
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
     uint32_t* d_bar = nullptr;      // grid barrier words (BAR_WORDS), see grid_sync
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
@@ -205,8 +204,8 @@ enum : int {
     CTL_EPOCH_FR = 13,  // epoch counter of the flat-resolution worklists
     CTL_FR_PASSES = 14, // flat-resolution passes of the last launch
     CTL_TR_ROUNDS = 15, // exit-jump rounds of the last tracing launch
-    CTL_FINAL = 16,     // count of the pit fill's final list
-    CTL_WORDS = 17
+    CTL_PHASE = 16,     // 3 phase-list counts (pit fill)
+    CTL_WORDS = 19
 };
 
 // Several rasters in one persistent launch: raster r owns the global 64x64
@@ -231,10 +230,9 @@ struct MultiState {
     int* d_base = nullptr;            // nr + 1 block offsets
     int2* d_bmap = nullptr;           // per block: (raster, block within it)
     uint2* d_rinfo = nullptr;         // per raster: pit-fill minimum, domain ok
-    int* d_work = nullptr;            // 4 lists of `total` entries
+    int* d_work = nullptr;            // 6 lists of `total` entries
     uint32_t* d_mark_pf = nullptr;    // per-block epoch stamps (pit fill)
     uint32_t* d_mark_fr = nullptr;    // per-block epoch stamps (flats)
-    uint32_t* d_topen = nullptr;      // per-block OPEN flag (pit fill)
     uint32_t* d_rows = nullptr;       // per block: dirty-row masks, 2 pass parities x 64 bits
     uint32_t* d_ctrl = nullptr;       // CTL_WORDS
     uint32_t* d_cert = nullptr;       // 3 rotating counters per raster

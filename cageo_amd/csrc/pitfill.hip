@@ -29,7 +29,11 @@
 // round that fixes nothing leaves only unreached cells; they keep Z.
 //
 // Execution: one persistent kernel runs every round.  Passes over a
-// de-duplicated worklist of 64x64 tiles are separated by grid barriers;
+// de-duplicated worklist of 64x64 tiles are separated by grid barriers; the
+// first pass of a phase takes the tiles that were open after their first
+// solve in the phase before (OPEN cells never reappear, so that covers every
+// open tile), and the fill's first pass is also the restart (OPEN cells,
+// its own and its halo's, read as +inf; it rewrites its own);
 // inside a tile, rows are solved exactly one at a time (wave scans along the
 // row, rows revisited while a neighbouring row changed).  Every decision that
 // ends a loop is read by all workgroups after a barrier, so they leave
@@ -119,9 +123,8 @@ struct PfArgs {
     const int2* __restrict__ bmap;  // per block: (raster, block within it)
     uint2* rinfo;                   // per raster: minimum (float bits), domain ok
     int nr, total;
-    int* work;            // 4 lists of `total` entries: 3 rotating, 1 final
+    int* work;            // 6 lists of `total` entries: 3 rotating pass lists, 3 phase lists
     uint32_t* mark;       // per-block epoch stamps
-    uint32_t* topen;      // per-block: block holds OPEN cells
     uint32_t* rows;       // per-block dirty-row masks: 2 pass parities x 64 bits
     uint32_t* ctrl;       // control block (CTL_*)
     uint32_t* cert;       // per raster: 3 rotating per-round fixed-cell counters
@@ -177,7 +180,9 @@ __device__ inline bool pf_raster_ok(const PfRaster& R, float& mn) {
 // Loads the tile's relays with a one-cell halo.  The state byte and the
 // value of a cell are loaded together (no load waits on another), over a
 // fully unrolled loop, so the loads of all iterations are in flight at once.
-__device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0) {
+// With `restart`, OPEN cells relay nothing: their stored values are the
+// last round's (a neighbour that lowers one in this pass activates the tile).
+__device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0, bool restart) {
     constexpr int N = HALO_W * HALO_W;
     constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
     constexpr int CHUNK = (ITER + 1) / 2;  // two batches keep the register use down
@@ -205,7 +210,7 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0)
             const int k = threadIdx.x + (c0 + j) * TILE_THREADS;
             if (c0 + j < ITER && k < N) {
                 const uint8_t st = sv[j];
-                const float r = (st == PF_OPEN || st == PF_FIXED_LIVE) ? wv[j] : PF_INF;
+                const float r = (st == PF_FIXED_LIVE || (st == PF_OPEN && !restart)) ? wv[j] : PF_INF;
                 S.rel[k] = make_float2(r, st == PF_FIXED_LIVE ? r : PF_INF);
             }
         }
@@ -217,14 +222,15 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0)
     }
 }
 
-// Ends a tile: OR of border bits, tile-open flag, neighbour activation with
-// the rows each neighbour must solve again in pass `pass` + 1.  `chg` holds
-// the thread's changed cells that its neighbours see (bit k = row k of its
-// strip).
+// Ends a tile: OR of border bits, the next phase's list (a tile still open
+// after the first pass of this phase joins phase list `phase_out`),
+// neighbour activation with the rows each neighbour must solve again in
+// pass `pass` + 1.  `chg` holds the thread's changed cells that its
+// neighbours see (bit k = row k of its strip).
 __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& b, uint32_t bits,
                                    uint32_t open_left, uint32_t nfixed, uint32_t chg,
                                    uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
-                                   uint32_t epoch, int pass) {
+                                   uint32_t epoch, int pass, int phase_out) {
     // per-lane: bits < 256, nfixed <= STRIP = 16
     bits = wave_or8(bits);
     open_left = __any(open_left != 0u) ? 1u : 0u;
@@ -236,7 +242,10 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
         if (nfixed) atomicAdd(cert_ctr, nfixed);
     }
     __syncthreads();
-    if (threadIdx.x == 0) a.topen[b.gbase + b.lb] = S.any;
+    if (pass == 0 && threadIdx.x == 0 && S.any) {
+        const uint32_t at = atomicAdd(&a.ctrl[CTL_PHASE + phase_out], 1u);
+        a.work[(size_t)(3 + phase_out) * a.total + at] = b.gbase + b.lb;
+    }
     if (threadIdx.x < 64 && S.bits) {  // uniform within the first wave
         unsigned long long w64, e64;
         edge_columns_get(S.wcol, S.ecol, w64, e64);
@@ -246,18 +255,45 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
     __syncthreads();
 }
 
+// OPEN cells of a raster whose last round fixed nothing are never reached
+// and keep their quantized elevation.
+__device__ inline void keep_quantized(const PfRaster& R, int lb) {
+    const int x0 = (lb % R.ntx) * TILE, y0 = (lb / R.ntx) * TILE;
+    const int gx = x0 + (threadIdx.x & 63);
+    if (gx >= R.W) return;
+    const int gy0 = y0 + (threadIdx.x >> 6) * STRIP;
+    uint8_t sv[STRIP];
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) sv[k] = gy0 + k < R.H ? R.st[(size_t)(gy0 + k) * R.W + gx] : 0;
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) {
+        if (gy0 + k >= R.H || sv[k] != PF_OPEN) continue;
+        const size_t gi = (size_t)(gy0 + k) * R.W + gx;
+        R.w[gi] = R.zq[gi];
+    }
+}
+
 // Fill relaxation of one tile (with immediate fixing at Z).  Cell state
 // lives in LDS (relays in rel, Z in zt); each thread keeps bit masks of its
 // strip's cells only, so the row loop can run in either direction without
 // per-thread arrays.
+// The first pass of a round is its restart: OPEN cells start from +inf and
+// all of the tile's are rewritten (in round 0 they already hold +inf).  In
+// it, a raster whose last round fixed nothing is finished instead.
 __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
                           uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
-                          int pass) {
+                          int pass, int round, int phase_out) {
     const PfRaster& a = b.R;
     const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
     if (dirty == 0) return;  // uniform
+    const bool restart = pass == 0;
+    if (restart && round > 0 &&
+        __builtin_amdgcn_readfirstlane(ld(&args.cert[3 * b.r + (round + 2) % 3])) == 0u) {
+        keep_quantized(a, b.lb);
+        return;
+    }
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    load_tile(S, a, x0, y0);
+    load_tile(S, a, x0, y0, restart);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
     uint32_t open = 0, zlm = 0;  // OPEN cells; OPEN cells whose Z is live
@@ -287,6 +323,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     }
     __syncthreads();
     clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
+    const uint32_t rewrite = restart && round > 0 ? open : 0u;  // stale values of the last round
 
     // Row step (row ly, all 64 columns at once):
     //  1. Z-rule: an OPEN cell with a live FIXED neighbour no higher than Z
@@ -380,7 +417,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     for (int k = 0; k < STRIP; ++k) {
         const int ly = strip * STRIP + k;
         const size_t gi = (size_t)(y0 + ly) * a.W + gx;
-        if (lowered & (1u << k))
+        if ((lowered | rewrite) & (1u << k))
             a.w[gi] = (fixed & (1u << k)) ? S.zt[ly * TILE + lx]
                                           : S.rel[(ly + 1) * HALO_W + lx + 1].x;
         if (fixed & (1u << k)) {
@@ -390,18 +427,18 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         if ((lowered | fixed) & (1u << k)) bits |= border_bits(lx, ly);
     }
     finish_tile(S, args, b, bits, open, nfixed, lowered | fixed, cert_ctr, work_out, count_out,
-                epoch, pass);
+                epoch, pass, phase_out);
 }
 
 // Certification of one tile after the fill has converged.
 __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
                           uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
-                          int pass) {
+                          int pass, int phase_out) {
     const PfRaster& a = b.R;
     const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
     if (dirty == 0) return;  // uniform
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    load_tile(S, a, x0, y0);
+    load_tile(S, a, x0, y0, false);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
     uint32_t open = 0, open_inf = 0;
@@ -486,95 +523,49 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
     }
     finish_tile(S, args, b, bits, open | open_inf, nfixed, fixed_live, cert_ctr, work_out,
-                count_out, epoch, pass);
+                count_out, epoch, pass, phase_out);
 }
 
-// Runs passes of `kind` (0 fill, 1 certify) from worklist `li` until a pass
-// lists nothing; returns the index of the (empty) list it stopped at, or -1
-// when the pass cap is hit.  Fixed cells count into counter `slot` of their
-// raster.
+// Runs the passes of phase `phase` (kind 0 fill, 1 certify) until a pass
+// lists nothing; returns the index of the (empty) rotating list it stopped
+// at, or -1 when the pass cap is hit.  The first pass reads phase list
+// phase % 3 (and plays the part of rotating list `li`, the empty one the
+// last phase stopped at); its still-open tiles form phase list
+// (phase + 1) % 3; list (phase + 2) % 3, last read before the barrier that
+// ended the phase before, is cleared for the phase after.  Fixed cells
+// count into counter `slot` of their raster.
 template <int KIND, bool MULTI>
-__device__ int run_passes(PfShared& S, const PfArgs& a, int li, int slot, uint32_t& epoch,
-                          float mn1, bool ok1) {
+__device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int slot, int round,
+                          uint32_t& epoch, float mn1, bool ok1) {
+    const int pin = phase % 3, pout = (phase + 1) % 3;
     for (int pass = 0; pass < a.max_passes; ++pass) {
-        const int cin = li, cout = (li + 1) % 3, cclr = (li + 2) % 3;
-        const uint32_t count = ld(&a.ctrl[CTL_COUNT + cin]);
-        if (count == 0) return cin;
+        const int cout = (li + 1) % 3, cclr = (li + 2) % 3;
+        const int lin = pass == 0 ? 3 + pin : li;
+        const uint32_t count = ld(&a.ctrl[pass == 0 ? CTL_PHASE + pin : CTL_COUNT + li]);
+        if (count == 0) return li;
         ++epoch;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.ctrl[CTL_COUNT + cclr] = 0;
+            if (pass == 0) a.ctrl[CTL_PHASE + (phase + 2) % 3] = 0;
             atomicAdd(&a.ctrl[CTL_PASSES], 1u);
         }
-        const int* win = a.work + (size_t)cin * a.total;
+        const int* win = a.work + (size_t)lin * a.total;
         int* wout = a.work + (size_t)cout * a.total;
         for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
             const PfBlock b = pf_block<MULTI>(a, win[i], mn1, ok1);
             if (!b.ok) continue;  // uniform; such blocks never open
             uint32_t* cert_ctr = &a.cert[3 * b.r + slot];
             if (KIND == 0)
-                fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass);
+                fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass,
+                          round, pout);
             else
-                cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass);
+                cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass,
+                          pout);
         }
         grid_sync(a.bar);
         li = cout;
     }
     return -1;
-}
-
-// Lists the blocks flagged open into list (b+1) % 3 and returns its index.
-// `b` is the list run_passes stopped at: some workgroups may still be reading
-// its (zero) count, so it is left untouched; (b+1) % 3 was cleared during the
-// last pass and nobody reads it; (b+2) % 3 was last read before the last
-// barrier and is cleared here for the first pass of the next phase.
-// With `slot` >= 0 (after certification), open blocks of a raster whose
-// round fixed nothing (counter `slot` zero) go to the final list instead:
-// that raster is done.  Without, the final list's count is cleared (its last
-// reads were before the barrier that ends the round's restart phase).  The
-// caller ends with a grid barrier.
-template <bool MULTI>
-__device__ inline int build_open_list(const PfArgs& a, int b, int slot) {
-    const int l = (b + 1) % 3;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctrl[CTL_COUNT + (b + 2) % 3] = 0;
-        if (slot < 0) a.ctrl[CTL_FINAL] = 0;
-    }
-    int* out = a.work + (size_t)l * a.total;
-    int* fin = a.work + (size_t)3 * a.total;
-    // whole waves iterate (the appends are wave-aggregated)
-    const int n = (a.total + 63) & ~63;
-    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
-        const bool open = g < a.total && a.topen[g] != 0u;
-        bool done = false;
-        if (open && slot >= 0) {
-            const int r = MULTI ? a.bmap[g].x : 0;
-            done = ld(&a.cert[3 * r + slot]) == 0u;
-        }
-        wave_append(open && done, g, fin, &a.ctrl[CTL_FINAL]);
-        wave_append(open && !done, g, out, &a.ctrl[CTL_COUNT + l]);
-    }
-    return l;
-}
-
-// OPEN cells of global block g: restart at +inf (`value` false) or keep
-// their quantized elevation (`value` true: never reached).
-template <bool MULTI>
-__device__ inline void open_cells_set(const PfArgs& a, int g, bool to_z) {
-    const PfBlock b = pf_block<MULTI>(a, g, 0.0f, true);
-    const PfRaster& R = b.R;
-    const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
-    const int gx = x0 + (threadIdx.x & 63);
-    if (gx >= R.W) return;
-    const int gy0 = y0 + (threadIdx.x >> 6) * STRIP;
-    uint8_t sv[STRIP];
-#pragma unroll
-    for (int k = 0; k < STRIP; ++k) sv[k] = gy0 + k < R.H ? R.st[(size_t)(gy0 + k) * R.W + gx] : 0;
-#pragma unroll
-    for (int k = 0; k < STRIP; ++k) {
-        if (gy0 + k >= R.H || sv[k] != PF_OPEN) continue;
-        const size_t gi = (size_t)(gy0 + k) * R.W + gx;
-        R.w[gi] = to_z ? R.zq[gi] : PF_INF;
-    }
 }
 
 // MULTI: a table of rasters (else the one raster of the arguments; two
@@ -607,10 +598,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     a.rinfo[b.r] = make_uint2(__float_as_uint(mn), ok ? 1u : 0u);
             }
             if (!ok) {
-                if (row0 == 0 && lane == 0) {
-                    a.topen[g] = 0;
-                    if (b.lb == 0) *R.status = (uint32_t)(-CAGEO_ERR_RANGE);
-                }
+                if (b.lb == 0 && row0 == 0 && lane == 0) *R.status = (uint32_t)(-CAGEO_ERR_RANGE);
                 continue;
             }
             const int x = (b.lb % R.ntx) * TILE + lane, y0 = (b.lb / R.ntx) * TILE + row0;
@@ -640,55 +628,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
     }
-    // list 0: every block (those of rasters outside the domain are skipped)
+    // phase list 0: every block (those of rasters outside the domain are skipped)
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x) {
-        a.work[g] = g;
+        a.work[(size_t)3 * a.total + g] = g;
         reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
     }
     for (int k = threadIdx.x; k < 3 * a.nr; k += blockDim.x)
         if (blockIdx.x == 0) a.cert[k] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctrl[CTL_COUNT + 0] = (uint32_t)a.total;
+        a.ctrl[CTL_COUNT + 0] = 0;
         a.ctrl[CTL_COUNT + 1] = 0;
         a.ctrl[CTL_COUNT + 2] = 0;
-        a.ctrl[CTL_FINAL] = 0;
+        a.ctrl[CTL_PHASE + 0] = (uint32_t)a.total;
+        a.ctrl[CTL_PHASE + 1] = 0;
+        a.ctrl[CTL_PHASE + 2] = 0;
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
     grid_sync(a.bar);
 
-    int li = 0;
-    int round = 0;
+    // Phases alternate fill and certification.  Round r counts fixed cells
+    // into counter slot r % 3 of each raster; its fill's first pass reads
+    // slot (r+2) % 3 (round r-1's); slot (r+1) % 3, last read by round r-1's
+    // first pass, is cleared once round r's fill is over.
+    int li = 0, phase = 0, round = 0;
     for (;; ++round) {
-        // Round r counts into counter slot r % 3 of each raster and clears slot
-        // (r+1) % 3, whose last read (end of round r-2) is separated from here
-        // by barriers.
         const int slot = round % 3;
-        if (round > 0) {
-            // OPEN cells of the listed blocks restart from +inf
-            const uint32_t count = ld(&a.ctrl[CTL_COUNT + li]);
-            const int* lst = a.work + (size_t)li * a.total;
-            for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
-                open_cells_set<MULTI>(a, lst[i], false);
-            if (blockIdx.x == 0)
-                for (int r = threadIdx.x; r < a.nr; r += blockDim.x) a.cert[3 * r + (round + 1) % 3] = 0;
-            grid_sync(a.bar);
-        }
-        li = run_passes<0, MULTI>(S, a, li, slot, epoch, mn1, ok1);
+        li = run_passes<0, MULTI>(S, a, li, phase, slot, round, epoch, mn1, ok1);
         if (li < 0) break;
-        li = build_open_list<MULTI>(a, li, -1);
-        grid_sync(a.bar);
-        if (ld(&a.ctrl[CTL_COUNT + li]) == 0) break;
-        li = run_passes<1, MULTI>(S, a, li, slot, epoch, mn1, ok1);
+        ++phase;
+        if (blockIdx.x == 0)
+            for (int r = threadIdx.x; r < a.nr; r += blockDim.x) a.cert[3 * r + (round + 1) % 3] = 0;
+        if (ld(&a.ctrl[CTL_PHASE + phase % 3]) == 0) break;
+        li = run_passes<1, MULTI>(S, a, li, phase, slot, round, epoch, mn1, ok1);
         if (li < 0) break;
-        li = build_open_list<MULTI>(a, li, slot);
-        grid_sync(a.bar);
-        // rasters whose round fixed nothing: their OPEN cells are never
-        // reached and keep their quantized elevation
-        const uint32_t nfinal = ld(&a.ctrl[CTL_FINAL]);
-        const int* fin = a.work + (size_t)3 * a.total;
-        for (uint32_t i = blockIdx.x; i < nfinal; i += gridDim.x) open_cells_set<MULTI>(a, fin[i], true);
-        if (ld(&a.ctrl[CTL_COUNT + li]) == 0) break;
+        ++phase;
+        if (ld(&a.ctrl[CTL_PHASE + phase % 3]) == 0) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_EPOCH] = epoch;
@@ -748,7 +723,6 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.total = c->ntiles;
     a.work = c->d_work3;
     a.mark = c->d_mark_pf;
-    a.topen = c->d_topen;
     a.rows = c->d_rows;
     a.ctrl = c->d_ctrl;
     a.cert = c->d_ctrl + CTL_CERT;
@@ -785,7 +759,6 @@ int pit_fill_multi(MultiState* m, hipStream_t s) {
     a.total = m->total;
     a.work = m->d_work;
     a.mark = m->d_mark_pf;
-    a.topen = m->d_topen;
     a.rows = m->d_rows;
     a.ctrl = m->d_ctrl;
     a.cert = m->d_cert;
