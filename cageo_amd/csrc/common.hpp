@@ -205,7 +205,8 @@ enum : int {
     CTL_FR_PASSES = 14, // flat-resolution passes of the last launch
     CTL_TR_ROUNDS = 15, // exit-jump rounds of the last tracing launch
     CTL_PHASE = 16,     // 3 phase-list counts (pit fill)
-    CTL_WORDS = 19
+    CTL_CLAIM = 19,     // 3 rotating tile-claim counters of the passes
+    CTL_WORDS = 22
 };
 
 // Several rasters in one persistent launch: raster r owns the global 64x64

@@ -58,6 +58,7 @@ struct PfShared {
     uint32_t wcol[TILE_THREADS / 64], ecol[TILE_THREADS / 64];  // changed rows of columns 0, TILE-1
     uint32_t bits;
     uint32_t any;
+    uint32_t next;  // the workgroup's next tile index in the pass
 };
 
 __device__ inline float min8x(const float2* s, int i) {
@@ -546,21 +547,29 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
         ++epoch;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.ctrl[CTL_COUNT + cclr] = 0;
+            a.ctrl[CTL_CLAIM + cout] = 0;  // the next pass's claims (this one's were cleared before)
             if (pass == 0) a.ctrl[CTL_PHASE + (phase + 2) % 3] = 0;
             atomicAdd(&a.ctrl[CTL_PASSES], 1u);
         }
         const int* win = a.work + (size_t)lin * a.total;
         int* wout = a.work + (size_t)cout * a.total;
-        for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        for (uint32_t i = blockIdx.x; i < count;) {
+            uint32_t next = count;
+            if (threadIdx.x == 0) next = claim_next(&a.ctrl[CTL_CLAIM + li], count);
             const PfBlock b = pf_block<MULTI>(a, win[i], mn1, ok1);
-            if (!b.ok) continue;  // uniform; such blocks never open
-            uint32_t* cert_ctr = &a.cert[3 * b.r + slot];
-            if (KIND == 0)
-                fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass,
-                          round, pout);
-            else
-                cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass,
-                          pout);
+            if (b.ok) {  // uniform; blocks of rasters outside the domain never open
+                uint32_t* cert_ctr = &a.cert[3 * b.r + slot];
+                if (KIND == 0)
+                    fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass,
+                              round, pout);
+                else
+                    cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch,
+                              pass, pout);
+            }
+            if (threadIdx.x == 0) S.next = next;
+            __syncthreads();
+            i = S.next;
+            __syncthreads();
         }
         grid_sync(a.bar);
         li = cout;
@@ -642,6 +651,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         a.ctrl[CTL_PHASE + 0] = (uint32_t)a.total;
         a.ctrl[CTL_PHASE + 1] = 0;
         a.ctrl[CTL_PHASE + 2] = 0;
+        a.ctrl[CTL_CLAIM + 0] = 0;
+        a.ctrl[CTL_CLAIM + 1] = 0;
+        a.ctrl[CTL_CLAIM + 2] = 0;
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);

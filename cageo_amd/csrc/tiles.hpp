@@ -176,6 +176,15 @@ __device__ inline void row_mark(uint32_t* rowdirty, uint32_t& rd, int k, int ly)
         rowdirty[ly + 1] = 1u;
 }
 
+// Tiles of a pass are taken in turn: the first gridDim.x by workgroup
+// number, the rest claimed from counter `claim` (a slow tile holds back no
+// tile behind it).  The claim is issued when the workgroup starts a tile
+// and read when it ends, so its latency hides behind the tile's loads.
+// Called by thread 0; returns the workgroup's next index (>= count: none).
+__device__ inline uint32_t claim_next(uint32_t* claim, uint32_t count) {
+    return count > gridDim.x ? gridDim.x + atomicAdd(claim, 1u) : count;
+}
+
 // Dirty-row masks of tiles solved in passes (two 32-bit words per pass
 // parity, four per tile, at `rows`).  The rows of global tile g due for a
 // solve in pass `pass` of a phase: every row in the phase's first pass;
