@@ -53,8 +53,9 @@ struct Context {
     int* d_work[2] = {nullptr, nullptr};
     uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
     uint32_t* d_mark_pf = nullptr;  // epoch stamps of the pit-fill kernel
+    uint32_t* d_topen = nullptr;    // per tile: holds OPEN pit-fill cells
     uint32_t* d_rows = nullptr;     // per tile: dirty-row masks, 2 pass parities x 64 bits
-    int* d_work3 = nullptr;         // 6 worklists of the persistent kernels
+    int* d_work3 = nullptr;         // 7 worklists of the persistent kernels
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
     uint32_t* d_bar = nullptr;      // grid barrier words (BAR_WORDS), see grid_sync
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
@@ -206,7 +207,8 @@ enum : int {
     CTL_TR_ROUNDS = 15, // exit-jump rounds of the last tracing launch
     CTL_PHASE = 16,     // 3 phase-list counts (pit fill)
     CTL_CLAIM = 19,     // 3 rotating tile-claim counters of the passes
-    CTL_WORDS = 22
+    CTL_COMPACT = 22,   // count of the pit fill's compacted fill list
+    CTL_WORDS = 23
 };
 
 // Several rasters in one persistent launch: raster r owns the global 64x64
@@ -231,10 +233,11 @@ struct MultiState {
     int* d_base = nullptr;            // nr + 1 block offsets
     int2* d_bmap = nullptr;           // per block: (raster, block within it)
     uint2* d_rinfo = nullptr;         // per raster: pit-fill minimum, domain ok
-    int* d_work = nullptr;            // 6 lists of `total` entries
+    int* d_work = nullptr;            // 7 lists of `total` entries
     uint32_t* d_mark_pf = nullptr;    // per-block epoch stamps (pit fill)
     uint32_t* d_mark_fr = nullptr;    // per-block epoch stamps (flats)
     uint32_t* d_rows = nullptr;       // per block: dirty-row masks, 2 pass parities x 64 bits
+    uint32_t* d_topen = nullptr;      // per block: holds OPEN pit-fill cells
     uint32_t* d_ctrl = nullptr;       // CTL_WORDS
     uint32_t* d_cert = nullptr;       // 3 rotating counters per raster
     uint32_t* d_bar = nullptr;        // grid barrier words

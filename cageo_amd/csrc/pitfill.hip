@@ -32,8 +32,10 @@
 // de-duplicated worklist of 64x64 tiles are separated by grid barriers; the
 // first pass of a phase takes the tiles that were open after their first
 // solve in the phase before (OPEN cells never reappear, so that covers every
-// open tile), and the fill's first pass is also the restart (OPEN cells,
-// its own and its halo's, read as +inf; it rewrites its own);
+// open tile; the fill's list is cut down to the tiles still open, since most
+// tiles certified at all are certified after the first pass), and the
+// fill's first pass is also the restart (OPEN cells, its own and its
+// halo's, read as +inf; it rewrites its own);
 // inside a tile, rows are solved exactly one at a time (wave scans along the
 // row, rows revisited while a neighbouring row changed).  Every decision that
 // ends a loop is read by all workgroups after a barrier, so they leave
@@ -124,8 +126,10 @@ struct PfArgs {
     const int2* __restrict__ bmap;  // per block: (raster, block within it)
     uint2* rinfo;                   // per raster: minimum (float bits), domain ok
     int nr, total;
-    int* work;            // 6 lists of `total` entries: 3 rotating pass lists, 3 phase lists
+    int* work;            // 7 lists of `total` entries: 3 rotating pass lists, 3 phase lists,
+                          // the compacted fill list
     uint32_t* mark;       // per-block epoch stamps
+    uint32_t* topen;      // per-block: holds OPEN cells (written by every solve)
     uint32_t* rows;       // per-block dirty-row masks: 2 pass parities x 64 bits
     uint32_t* ctrl;       // control block (CTL_*)
     uint32_t* cert;       // per raster: 3 rotating per-round fixed-cell counters
@@ -243,6 +247,7 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
         if (nfixed) atomicAdd(cert_ctr, nfixed);
     }
     __syncthreads();
+    if (threadIdx.x == 0) a.topen[b.gbase + b.lb] = S.any;
     if (pass == 0 && threadIdx.x == 0 && S.any) {
         const uint32_t at = atomicAdd(&a.ctrl[CTL_PHASE + phase_out], 1u);
         a.work[(size_t)(3 + phase_out) * a.total + at] = b.gbase + b.lb;
@@ -438,6 +443,8 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     const PfRaster& a = b.R;
     const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
     if (dirty == 0) return;  // uniform
+    if (pass == 0 && __builtin_amdgcn_readfirstlane(ld(&args.topen[b.gbase + b.lb])) == 0u)
+        return;  // closed since it joined the phase list
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     load_tile(S, a, x0, y0, false);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
@@ -535,20 +542,24 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
 // (phase + 1) % 3; list (phase + 2) % 3, last read before the barrier that
 // ended the phase before, is cleared for the phase after.  Fixed cells
 // count into counter `slot` of their raster.
+// With `compacted`, the first pass reads the compacted fill list instead.
 template <int KIND, bool MULTI>
 __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int slot, int round,
-                          uint32_t& epoch, float mn1, bool ok1) {
+                          bool compacted, uint32_t& epoch, float mn1, bool ok1) {
     const int pin = phase % 3, pout = (phase + 1) % 3;
     for (int pass = 0; pass < a.max_passes; ++pass) {
         const int cout = (li + 1) % 3, cclr = (li + 2) % 3;
-        const int lin = pass == 0 ? 3 + pin : li;
-        const uint32_t count = ld(&a.ctrl[pass == 0 ? CTL_PHASE + pin : CTL_COUNT + li]);
+        const int lin = pass > 0 ? li : compacted ? 6 : 3 + pin;
+        const int cin = pass > 0 ? CTL_COUNT + li : compacted ? CTL_COMPACT : CTL_PHASE + pin;
+        const uint32_t count = ld(&a.ctrl[cin]);
         if (count == 0) return li;
         ++epoch;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.ctrl[CTL_COUNT + cclr] = 0;
             a.ctrl[CTL_CLAIM + cout] = 0;  // the next pass's claims (this one's were cleared before)
             if (pass == 0) a.ctrl[CTL_PHASE + (phase + 2) % 3] = 0;
+            // a certification follows the fill that last read the compacted list
+            if (pass == 0 && KIND == 1) a.ctrl[CTL_COMPACT] = 0;
             atomicAdd(&a.ctrl[CTL_PASSES], 1u);
         }
         const int* win = a.work + (size_t)lin * a.total;
@@ -575,6 +586,25 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
         li = cout;
     }
     return -1;
+}
+
+// The tiles of phase list `from` still open, into the compacted fill list
+// (cleared by the certification's first pass; the caller ends with a grid
+// barrier).
+__device__ inline void compact_open(const PfArgs& a, int from) {
+    const uint32_t n = ld(&a.ctrl[CTL_PHASE + from]);
+    const int* src = a.work + (size_t)(3 + from) * a.total;
+    int* dst = a.work + (size_t)6 * a.total;
+    const uint32_t nw = (n + 63u) & ~63u;  // whole waves (the appends are wave-aggregated)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
+        int g = 0;
+        bool keep = false;
+        if (i < n) {
+            g = src[i];
+            keep = a.topen[g] != 0u;
+        }
+        wave_append(keep, g, dst, &a.ctrl[CTL_COMPACT]);
+    }
 }
 
 // MULTI: a table of rasters (else the one raster of the arguments; two
@@ -640,6 +670,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // phase list 0: every block (those of rasters outside the domain are skipped)
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x) {
         a.work[(size_t)3 * a.total + g] = g;
+        a.topen[g] = 1u;  // until its first solve
         reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
     }
     for (int k = threadIdx.x; k < 3 * a.nr; k += blockDim.x)
@@ -654,6 +685,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         a.ctrl[CTL_CLAIM + 0] = 0;
         a.ctrl[CTL_CLAIM + 1] = 0;
         a.ctrl[CTL_CLAIM + 2] = 0;
+        a.ctrl[CTL_COMPACT] = 0;
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
@@ -666,16 +698,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     int li = 0, phase = 0, round = 0;
     for (;; ++round) {
         const int slot = round % 3;
-        li = run_passes<0, MULTI>(S, a, li, phase, slot, round, epoch, mn1, ok1);
+        li = run_passes<0, MULTI>(S, a, li, phase, slot, round, round > 0, epoch, mn1, ok1);
         if (li < 0) break;
         ++phase;
         if (blockIdx.x == 0)
             for (int r = threadIdx.x; r < a.nr; r += blockDim.x) a.cert[3 * r + (round + 1) % 3] = 0;
         if (ld(&a.ctrl[CTL_PHASE + phase % 3]) == 0) break;
-        li = run_passes<1, MULTI>(S, a, li, phase, slot, round, epoch, mn1, ok1);
+        li = run_passes<1, MULTI>(S, a, li, phase, slot, round, false, epoch, mn1, ok1);
         if (li < 0) break;
         ++phase;
         if (ld(&a.ctrl[CTL_PHASE + phase % 3]) == 0) break;
+        compact_open(a, phase % 3);
+        grid_sync(a.bar);
+        if (ld(&a.ctrl[CTL_COMPACT]) == 0) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_EPOCH] = epoch;
@@ -735,6 +770,7 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.total = c->ntiles;
     a.work = c->d_work3;
     a.mark = c->d_mark_pf;
+    a.topen = c->d_topen;
     a.rows = c->d_rows;
     a.ctrl = c->d_ctrl;
     a.cert = c->d_ctrl + CTL_CERT;
@@ -771,6 +807,7 @@ int pit_fill_multi(MultiState* m, hipStream_t s) {
     a.total = m->total;
     a.work = m->d_work;
     a.mark = m->d_mark_pf;
+    a.topen = m->d_topen;
     a.rows = m->d_rows;
     a.ctrl = m->d_ctrl;
     a.cert = m->d_cert;
