@@ -53,7 +53,7 @@ int alloc(T** p, size_t count) {
 }
 
 void free_all(Context* c) {
-    void* ptrs[] = {c->d_work[0], c->d_work[1], c->d_mark, c->d_mark_pf, c->d_topen, c->d_rows, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_filter,
+    void* ptrs[] = {c->d_work[0], c->d_work[1], c->d_mark, c->d_mark_pf, c->d_topen, c->d_tcost, c->d_rows, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_filter,
                     c->d_sflags, c->d_slabel, c->d_off, c->d_factor, c->d_a, c->d_b, c->d_z,
                     c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr,
                     c->d_lr_col, c->d_lr_row, c->d_lr_planes};
@@ -231,6 +231,7 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
     A(cageo::alloc(&c->d_mark_pf, c->ntiles));
     A(cageo::alloc(&c->d_rows, 4 * (size_t)c->ntiles));
     A(cageo::alloc(&c->d_topen, c->ntiles));
+    A(cageo::alloc(&c->d_tcost, c->ntiles));
     A(cageo::alloc(&c->d_work3, 7 * (size_t)c->ntiles));
     A(cageo::alloc(&c->d_ctrl, cageo::CTL_WORDS));
     A(cageo::alloc(&c->d_bar, cageo::BAR_WORDS));

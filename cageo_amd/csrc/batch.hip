@@ -218,6 +218,7 @@ int multi_setup(MultiState* m, Context* const* ctxs, int nr) {
     A(dev_alloc(&m->d_mark_fr, (size_t)m->total));
     A(dev_alloc(&m->d_rows, 4 * (size_t)m->total));
     A(dev_alloc(&m->d_topen, (size_t)m->total));
+    A(dev_alloc(&m->d_tcost, (size_t)m->total));
     A(dev_alloc(&m->d_ctrl, (size_t)CTL_WORDS));
     A(dev_alloc(&m->d_cert, 3 * (size_t)nr));
     A(dev_alloc(&m->d_bar, (size_t)BAR_WORDS));
@@ -243,7 +244,7 @@ int multi_setup(MultiState* m, Context* const* ctxs, int nr) {
 }
 
 void multi_release(MultiState* m) {
-    void* ptrs[] = {m->d_base, m->d_bmap, m->d_rinfo, m->d_work, m->d_mark_pf, m->d_mark_fr, m->d_rows, m->d_topen, m->d_ctrl,
+    void* ptrs[] = {m->d_base, m->d_bmap, m->d_rinfo, m->d_work, m->d_mark_pf, m->d_mark_fr, m->d_rows, m->d_topen, m->d_tcost, m->d_ctrl,
                     m->d_cert, m->d_bar, m->d_pf, m->d_fr, m->d_tr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

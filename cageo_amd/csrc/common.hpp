@@ -54,6 +54,7 @@ struct Context {
     uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
     uint32_t* d_mark_pf = nullptr;  // epoch stamps of the pit-fill kernel
     uint32_t* d_topen = nullptr;    // per tile: holds OPEN pit-fill cells
+    uint32_t* d_tcost = nullptr;    // per tile: row steps of its last pit-fill solve
     uint32_t* d_rows = nullptr;     // per tile: dirty-row masks, 2 pass parities x 64 bits
     int* d_work3 = nullptr;         // 7 worklists of the persistent kernels
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
@@ -224,7 +225,10 @@ enum : int {
     CTL_PHASE = 16,     // 3 phase-list counts (pit fill)
     CTL_CLAIM = 19,     // 3 rotating tile-claim counters of the passes
     CTL_COMPACT = 22,   // count of the pit fill's compacted fill list
-    CTL_WORDS = 23
+    CTL_COUNTB = 23,    // 3 back counts of the pit fill's two-ended pass lists
+    CTL_PHASEB = 26,    // 3 back counts of its phase lists
+    CTL_COMPACTB = 29,  // back count of its compacted fill list
+    CTL_WORDS = 30
 };
 
 // Several rasters in one persistent launch: raster r owns the global 64x64
@@ -254,6 +258,7 @@ struct MultiState {
     uint32_t* d_mark_fr = nullptr;    // per-block epoch stamps (flats)
     uint32_t* d_rows = nullptr;       // per block: dirty-row masks, 2 pass parities x 64 bits
     uint32_t* d_topen = nullptr;      // per block: holds OPEN pit-fill cells
+    uint32_t* d_tcost = nullptr;      // per block: row steps of its last pit-fill solve
     uint32_t* d_ctrl = nullptr;       // CTL_WORDS
     uint32_t* d_cert = nullptr;       // 3 rotating counters per raster
     uint32_t* d_bar = nullptr;        // grid barrier words

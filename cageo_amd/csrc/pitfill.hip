@@ -61,7 +61,12 @@ struct PfShared {
     uint32_t bits;
     uint32_t any;
     uint32_t next;  // the workgroup's next tile index in the pass
+    uint32_t steps; // row steps of the tile's solve, all waves
 };
+
+// A solve of at least this many row steps (all waves; one sweep is 64)
+// ranks its tile heavy for the next pass (see Ranked).
+constexpr uint32_t PF_HEAVY = 256;
 
 __device__ inline float min8x(const float2* s, int i) {
     float a = fminf(s[i - HALO_W - 1].x, s[i - HALO_W].x);
@@ -130,6 +135,7 @@ struct PfArgs {
                           // the compacted fill list
     uint32_t* mark;       // per-block epoch stamps
     uint32_t* topen;      // per-block: holds OPEN cells (written by every solve)
+    uint32_t* tcost;      // per-block: row steps of its last solve (list ranking)
     uint32_t* rows;       // per-block dirty-row masks: 2 pass parities x 64 bits
     uint32_t* ctrl;       // control block (CTL_*)
     uint32_t* cert;       // per raster: 3 rotating per-round fixed-cell counters
@@ -224,6 +230,7 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0,
     if (threadIdx.x == 0) {
         S.bits = 0;
         S.any = 0;
+        S.steps = 0;
     }
 }
 
@@ -235,7 +242,7 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0,
 __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& b, uint32_t bits,
                                    uint32_t open_left, uint32_t nfixed, uint32_t chg,
                                    uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
-                                   uint32_t epoch, int pass, int phase_out) {
+                                   uint32_t* count_back, uint32_t epoch, int pass, int phase_out) {
     // per-lane: bits < 256, nfixed <= STRIP = 16
     bits = wave_or8(bits);
     open_left = __any(open_left != 0u) ? 1u : 0u;
@@ -247,16 +254,24 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
         if (nfixed) atomicAdd(cert_ctr, nfixed);
     }
     __syncthreads();
-    if (threadIdx.x == 0) a.topen[b.gbase + b.lb] = S.any;
+    if (threadIdx.x == 0) {
+        a.topen[b.gbase + b.lb] = S.any;
+        a.tcost[b.gbase + b.lb] = S.steps;
+    }
     if (pass == 0 && threadIdx.x == 0 && S.any) {
-        const uint32_t at = atomicAdd(&a.ctrl[CTL_PHASE + phase_out], 1u);
-        a.work[(size_t)(3 + phase_out) * a.total + at] = b.gbase + b.lb;
+        int* lst = a.work + (size_t)(3 + phase_out) * a.total;
+        if (S.steps >= PF_HEAVY)
+            lst[atomicAdd(&a.ctrl[CTL_PHASE + phase_out], 1u)] = b.gbase + b.lb;
+        else
+            lst[a.total - 1 - (int)atomicAdd(&a.ctrl[CTL_PHASEB + phase_out], 1u)] = b.gbase + b.lb;
     }
     if (threadIdx.x < 64 && S.bits) {  // uniform within the first wave
         unsigned long long w64, e64;
         edge_columns_get(S.wcol, S.ecol, w64, e64);
+        const Ranked rank{a.tcost, PF_HEAVY, count_back, a.total};
         activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
-                       work_out, count_out, a.mark, epoch, a.rows, (pass + 1) & 1, w64, e64);
+                       work_out, count_out, a.mark, epoch, a.rows, (pass + 1) & 1, w64, e64,
+                       &rank);
     }
     __syncthreads();
 }
@@ -287,8 +302,9 @@ __device__ inline void keep_quantized(const PfRaster& R, int lb) {
 // all of the tile's are rewritten (in round 0 they already hold +inf).  In
 // it, a raster whose last round fixed nothing is finished instead.
 __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
-                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
-                          int pass, int round, int phase_out) {
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
+                          uint32_t* count_back, uint32_t epoch, int pass, int round,
+                          int phase_out) {
     const PfRaster& a = b.R;
     const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
     if (dirty == 0) return;  // uniform
@@ -343,6 +359,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     //     the map chains from the left and right halo cells.
     const int hl = 0, hr = HALO_W - 1;  // halo columns
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
+    uint32_t nrow = 0;  // rows solved by this wave
     uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
@@ -354,6 +371,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             const uint32_t bit = 1u << k;
             const bool op = (open & bit) != 0;
             if (__ballot(op) == 0) continue;  // no OPEN cell: nothing in the row can change
+            ++nrow;
             const int li = (ly + 1) * HALO_W + lx + 1;
             // The step's reads at once: the cell, its neighbours in the rows
             // above and below (relay and live value), the live values of its
@@ -417,6 +435,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
         if (!__syncthreads_or(changed)) break;
     }
+    if ((threadIdx.x & 63) == 0 && nrow) atomicAdd(&S.steps, nrow);
 
     uint32_t bits = 0, nfixed = 0;
 #pragma unroll
@@ -433,13 +452,13 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         if ((lowered | fixed) & (1u << k)) bits |= border_bits(lx, ly);
     }
     finish_tile(S, args, b, bits, open, nfixed, lowered | fixed, cert_ctr, work_out, count_out,
-                epoch, pass, phase_out);
+                count_back, epoch, pass, phase_out);
 }
 
 // Certification of one tile after the fill has converged.
 __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
-                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out, uint32_t epoch,
-                          int pass, int phase_out) {
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
+                          uint32_t* count_back, uint32_t epoch, int pass, int phase_out) {
     const PfRaster& a = b.R;
     const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
     if (dirty == 0) return;  // uniform
@@ -478,6 +497,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     // to an OPEN neighbour whose value is not lower (segmented propagation
     // over the lane masks).
     uint32_t fixed = 0, fixed_live = 0;
+    uint32_t nrow = 0;  // rows solved by this wave
     uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
@@ -489,6 +509,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
             const uint32_t bit = 1u << k;
             const bool op = (open & bit) != 0;
             if (__ballot(op) == 0) continue;  // no OPEN cell: nothing in the row can change
+            ++nrow;
             const int li = (ly + 1) * HALO_W + lx + 1;
             const float w = S.rel[li].x;
             const bool live = op && ((lm >> k) & 1u);
@@ -518,6 +539,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
         if (!__syncthreads_or(changed)) break;
     }
+    if ((threadIdx.x & 63) == 0 && nrow) atomicAdd(&S.steps, nrow);
 
     uint32_t bits = 0, nfixed = 0;
 #pragma unroll
@@ -531,7 +553,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
     }
     finish_tile(S, args, b, bits, open | open_inf, nfixed, fixed_live, cert_ctr, work_out,
-                count_out, epoch, pass, phase_out);
+                count_out, count_back, epoch, pass, phase_out);
 }
 
 // Runs the passes of phase `phase` (kind 0 fill, 1 certify) until a pass
@@ -550,16 +572,25 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
     for (int pass = 0; pass < a.max_passes; ++pass) {
         const int cout = (li + 1) % 3, cclr = (li + 2) % 3;
         const int lin = pass > 0 ? li : compacted ? 6 : 3 + pin;
-        const int cin = pass > 0 ? CTL_COUNT + li : compacted ? CTL_COMPACT : CTL_PHASE + pin;
-        const uint32_t count = ld(&a.ctrl[cin]);
+        const int cf = pass > 0 ? CTL_COUNT + li : compacted ? CTL_COMPACT : CTL_PHASE + pin;
+        const int cb = pass > 0 ? CTL_COUNTB + li : compacted ? CTL_COMPACTB : CTL_PHASEB + pin;
+        const uint32_t nfront = ld(&a.ctrl[cf]);
+        const uint32_t count = nfront + ld(&a.ctrl[cb]);
         if (count == 0) return li;
         ++epoch;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.ctrl[CTL_COUNT + cclr] = 0;
+            a.ctrl[CTL_COUNTB + cclr] = 0;
             a.ctrl[CTL_CLAIM + cout] = 0;  // the next pass's claims (this one's were cleared before)
-            if (pass == 0) a.ctrl[CTL_PHASE + (phase + 2) % 3] = 0;
+            if (pass == 0) {
+                a.ctrl[CTL_PHASE + (phase + 2) % 3] = 0;
+                a.ctrl[CTL_PHASEB + (phase + 2) % 3] = 0;
+            }
             // a certification follows the fill that last read the compacted list
-            if (pass == 0 && KIND == 1) a.ctrl[CTL_COMPACT] = 0;
+            if (pass == 0 && KIND == 1) {
+                a.ctrl[CTL_COMPACT] = 0;
+                a.ctrl[CTL_COMPACTB] = 0;
+            }
             atomicAdd(&a.ctrl[CTL_PASSES], 1u);
         }
         const int* win = a.work + (size_t)lin * a.total;
@@ -567,15 +598,15 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
         for (uint32_t i = blockIdx.x; i < count;) {
             uint32_t next = count;
             if (threadIdx.x == 0) next = claim_next(&a.ctrl[CTL_CLAIM + li], count);
-            const PfBlock b = pf_block<MULTI>(a, win[i], mn1, ok1);
+            const PfBlock b = pf_block<MULTI>(a, ranked_entry(win, a.total, nfront, i), mn1, ok1);
             if (b.ok) {  // uniform; blocks of rasters outside the domain never open
                 uint32_t* cert_ctr = &a.cert[3 * b.r + slot];
                 if (KIND == 0)
-                    fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch, pass,
-                              round, pout);
+                    fill_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout],
+                              &a.ctrl[CTL_COUNTB + cout], epoch, pass, round, pout);
                 else
-                    cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout], epoch,
-                              pass, pout);
+                    cert_tile(S, a, b, b.mn, cert_ctr, wout, &a.ctrl[CTL_COUNT + cout],
+                              &a.ctrl[CTL_COUNTB + cout], epoch, pass, pout);
             }
             if (threadIdx.x == 0) S.next = next;
             __syncthreads();
@@ -592,18 +623,21 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
 // (cleared by the certification's first pass; the caller ends with a grid
 // barrier).
 __device__ inline void compact_open(const PfArgs& a, int from) {
-    const uint32_t n = ld(&a.ctrl[CTL_PHASE + from]);
+    const uint32_t nf = ld(&a.ctrl[CTL_PHASE + from]);
+    const uint32_t n = nf + ld(&a.ctrl[CTL_PHASEB + from]);
     const int* src = a.work + (size_t)(3 + from) * a.total;
     int* dst = a.work + (size_t)6 * a.total;
     const uint32_t nw = (n + 63u) & ~63u;  // whole waves (the appends are wave-aggregated)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
         int g = 0;
-        bool keep = false;
+        bool keep = false, heavy = false;
         if (i < n) {
-            g = src[i];
+            g = ranked_entry(src, a.total, nf, i);
             keep = a.topen[g] != 0u;
+            heavy = a.tcost[g] >= PF_HEAVY;
         }
-        wave_append(keep, g, dst, &a.ctrl[CTL_COMPACT]);
+        wave_append(keep && heavy, g, dst, &a.ctrl[CTL_COMPACT]);
+        wave_append_back(keep && !heavy, g, dst, &a.ctrl[CTL_COMPACTB], a.total);
     }
 }
 
@@ -671,6 +705,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x) {
         a.work[(size_t)3 * a.total + g] = g;
         a.topen[g] = 1u;  // until its first solve
+        a.tcost[g] = 0u;
         reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
     }
     for (int k = threadIdx.x; k < 3 * a.nr; k += blockDim.x)
@@ -686,6 +721,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         a.ctrl[CTL_CLAIM + 1] = 0;
         a.ctrl[CTL_CLAIM + 2] = 0;
         a.ctrl[CTL_COMPACT] = 0;
+        a.ctrl[CTL_COUNTB + 0] = 0;
+        a.ctrl[CTL_COUNTB + 1] = 0;
+        a.ctrl[CTL_COUNTB + 2] = 0;
+        a.ctrl[CTL_PHASEB + 0] = 0;
+        a.ctrl[CTL_PHASEB + 1] = 0;
+        a.ctrl[CTL_PHASEB + 2] = 0;
+        a.ctrl[CTL_COMPACTB] = 0;
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
@@ -703,14 +745,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         ++phase;
         if (blockIdx.x == 0)
             for (int r = threadIdx.x; r < a.nr; r += blockDim.x) a.cert[3 * r + (round + 1) % 3] = 0;
-        if (ld(&a.ctrl[CTL_PHASE + phase % 3]) == 0) break;
+        if (ld(&a.ctrl[CTL_PHASE + phase % 3]) + ld(&a.ctrl[CTL_PHASEB + phase % 3]) == 0) break;
         li = run_passes<1, MULTI>(S, a, li, phase, slot, round, false, epoch, mn1, ok1);
         if (li < 0) break;
         ++phase;
-        if (ld(&a.ctrl[CTL_PHASE + phase % 3]) == 0) break;
+        if (ld(&a.ctrl[CTL_PHASE + phase % 3]) + ld(&a.ctrl[CTL_PHASEB + phase % 3]) == 0) break;
         compact_open(a, phase % 3);
         grid_sync(a.bar);
-        if (ld(&a.ctrl[CTL_COMPACT]) == 0) break;
+        if (ld(&a.ctrl[CTL_COMPACT]) + ld(&a.ctrl[CTL_COMPACTB]) == 0) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_EPOCH] = epoch;
@@ -771,6 +813,7 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.work = c->d_work3;
     a.mark = c->d_mark_pf;
     a.topen = c->d_topen;
+    a.tcost = c->d_tcost;
     a.rows = c->d_rows;
     a.ctrl = c->d_ctrl;
     a.cert = c->d_ctrl + CTL_CERT;
@@ -808,6 +851,7 @@ int pit_fill_multi(MultiState* m, hipStream_t s) {
     a.work = m->d_work;
     a.mark = m->d_mark_pf;
     a.topen = m->d_topen;
+    a.tcost = m->d_tcost;
     a.rows = m->d_rows;
     a.ctrl = m->d_ctrl;
     a.cert = m->d_cert;

@@ -285,18 +285,36 @@ __device__ inline unsigned long long neighbour_rows(int d, unsigned long long w6
 // the next pass (two words at rows + 4 t + 2 rpar) also gets the rows the
 // change unsettles; those words are read and cleared by the neighbour in
 // that pass, and the other parity is the one being read in this pass.
+// With `cost` set, the list is two-ended (see Ranked): a neighbour whose
+// last solve took at least `heavy` row steps goes to the front.
+struct Ranked {
+    // A list of `cap` entries filled from both ends: heavy tiles from the
+    // front (count *front), light ones from the back (count *back), so the
+    // entries handed out first, by workgroup number, are the heavy ones
+    // and a long solve overlaps the short ones instead of ending the pass.
+    const uint32_t* cost;
+    uint32_t heavy;
+    uint32_t* back;
+    int cap;
+};
+__device__ inline int ranked_entry(const int* lst, int cap, uint32_t nfront, uint32_t i) {
+    return i < nfront ? lst[i] : lst[cap - 1 - (int)(i - nfront)];
+}
 __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty, int gbase,
                                       int* __restrict__ work_out, uint32_t* count_out,
                                       uint32_t* __restrict__ mark, uint32_t epoch,
                                       uint32_t* rows = nullptr, int rpar = 0,
-                                      unsigned long long w64 = 0, unsigned long long e64 = 0) {
+                                      unsigned long long w64 = 0, unsigned long long e64 = 0,
+                                      const Ranked* rank = nullptr) {
     const int lane = threadIdx.x & 63;
-    bool push = false;
+    bool push = false, heavy = true;
     int t = 0;
     if (lane < 8 && (bits & (1u << lane))) {
         const int nx = tx + TDX[lane], ny = ty + TDY[lane];
         if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) {
             t = gbase + ny * ntx + nx;
+            uint32_t c = 0;
+            if (rank) c = rank->cost[t];  // in flight with the atomics below
             if (rows) {
                 const unsigned long long m = neighbour_rows(lane, w64, e64);
                 uint32_t* p = rows + 4 * (size_t)t + 2 * rpar;
@@ -304,14 +322,37 @@ __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, in
                 if ((uint32_t)(m >> 32)) atomicOr(p + 1, (uint32_t)(m >> 32));
             }
             push = atomicExch(&mark[t], epoch) != epoch;
+            if (rank) heavy = c >= rank->heavy;
         }
     }
-    const unsigned long long m = __ballot(push);
+    const unsigned long long mh = __ballot(push && heavy);
+    const unsigned long long ml = __ballot(push && !heavy);
+    if (mh) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(count_out, (uint32_t)__popcll(mh));
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+        if (push && heavy) work_out[base + lanes_below(mh)] = t;
+    }
+    if (ml) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(rank->back, (uint32_t)__popcll(ml));
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+        if (push && !heavy) work_out[rank->cap - 1 - (int)(base + lanes_below(ml))] = t;
+    }
+}
+
+// Appends `item` for the lanes with `take` to the back of a two-ended list of
+// `cap` entries (see Ranked): one atomic per wave.
+__device__ inline void wave_append_back(bool take, int item, int* __restrict__ out,
+                                        uint32_t* count, int cap) {
+    const unsigned long long m = __ballot(take);
     if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((long long)m) - 1;
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(count_out, (uint32_t)__popcll(m));
-    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-    if (push) work_out[base + lanes_below(m)] = t;
+    if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, first, 64);
+    if (take) out[cap - 1 - (int)(base + lanes_below(m))] = item;
 }
 
 // Appends `item` for the lanes with `take` to a list: one atomic per wave.
