@@ -11,16 +11,6 @@ namespace cageo {
 
 namespace {
 
-__global__ void k_iota(int* __restrict__ out, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[i] = i;
-}
-
-__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int count) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
 // min <- max ordered value, max and "bad" <- 0.  A kernel rather than memsets
 // so that a captured graph holds no memset nodes.
 __global__ void k_reset_minmax(uint32_t* __restrict__ scal) {
@@ -53,7 +43,7 @@ int alloc(T** p, size_t count) {
 }
 
 void free_all(Context* c) {
-    void* ptrs[] = {c->d_work[0], c->d_work[1], c->d_mark, c->d_mark_pf, c->d_topen, c->d_tcost, c->d_rows, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_filter,
+    void* ptrs[] = {c->d_mark, c->d_mark_pf, c->d_topen, c->d_tcost, c->d_rows, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_filter,
                     c->d_sflags, c->d_slabel, c->d_off, c->d_factor, c->d_a, c->d_b, c->d_z,
                     c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr,
                     c->d_lr_col, c->d_lr_row, c->d_lr_planes};
@@ -101,20 +91,6 @@ int realization_body(Context* c, float* mc, hipEvent_t* evs) {
     return CAGEO_OK;
 }
 
-
-int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, int count) {
-    const int grid = std::max(1, std::min((count + 255) / 256, 1024));
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, c->stream, p, value, count);
-    CAGEO_HIP_TRY(hipGetLastError());
-    return CAGEO_OK;
-}
-
-int worklist_all_tiles(Context* c, int buf) {
-    hipLaunchKernelGGL(k_iota, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
-                       c->d_work[buf], c->ntiles);
-    CAGEO_HIP_TRY(hipGetLastError());
-    return CAGEO_OK;
-}
 
 int read_scalars(Context* c) {
     CAGEO_HIP_TRY(hipMemcpyAsync(c->h_scal, c->d_scal, SC_NSLOTS * sizeof(uint32_t),
@@ -176,13 +152,6 @@ int prepare_low_rank(Context* c) {
     return CAGEO_OK;
 }
 
-uint32_t next_epoch(Context* c) {
-    if (++c->epoch == 0) {
-        (void)hipMemsetAsync(c->d_mark, 0, (size_t)c->ntiles * sizeof(uint32_t), c->stream);
-        c->epoch = 1;
-    }
-    return c->epoch;
-}
 
 }  // namespace cageo
 
@@ -225,8 +194,6 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
     auto A = [&](int s) {
         if (st == CAGEO_OK) st = s;
     };
-    A(cageo::alloc(&c->d_work[0], c->ntiles));
-    A(cageo::alloc(&c->d_work[1], c->ntiles));
     A(cageo::alloc(&c->d_mark, c->ntiles));
     A(cageo::alloc(&c->d_mark_pf, c->ntiles));
     A(cageo::alloc(&c->d_rows, 4 * (size_t)c->ntiles));

@@ -50,8 +50,7 @@ struct Context {
 
     // tiles
     int ntx = 0, nty = 0, ntiles = 0;
-    int* d_work[2] = {nullptr, nullptr};
-    uint32_t* d_mark = nullptr;  // per-tile epoch stamp for worklist dedupe
+    uint32_t* d_mark = nullptr;  // per-tile epoch stamps of the flat-resolution worklists
     uint32_t* d_mark_pf = nullptr;  // epoch stamps of the pit-fill kernel
     uint32_t* d_topen = nullptr;    // per tile: holds OPEN pit-fill cells
     uint32_t* d_tcost = nullptr;    // per tile: row steps of its last pit-fill solve
@@ -60,7 +59,6 @@ struct Context {
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
     uint32_t* d_bar = nullptr;      // grid barrier words (BAR_WORDS), see grid_sync
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
-    uint32_t epoch = 0;
     int conv_ty = 0;  // tuned row-block height of the 2-D convolution (0 = largest that fits)
     // Fast mode of the 2-D convolution: the filter's low-rank separable form.
     int conv_fast = 0;
@@ -269,9 +267,7 @@ struct MultiState {
 };
 
 // Worklist helpers --------------------------------------------------------
-int worklist_all_tiles(Context* c, int buf);
 int read_scalars(Context* c);
-uint32_t next_epoch(Context* c);
 
 // Stage drivers implemented in the stage translation units -----------------
 int launch_random_field(Context* c, float* out, uint64_t seed, uint32_t realization,
@@ -286,7 +282,6 @@ int lr_cols_geometry(int r, int* LC, size_t* lds);
 int lr_rows_geometry(int r, int* LP, int* S, size_t* lds);
 int prepare_low_rank(Context* c);
 constexpr size_t LDS_LIMIT = 160 * 1024;
-int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, int count);
 int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes);
 int launch_realistic(Context* c, const float* dem, float* field, float nodata);
 int launch_burn(Context* c, const float* zin, float* zout, const uint8_t* sflags,

@@ -116,17 +116,6 @@ __device__ inline void scan_clamp_down(float& lo, float& hi) {
     CAGEO_SCAN_DOWN(dpp_f, lane_read, float, lo, hi, NI, PI, clamp_after);
 }
 
-// Row flags of the sweeps: rows ly-1 and ly+1, and ly itself when `self`,
-// need another solve.  Plain stores of 1 (the flags only ever go 0 -> 1
-// between the owner's read-and-clear steps).
-__device__ inline void mark_rows(uint32_t* rd, int ly, bool self) {
-    if ((threadIdx.x & 63) == 0) {
-        if (ly > 0) rd[ly - 1] = 1u;
-        if (ly < TILE - 1) rd[ly + 1] = 1u;
-        if (self) rd[ly] = 1u;
-    }
-}
-
 // Rows due for a row solve.  Each wave keeps the rows of its own strip in
 // one wave-uniform word (bit k = row k of the strip): the low half holds
 // rows reached by a change below them (an upward sweep carries it on), the
@@ -277,21 +266,12 @@ __device__ inline unsigned long long neighbour_rows(int d, unsigned long long w6
     return m | (m << 1) | (m >> 1);
 }
 
-// Lists the neighbour tiles of tile (tx, ty) named by `bits` for the next
-// pass, each at most once per epoch.  Called by all 64 lanes of one wave:
-// lanes 0..7 take one direction each (their mark exchanges are in flight
-// together), and one atomic reserves the list slots of the whole wave.
-// With `rows` set, each neighbour's dirty-row mask for
-// the next pass (two words at rows + 4 t + 2 rpar) also gets the rows the
-// change unsettles; those words are read and cleared by the neighbour in
-// that pass, and the other parity is the one being read in this pass.
-// With `cost` set, the list is two-ended (see Ranked): a neighbour whose
-// last solve took at least `heavy` row steps goes to the front.
 struct Ranked {
     // A list of `cap` entries filled from both ends: heavy tiles from the
-    // front (count *front), light ones from the back (count *back), so the
-    // entries handed out first, by workgroup number, are the heavy ones
-    // and a long solve overlaps the short ones instead of ending the pass.
+    // front (the list's own count word), light ones from the back (count
+    // *back), so the entries handed out first, by workgroup number, are the
+    // heavy ones and a long solve overlaps the short ones instead of ending
+    // the pass.
     const uint32_t* cost;
     uint32_t heavy;
     uint32_t* back;
@@ -300,6 +280,16 @@ struct Ranked {
 __device__ inline int ranked_entry(const int* lst, int cap, uint32_t nfront, uint32_t i) {
     return i < nfront ? lst[i] : lst[cap - 1 - (int)(i - nfront)];
 }
+// Lists the neighbour tiles of tile (tx, ty) named by `bits` for the next
+// pass, each at most once per epoch.  Called by all 64 lanes of one wave:
+// lanes 0..7 take one direction each (their mark exchanges are in flight
+// together), and one atomic reserves the list slots of the whole wave.
+// With `rows` set, each neighbour's dirty-row mask for
+// the next pass (two words at rows + 4 t + 2 rpar) also gets the rows the
+// change unsettles; those words are read and cleared by the neighbour in
+// that pass, and the other parity is the one being read in this pass.
+// With `rank` set, the list is two-ended (see Ranked): a neighbour whose
+// last solve took at least `rank->heavy` row steps goes to the front.
 __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty, int gbase,
                                       int* __restrict__ work_out, uint32_t* count_out,
                                       uint32_t* __restrict__ mark, uint32_t epoch,
@@ -367,37 +357,5 @@ __device__ inline void wave_append(bool take, int item, int* __restrict__ out, u
     if (take) out[base + lanes_below(m)] = item;
 }
 
-// Block-wide OR of per-thread border bits; returns the result in all threads.
-__device__ inline uint32_t block_or_bits(uint32_t bits) {
-    __shared__ uint32_t s_bits;
-    if (threadIdx.x == 0) s_bits = 0;
-    __syncthreads();
-    for (int off = 32; off > 0; off >>= 1) bits |= (uint32_t)__shfl_xor((int)bits, off);
-    if ((threadIdx.x & 63) == 0 && bits) atomicOr(&s_bits, bits);
-    __syncthreads();
-    return s_bits;
-}
-
-// Host side: run passes until no tile is listed.  `launch(work_in, count,
-// work_out, count_slot, epoch)` enqueues one pass.  `count` tiles are in
-// d_work[buf] on entry.
-template <class Launch>
-int run_tile_passes(Context* c, int buf, uint32_t count, int max_passes, Launch launch,
-                    int* passes_out) {
-    int passes = 0;
-    while (count > 0) {
-        if (passes >= max_passes) return CAGEO_ERR_NOT_CONVERGED;
-        const int out = 1 - buf;
-        const uint32_t epoch = next_epoch(c);
-        CAGEO_HIP_TRY(hipMemsetAsync(c->d_scal + SC_COUNT0 + out, 0, sizeof(uint32_t), c->stream));
-        CAGEO_TRY(launch(c->d_work[buf], count, c->d_work[out], c->d_scal + SC_COUNT0 + out, epoch));
-        CAGEO_TRY(read_scalars(c));
-        count = c->h_scal[SC_COUNT0 + out];
-        buf = out;
-        ++passes;
-    }
-    if (passes_out) *passes_out += passes;
-    return CAGEO_OK;
-}
 
 }  // namespace cageo
