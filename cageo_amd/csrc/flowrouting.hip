@@ -366,15 +366,23 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
             const int32_t d = dv[k];
             if (y >= R.H || d < 1 || d >= D_INF) continue;
             const size_t i = (size_t)y * R.W + x;
+            // the cell's loads at once: no load waits on a test of another
+            // (a neighbour outside the grid reads the cell itself, distance d)
             const float zc = R.z[i];
-            int sk = -1;
+            int32_t nd[8];
+            float nz[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const int nx = x + FDX[q], ny = y + FDY[q];
-                if (nx < 0 || ny < 0 || nx >= R.W || ny >= R.H) continue;
-                const size_t ni = (size_t)ny * R.W + nx;
-                if (R.dist[ni] == d - 1 && R.z[ni] <= zc) sk = q;
+                const bool in = nx >= 0 && ny >= 0 && nx < R.W && ny < R.H;
+                const size_t ni = in ? (size_t)ny * R.W + nx : i;
+                nd[q] = R.dist[ni];
+                nz[q] = R.z[ni];
             }
+            int sk = -1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (nd[q] == d - 1 && nz[q] <= zc) sk = q;
             if (sk >= 0)
                 reinterpret_cast<char2*>(R.fd)[i] = make_char2((int8_t)FDX[sk], (int8_t)FDY[sk]);
         }
