@@ -109,8 +109,12 @@ __device__ inline uint32_t ld(const uint32_t* p) {
 
 // D8 of one cell (FlowRoutingAlgorithmCPUOpenMP.h:60-150); returns the
 // initial distance: 0 has a direction, D_STREAM, or D_INF (flat).
-__device__ inline int32_t d8_cell(const float* __restrict__ z, int8_t* __restrict__ fd, int x,
-                                  int y, int W, int H) {
+// D8 of cell (x, y) from the block's halo tile of elevations `zt` (HALO_W
+// wide; (lx, ly) the cell's place in the block).  Edge cells point out of
+// the grid and read no neighbour, so the halo's cells outside the grid are
+// never used.
+__device__ inline int32_t d8_cell(const float* zt, int8_t* __restrict__ fd, int x, int y, int lx,
+                                  int ly, int W, int H) {
     const size_t i = (size_t)y * W + x;
     int dxi = 0, dyi = 0;
     if (x == 0) dxi = -1;
@@ -124,7 +128,8 @@ __device__ inline int32_t d8_cell(const float* __restrict__ z, int8_t* __restric
         oy = (int8_t)dyi;
         d = 0;
     } else {
-        const float zc = z[i];
+        const int c = (ly + 1) * HALO_W + lx + 1;
+        const float zc = zt[c];
         if (zc == 0.0f) {
             d = D_STREAM;
         } else {
@@ -132,7 +137,7 @@ __device__ inline int32_t d8_cell(const float* __restrict__ z, int8_t* __restric
             int sk = -1;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                float s = zc - z[(size_t)(y + FDY[k]) * W + (x + FDX[k])];
+                float s = zc - zt[c + FDY[k] * HALO_W + FDX[k]];
                 if (s > 0.0f) {
                     if (k & 1) s = s / 1.41421356237309505f;
                     if (s > steepest) {
@@ -297,19 +302,43 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
     }
     grid_sync(a.bar);
 
-    // D8 block by block; blocks holding flats open list 0.
+    // D8 block by block, from the block's halo tile of elevations staged in
+    // LDS (one batch of loads); blocks holding flats open list 0.
+    float* zt = reinterpret_cast<float*>(S.dz);  // HALO_W x HALO_W floats
     for (int g = blockIdx.x; g < a.total; g += gridDim.x) {
         const FrBlock b = fr_block<MULTI>(a, g);
         const FrRaster& R = b.R;
         const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
-        const int x = x0 + (threadIdx.x & 63);
+        {
+            constexpr int N = HALO_W * HALO_W;
+            constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
+            float v[ITER];
+#pragma unroll
+            for (int j = 0; j < ITER; ++j) {
+                const int k = threadIdx.x + j * TILE_THREADS;
+                const int hy = k / HALO_W, hx = k - hy * HALO_W;
+                const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+                v[j] = 0.0f;
+                if (k < N && gx >= 0 && gy >= 0 && gx < R.W && gy < R.H)
+                    v[j] = R.z[(size_t)gy * R.W + gx];
+            }
+#pragma unroll
+            for (int j = 0; j < ITER; ++j) {
+                const int k = threadIdx.x + j * TILE_THREADS;
+                if (k < N) zt[k] = v[j];
+            }
+        }
+        __syncthreads();
+        const int lx = threadIdx.x & 63;
+        const int x = x0 + lx;
         int has_flat = 0;
         if (x < R.W) {
 #pragma unroll
-            for (int k = 0; k < STRIP; ++k) {  // no early exit: the rows' loads overlap
-                const int y = y0 + (threadIdx.x >> 6) * STRIP + k;
+            for (int k = 0; k < STRIP; ++k) {
+                const int ly = (threadIdx.x >> 6) * STRIP + k;
+                const int y = y0 + ly;
                 if (y < R.H) {
-                    const int32_t d = d8_cell(R.z, R.fd, x, y, R.W, R.H);
+                    const int32_t d = d8_cell(zt, R.fd, x, y, lx, ly, R.W, R.H);
                     R.dist[(size_t)y * R.W + x] = d;
                     has_flat |= d == D_INF;
                 }
