@@ -59,37 +59,66 @@ __device__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
 
+    // Two batches of rows; in each, the cells' flags and directions are
+    // loaded together, then their targets' flags and labels (no load waits
+    // on a test; a target outside the grid reads the cell itself, unused).
     uint32_t local = 0;
+    constexpr int HALF = STRIP / 2;
 #pragma unroll
-    for (int k = 0; k < STRIP; ++k) {
-        const int ly = strip * STRIP + k;
-        const int gy = y0 + ly;
-        uint64_t e = pack(K_STUCK, 0, 0);
-        if (gx < W && gy < H) {
-            const size_t i = (size_t)gy * W + gx;
-            if (sf[i] & SF_TRACE) {
-                e = pack(K_RESOLVED, 0, CAGEO_FLOWTRACE_LAKE);
-            } else {
-                const char2 d = reinterpret_cast<const char2*>(fd)[i];
-                const int nx = gx + d.x, ny = gy + d.y;
-                if (nx < 0 || ny < 0 || nx >= W || ny >= H) {
-                    e = pack(K_RESOLVED, 1, CAGEO_FLOWTRACE_OUTSIDE);
-                } else {
-                    const size_t ni = (size_t)ny * W + nx;
-                    if (sf[ni] & SF_TRACE) {
-                        e = pack(K_RESOLVED, 1, slabel[ni]);
-                    } else if (d.x == 0 && d.y == 0) {
-                        e = pack(K_STUCK, 0, 0);
-                    } else if (nx >= x0 && ny >= y0 && nx < x0 + TILE && ny < y0 + TILE) {
-                        e = pack(K_LOCAL, 1, (ny - y0) * TILE + (nx - x0));
-                        local |= 1u << k;
-                    } else {
-                        e = pack(K_EXIT, 1, (int32_t)ni);
-                    }
-                }
+    for (int k0 = 0; k0 < STRIP; k0 += HALF) {
+        uint8_t sfc[HALF], sfn[HALF];
+        char2 dv[HALF];
+        int32_t sln[HALF];
+#pragma unroll
+        for (int j = 0; j < HALF; ++j) {
+            const int gy = y0 + strip * STRIP + k0 + j;
+            sfc[j] = 0;
+            dv[j] = make_char2(0, 0);
+            if (gx < W && gy < H) {
+                const size_t i = (size_t)gy * W + gx;
+                sfc[j] = sf[i];
+                dv[j] = reinterpret_cast<const char2*>(fd)[i];
             }
         }
-        L[ly * TILE + lx] = e;
+#pragma unroll
+        for (int j = 0; j < HALF; ++j) {
+            const int gy = y0 + strip * STRIP + k0 + j;
+            const int nx = gx + dv[j].x, ny = gy + dv[j].y;
+            const bool in = gx < W && gy < H && nx >= 0 && ny >= 0 && nx < W && ny < H;
+            const size_t ni = in ? (size_t)ny * W + nx : 0;
+            sfn[j] = 0;
+            sln[j] = 0;
+            if (in) {
+                sfn[j] = sf[ni];
+                sln[j] = slabel[ni];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < HALF; ++j) {
+            const int k = k0 + j;
+            const int ly = strip * STRIP + k;
+            const int gy = y0 + ly;
+            uint64_t e = pack(K_STUCK, 0, 0);
+            if (gx < W && gy < H) {
+                const char2 d = dv[j];
+                const int nx = gx + d.x, ny = gy + d.y;
+                if (sfc[j] & SF_TRACE) {
+                    e = pack(K_RESOLVED, 0, CAGEO_FLOWTRACE_LAKE);
+                } else if (nx < 0 || ny < 0 || nx >= W || ny >= H) {
+                    e = pack(K_RESOLVED, 1, CAGEO_FLOWTRACE_OUTSIDE);
+                } else if (sfn[j] & SF_TRACE) {
+                    e = pack(K_RESOLVED, 1, sln[j]);
+                } else if (d.x == 0 && d.y == 0) {
+                    e = pack(K_STUCK, 0, 0);
+                } else if (nx >= x0 && ny >= y0 && nx < x0 + TILE && ny < y0 + TILE) {
+                    e = pack(K_LOCAL, 1, (ny - y0) * TILE + (nx - x0));
+                    local |= 1u << k;
+                } else {
+                    e = pack(K_EXIT, 1, (int32_t)((size_t)ny * W + nx));
+                }
+            }
+            L[ly * TILE + lx] = e;
+        }
     }
     __syncthreads();
 
