@@ -33,9 +33,19 @@ __global__ __launch_bounds__(256) void k_conv_cols(float* __restrict__ out,
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int rows = TILE + 2 * r;
     const int x = x0 + tx;
-    for (int k = ty; k < rows; k += 4) {
-        int gy = wrap_index(y0 - r + k, H);
-        lds[k * TILE + tx] = (x < W) ? in[(size_t)gy * W + x] : 0.0f;
+    // batches of independent loads (each store waits only for its batch)
+    constexpr int B = 8;
+    for (int k0 = ty; k0 < rows; k0 += 4 * B) {
+        float v[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int k = k0 + 4 * j;
+            v[j] = 0.0f;
+            if (k < rows && x < W) v[j] = in[(size_t)wrap_index(y0 - r + k, H) * W + x];
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if (k0 + 4 * j < rows) lds[(k0 + 4 * j) * TILE + tx] = v[j];
     }
     __syncthreads();
     float acc[STRIP];
@@ -70,11 +80,22 @@ __global__ __launch_bounds__(256) void k_conv_rows(float* __restrict__ out,
     const int S = TILE + 2 * r;
     const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int idx = threadIdx.x; idx < TILE * S; idx += 256) {
-        int row = idx / S, k = idx - row * S;
-        int gy = y0 + row;
-        int gx = wrap_index(x0 - r + k, W);
-        lds[idx] = (gy < H) ? in[(size_t)gy * W + gx] : 0.0f;
+    constexpr int B = 8;  // batches of independent loads
+    for (int i0 = threadIdx.x; i0 < TILE * S; i0 += 256 * B) {
+        float v[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int idx = i0 + 256 * j;
+            v[j] = 0.0f;
+            if (idx < TILE * S) {
+                const int row = idx / S, k = idx - row * S;
+                const int gy = y0 + row;
+                if (gy < H) v[j] = in[(size_t)gy * W + wrap_index(x0 - r + k, W)];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if (i0 + 256 * j < TILE * S) lds[i0 + 256 * j] = v[j];
     }
     __syncthreads();
     float acc[STRIP];
