@@ -70,7 +70,11 @@ __global__ __launch_bounds__(256) void k_conv_cols(float* __restrict__ out,
     }
 }
 
-// Rows pass: out[y][x] = sum_fx in[y][wrap(x - r + fx)] * f[fx].
+// Rows pass: out[y][x] = sum_fx in[y][wrap(x - r + fx)] * f[fx].  The grid
+// is transposed (blockIdx.x walks tile rows): blocks go round robin over the
+// 8 XCDs, so the left and right neighbours of a tile, whose halos overlap it,
+// are gridDim.x blocks away, on the same XCD and its L2 when gridDim.x is a
+// multiple of 8.
 __global__ __launch_bounds__(256) void k_conv_rows(float* __restrict__ out,
                                                    const float* __restrict__ in,
                                                    const float* __restrict__ f, int r, int W,
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(256) void k_conv_rows(float* __restrict__ out,
     extern __shared__ float lds[];  // 64 rows x (64 + 2r)
     const int L = 2 * r + 1;
     const int S = TILE + 2 * r;
-    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
+    const int x0 = blockIdx.y * TILE, y0 = blockIdx.x * TILE;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     constexpr int B = 8;  // batches of independent loads
     for (int i0 = threadIdx.x; i0 < TILE * S; i0 += 256 * B) {
@@ -239,7 +243,7 @@ int launch_convolve_to(Context* c, float* field, float* buf, float** result) {
         hipLaunchKernelGGL(k_conv_cols, grid, dim3(256), lds, c->stream, buf, field,
                            c->d_filter, r, W, H);
         CAGEO_HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_conv_rows, grid, dim3(256), lds, c->stream, field, buf,
+        hipLaunchKernelGGL(k_conv_rows, dim3(grid.y, grid.x), dim3(256), lds, c->stream, field, buf,
                            c->d_filter, r, W, H);
         CAGEO_HIP_TRY(hipGetLastError());
         *result = field;
