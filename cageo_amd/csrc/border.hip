@@ -17,8 +17,15 @@ __device__ constexpr int BDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 __global__ __launch_bounds__(256) void k_border(float* __restrict__ mc,
                                                 const int32_t* __restrict__ lab, int W,
                                                 int H) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    // Blocks go round robin over the 8 XCDs; renumbering them so that each
+    // XCD takes a contiguous band of tiles keeps a tile's neighbours, whose
+    // labels it reads at its edges, in the same L2 (a bijection on the
+    // block numbers when their count is a multiple of 8).
+    const unsigned nb = gridDim.x * gridDim.y;
+    unsigned t = blockIdx.y * gridDim.x + blockIdx.x;
+    if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
+    const int x = (int)(t % gridDim.x) * 64 + (threadIdx.x & 63);
+    const int y = (int)(t / gridDim.x) * 4 + (threadIdx.x >> 6);
     if (x >= W || y >= H) return;
     const size_t i = (size_t)y * W + x;
     const int32_t c = lab[i];
