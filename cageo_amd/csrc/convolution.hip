@@ -124,17 +124,56 @@ __global__ __launch_bounds__(256) void k_conv_rows(float* __restrict__ out,
     }
 }
 
+// One step of the 2-D pass: loads the row segments of step b + 2 into `fut`
+// and the taps of step b + 1, then adds the four taps `t` of step b times
+// the window (cur, nxt) into the 4 x 4 outputs (row j, column k).  On
+// return `t` holds the taps of step b + 1.
+__device__ __forceinline__ void conv_2d_step(float (&acc)[4][4], const float4 (&cur)[4],
+                                             const float4 (&nxt)[4], float4 (&fut)[4],
+                                             float4& t, const float* row0, int S,
+                                             const float* frow, int b) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        fut[j] = *reinterpret_cast<const float4*>(row0 + j * S + 4 * b + 8);
+    const float4 t2 = *reinterpret_cast<const float4*>(frow + 4 * b + 4);
+    // keeps those loads issued ahead of the products: the scheduler would
+    // otherwise sink them below to shorten their live ranges
+    __builtin_amdgcn_sched_barrier(0);
+    const float tap[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float win[8] = {cur[j].x, cur[j].y, cur[j].z, cur[j].w,
+                              nxt[j].x, nxt[j].y, nxt[j].z, nxt[j].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float prod = win[k + i] * tap[i];
+                acc[j][k] = acc[j][k] + prod;
+            }
+    }
+    t = t2;
+}
+
 // Full 2-D pass (exponential model).  Each thread owns a 4 x 4 block of
 // outputs; for every filter row the 4 x 4 outputs slide over one LDS row
 // segment read as float4, with the taps of that row held in scalar
 // registers.  The filter is stored with a row stride `fs` (multiple of 4,
-// zero padded) so a tap quadruple is one aligned scalar load.
-template <int TY>
+// zero padded) so a tap quadruple is one aligned load.
+//
+// TAPS_LDS: the filter is copied into LDS behind the tile.  Scalar loads
+// return out of order, so a wave that takes its taps through them waits
+// for every outstanding load before each use; LDS reads return in order,
+// which lets the next step's row segment and taps be in flight while the
+// current step computes (a CU holds one wave per SIMD at large radii, so
+// nothing else hides that latency).  Without it (filters too large for
+// LDS) the taps come through scalar loads.
+template <int TY, bool TAPS_LDS>
 __global__ __launch_bounds__(16 * TY) void k_conv_2d(float* __restrict__ out,
                                                      const float* __restrict__ in,
                                                      const float* __restrict__ f, int r, int fs,
                                                      int S, int W, int H) {
-    extern __shared__ float lds[];  // (4*TY + 2r) rows x S
+    extern __shared__ float lds[];  // (4*TY + 2r) rows x S, then L x fs taps
     constexpr int NT = 16 * TY;
     constexpr int OUT_H = 4 * TY;
     const int L = 2 * r + 1;
@@ -151,6 +190,12 @@ __global__ __launch_bounds__(16 * TY) void k_conv_2d(float* __restrict__ out,
         }
         lds[k] = v;
     }
+    const float* taps = f;
+    if (TAPS_LDS) {
+        float* fl = lds + rows * S;
+        for (int k = threadIdx.y * 16 + threadIdx.x; k < L * fs; k += NT) fl[k] = f[k];
+        taps = fl;
+    }
     __syncthreads();
 
     const int tx4 = threadIdx.x, ty4 = threadIdx.y;
@@ -162,31 +207,42 @@ __global__ __launch_bounds__(16 * TY) void k_conv_2d(float* __restrict__ out,
 
     const int nb = L >> 2;      // full tap quadruples
     const int rem = L & 3;      // 1 or 3 (L is odd)
+    // Each step takes one tap quadruple for the four output rows (128
+    // products and sums) while the next step's row segments and taps load.
+    // Per accumulator the order is filter row, then ascending taps.
     for (int fy = 0; fy < L; ++fy) {
-        const float* frow = f + (size_t)fy * fs;
+        const float* frow = taps + (size_t)fy * fs;
+        const float* row0 = lds + (ty4 * 4 + fy) * S + tx4 * 4;
+        float4 cur[4], nxt[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float* row = lds + (ty4 * 4 + j + fy) * S + tx4 * 4;
-            float4 cur = *reinterpret_cast<const float4*>(row);
-            for (int b = 0; b < nb; ++b) {
-                float4 nxt = *reinterpret_cast<const float4*>(row + 4 * b + 4);
-                float4 t = *reinterpret_cast<const float4*>(frow + 4 * b);
-                float win[8] = {cur.x, cur.y, cur.z, cur.w, nxt.x, nxt.y, nxt.z, nxt.w};
-                float tap[4] = {t.x, t.y, t.z, t.w};
+            cur[j] = *reinterpret_cast<const float4*>(row0 + j * S);
+            nxt[j] = *reinterpret_cast<const float4*>(row0 + j * S + 4);
+        }
+        float4 t = *reinterpret_cast<const float4*>(frow);
+        float4 fut[4];
+        // three steps per trip: the roles of the three row-segment sets
+        // rotate with period 3, so the rotation costs no register moves
+        int b = 0;
+        for (; b + 3 <= nb; b += 3) {
+            conv_2d_step(acc, cur, nxt, fut, t, row0, S, frow, b);
+            conv_2d_step(acc, nxt, fut, cur, t, row0, S, frow, b + 1);
+            conv_2d_step(acc, fut, cur, nxt, t, row0, S, frow, b + 2);
+        }
+        for (; b < nb; ++b) {
+            conv_2d_step(acc, cur, nxt, fut, t, row0, S, frow, b);
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        float prod = win[k + i] * tap[i];
-                        acc[j][k] = acc[j][k] + prod;
-                    }
-                cur = nxt;
+            for (int j = 0; j < 4; ++j) {
+                cur[j] = nxt[j];
+                nxt[j] = fut[j];
             }
-            if (rem) {
-                float4 nxt = *reinterpret_cast<const float4*>(row + 4 * nb + 4);
-                float4 t = *reinterpret_cast<const float4*>(frow + 4 * nb);
-                float win[8] = {cur.x, cur.y, cur.z, cur.w, nxt.x, nxt.y, nxt.z, nxt.w};
-                float tap[4] = {t.x, t.y, t.z, t.w};
+        }
+        if (rem) {
+            const float tap[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float win[8] = {cur[j].x, cur[j].y, cur[j].z, cur[j].w,
+                                      nxt[j].x, nxt[j].y, nxt[j].z, nxt[j].w};
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     if (i < rem) {
@@ -266,21 +322,29 @@ int launch_convolve_to(Context* c, float* field, float* buf, float** result) {
         }
     }
     const int fs = (2 * r + 1 + 3 + 3) & ~3;
+    // the taps go into LDS behind the tile when both fit
+    const size_t taps_bytes = (size_t)(2 * r + 1) * fs * sizeof(float);
+    const bool taps_lds = lds + taps_bytes <= LDS_LIMIT;
+    if (taps_lds) lds += taps_bytes;
     dim3 g2((W + TILE - 1) / TILE, (H + 4 * ty - 1) / (4 * ty));
+#define CAGEO_CONV_2D(TY_, TL_)                                                              \
+    hipLaunchKernelGGL((k_conv_2d<TY_, TL_>), g2, dim3(16, TY_), lds, c->stream, buf, field, \
+                       c->d_filter, r, fs, S, W, H)
     switch (ty) {
         case 16:
-            hipLaunchKernelGGL(k_conv_2d<16>, g2, dim3(16, 16), lds, c->stream, buf, field,
-                               c->d_filter, r, fs, S, W, H);
+            if (taps_lds) CAGEO_CONV_2D(16, true);
+            else CAGEO_CONV_2D(16, false);
             break;
         case 8:
-            hipLaunchKernelGGL(k_conv_2d<8>, g2, dim3(16, 8), lds, c->stream, buf, field,
-                               c->d_filter, r, fs, S, W, H);
+            if (taps_lds) CAGEO_CONV_2D(8, true);
+            else CAGEO_CONV_2D(8, false);
             break;
         default:
-            hipLaunchKernelGGL(k_conv_2d<4>, g2, dim3(16, 4), lds, c->stream, buf, field,
-                               c->d_filter, r, fs, S, W, H);
+            if (taps_lds) CAGEO_CONV_2D(4, true);
+            else CAGEO_CONV_2D(4, false);
             break;
     }
+#undef CAGEO_CONV_2D
     CAGEO_HIP_TRY(hipGetLastError());
     *result = buf;
     return CAGEO_OK;

@@ -47,8 +47,10 @@ def test_random_field_bit_exact(ca, torch_cuda, shape):
 
 
 # ----------------------------------------------------------------- convolution
+# PR 260 m (r = 65): the exponential taps do not fit in LDS beside the tile
 @pytest.mark.parametrize("shape,pr", [((1024, 1024), 60), ((200, 300), 60), ((31, 40), 60),
-                                      ((96, 130), 200), ((256, 256), 200), ((17, 17), 30)])
+                                      ((96, 130), 200), ((256, 256), 200), ((17, 17), 30),
+                                      ((150, 160), 260)])
 def test_convolution_bit_exact(ca, torch_cuda, shape, pr):
     torch = torch_cuda
     h, w = shape
