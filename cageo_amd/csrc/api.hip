@@ -129,7 +129,9 @@ int prepare_low_rank(Context* c) {
     double resid = 0.0;
     CAGEO_TRY(cageo_filter_low_rank(c->h_taps.data(), r, KMAX, 1e-6, vec.data(), val.data(), &K,
                                     &resid));
-    std::vector<float> col((size_t)K * LC, 0.0f), row((size_t)K * LP, 0.0f);
+    // row taps carry 4 zero floats after the last term: the row pass reads
+    // one tap quadruple ahead
+    std::vector<float> col((size_t)K * LC, 0.0f), row((size_t)K * LP + 4, 0.0f);
     for (int k = 0; k < K; ++k)
         for (int i = 0; i < L; ++i) {
             col[(size_t)k * LC + i] = vec[(size_t)k * L + i];

@@ -94,15 +94,57 @@ __global__ __launch_bounds__(256) void k_lr_cols(float* __restrict__ planes,
     }
 }
 
-// rtaps: K rows of LP = (L + 3) & ~3 taps (lambda_k v_k, zero padded).
+// One 4-tap step of the row pass: loads the row segments of step q + 2 into
+// `fut` and the taps of step q + 1, then adds the taps `t` of step q times
+// the window (cur, nxt) into the 4 x 4 outputs (row a, column c).  On
+// return `t` holds the taps of step q + 1.
+__device__ __forceinline__ void lr_row_step(float (&acc)[4][4], const float4 (&cur)[4],
+                                            const float4 (&nxt)[4], float4 (&fut)[4], float4& t,
+                                            const float* row0, int RS, const float* T, int q) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        fut[a] = *reinterpret_cast<const float4*>(row0 + a * RS + 4 * q + 8);
+    const float4 t2 = *reinterpret_cast<const float4*>(T + 4 * q + 4);
+    // keeps those loads issued ahead of the products: the scheduler would
+    // otherwise sink them below to shorten their live ranges
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const float win[8] = {cur[a].x, cur[a].y, cur[a].z, cur[a].w,
+                              nxt[a].x, nxt[a].y, nxt[a].z, nxt[a].w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float s = acc[a][c];
+            s = __builtin_fmaf(t.x, win[c], s);
+            s = __builtin_fmaf(t.y, win[c + 1], s);
+            s = __builtin_fmaf(t.z, win[c + 2], s);
+            s = __builtin_fmaf(t.w, win[c + 3], s);
+            acc[a][c] = s;
+        }
+    }
+    t = t2;
+}
+
+// rtaps: K rows of LP = (L + 3) & ~3 taps (lambda_k v_k, zero padded), then
+// 4 zero floats.  TAPS_LDS: the taps are copied into LDS behind the tile, so
+// that every operand load returns in order and the next step's loads stay
+// in flight during the current step's products (scalar loads return out of
+// order: a wave waits for all of them at each use).
+template <bool TAPS_LDS>
 __global__ __launch_bounds__(256) void k_lr_rows(float* __restrict__ out,
                                                  const float* __restrict__ planes,
                                                  const float* __restrict__ rtaps, int K, int r,
                                                  int LP, int S, int W, int H) {
-    extern __shared__ float lds[];  // 64 rows x S (S >= 64 + LP, multiple of 4)
+    extern __shared__ float lds[];  // 64 rows x S (S >= 68 + LP, multiple of 4), taps
     const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
     const int tx4 = threadIdx.x & 15, ty4 = threadIdx.x >> 4;
     const int cols = TILE + 2 * r;
+    const float* taps = rtaps;
+    if (TAPS_LDS) {
+        float* tl = lds + TILE * S;
+        for (int i = threadIdx.x; i < K * LP + 4; i += 256) tl[i] = rtaps[i];
+        taps = tl;  // visible after the first barrier below
+    }
     float acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -119,24 +161,32 @@ __global__ __launch_bounds__(256) void k_lr_rows(float* __restrict__ out,
             lds[idx] = v;
         }
         __syncthreads();
-        const float* T = rtaps + (size_t)k * LP;
+        // rows ty4 + 16 a; each step takes one tap quadruple for all four
+        const float* T = taps + (size_t)k * LP;
+        const float* row0 = lds + ty4 * S + 4 * tx4;
+        const int RS = 16 * S;
+        float4 cur[4], nxt[4], fut[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const float* row = lds + (ty4 + 16 * a) * S + 4 * tx4;
-            for (int jq = 0; jq < LP; jq += 4) {
-                const float4 p = *reinterpret_cast<const float4*>(row + jq);
-                const float4 q = *reinterpret_cast<const float4*>(row + jq + 4);
-                const float win[8] = {p.x, p.y, p.z, p.w, q.x, q.y, q.z, q.w};
-                const float t0 = T[jq], t1 = T[jq + 1], t2 = T[jq + 2], t3 = T[jq + 3];
+            cur[a] = *reinterpret_cast<const float4*>(row0 + a * RS);
+            nxt[a] = *reinterpret_cast<const float4*>(row0 + a * RS + 4);
+        }
+        float4 t = *reinterpret_cast<const float4*>(T);
+        // three steps per trip: the roles of the three row-segment sets
+        // rotate with period 3, so the rotation costs no register moves
+        const int nq = LP >> 2;
+        int q = 0;
+        for (; q + 3 <= nq; q += 3) {
+            lr_row_step(acc, cur, nxt, fut, t, row0, RS, T, q);
+            lr_row_step(acc, nxt, fut, cur, t, row0, RS, T, q + 1);
+            lr_row_step(acc, fut, cur, nxt, t, row0, RS, T, q + 2);
+        }
+        for (; q < nq; ++q) {
+            lr_row_step(acc, cur, nxt, fut, t, row0, RS, T, q);
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    float s = acc[a][c];
-                    s = __builtin_fmaf(t0, win[c], s);
-                    s = __builtin_fmaf(t1, win[c + 1], s);
-                    s = __builtin_fmaf(t2, win[c + 2], s);
-                    s = __builtin_fmaf(t3, win[c + 3], s);
-                    acc[a][c] = s;
-                }
+            for (int a = 0; a < 4; ++a) {
+                cur[a] = nxt[a];
+                nxt[a] = fut[a];
             }
         }
         __syncthreads();
@@ -164,7 +214,7 @@ int lr_cols_geometry(int r, int* LC, size_t* lds) {
 int lr_rows_geometry(int r, int* LP, int* S, size_t* lds) {
     const int L = 2 * r + 1;
     *LP = (L + 3) & ~3;
-    *S = (TILE + *LP + 4 + 3) & ~3;  // float4 pairs read up to col 60 + LP + 3
+    *S = (TILE + *LP + 4 + 3) & ~3;  // float4 reads up to col 60 + LP + 7
     *lds = (size_t)TILE * (*S) * sizeof(float);
     return *lds <= LDS_LIMIT ? CAGEO_OK : CAGEO_ERR_ARG;
 }
@@ -183,8 +233,14 @@ int launch_convolve_fast(Context* c, const float* field, float* out) {
                        c->d_lr_col, K, r, LC, W, H);
     CAGEO_HIP_TRY(hipGetLastError());
     dim3 gr((W + TILE - 1) / TILE, (H + TILE - 1) / TILE);
-    hipLaunchKernelGGL(k_lr_rows, gr, dim3(256), lds_r, c->stream, out, c->d_lr_planes,
-                       c->d_lr_row, K, r, LP, S, W, H);
+    // the taps go into LDS behind the tile when both fit
+    const size_t taps_bytes = ((size_t)K * LP + 4) * sizeof(float);
+    if (lds_r + taps_bytes <= LDS_LIMIT)
+        hipLaunchKernelGGL(k_lr_rows<true>, gr, dim3(256), lds_r + taps_bytes, c->stream, out,
+                           c->d_lr_planes, c->d_lr_row, K, r, LP, S, W, H);
+    else
+        hipLaunchKernelGGL(k_lr_rows<false>, gr, dim3(256), lds_r, c->stream, out,
+                           c->d_lr_planes, c->d_lr_row, K, r, LP, S, W, H);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
