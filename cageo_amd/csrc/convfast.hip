@@ -43,10 +43,18 @@ __global__ __launch_bounds__(256) void k_lr_cols(float* __restrict__ planes,
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int rows = TILE + 2 * r, lrows = TILE + LC;
     const int x = x0 + tx;
-    for (int k = ty; k < lrows; k += 4) {
-        float v = 0.0f;
-        if (k < rows && x < W) v = in[(size_t)wrap_i(y0 - r + k, H) * W + x];
-        lds[k * TILE + tx] = v;
+    constexpr int B = 8;  // batches of independent loads
+    for (int k0 = ty; k0 < lrows; k0 += 4 * B) {
+        float v[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int k = k0 + 4 * j;
+            v[j] = 0.0f;
+            if (k < rows && x < W) v[j] = in[(size_t)wrap_i(y0 - r + k, H) * W + x];
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if (k0 + 4 * j < lrows) lds[(k0 + 4 * j) * TILE + tx] = v[j];
     }
     __syncthreads();
     float acc[LR_KG][STRIP];
@@ -136,7 +144,10 @@ __global__ __launch_bounds__(256) void k_lr_rows(float* __restrict__ out,
                                                  const float* __restrict__ rtaps, int K, int r,
                                                  int LP, int S, int W, int H) {
     extern __shared__ float lds[];  // 64 rows x S (S >= 68 + LP, multiple of 4), taps
-    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
+    // transposed grid (blockIdx.x walks tile rows): the left and right
+    // neighbours, whose columns form this tile's halo, are gridDim.x blocks
+    // away, on the same XCD and L2 when gridDim.x is a multiple of 8
+    const int x0 = blockIdx.y * TILE, y0 = blockIdx.x * TILE;
     const int tx4 = threadIdx.x & 15, ty4 = threadIdx.x >> 4;
     const int cols = TILE + 2 * r;
     const float* taps = rtaps;
@@ -153,12 +164,22 @@ __global__ __launch_bounds__(256) void k_lr_rows(float* __restrict__ out,
     const size_t n = (size_t)W * H;
     for (int k = 0; k < K; ++k) {
         const float* P = planes + (size_t)k * n;
-        for (int idx = threadIdx.x; idx < TILE * S; idx += 256) {
-            const int row = idx / S, cc = idx - row * S;
-            const int gy = y0 + row;
-            float v = 0.0f;
-            if (cc < cols && gy < H) v = P[(size_t)gy * W + wrap_i(x0 - r + cc, W)];
-            lds[idx] = v;
+        constexpr int B = 8;  // batches of independent loads
+        for (int i0 = threadIdx.x; i0 < TILE * S; i0 += 256 * B) {
+            float v[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int idx = i0 + 256 * j;
+                v[j] = 0.0f;
+                if (idx < TILE * S) {
+                    const int row = idx / S, cc = idx - row * S;
+                    const int gy = y0 + row;
+                    if (cc < cols && gy < H) v[j] = P[(size_t)gy * W + wrap_i(x0 - r + cc, W)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j)
+                if (i0 + 256 * j < TILE * S) lds[i0 + 256 * j] = v[j];
         }
         __syncthreads();
         // rows ty4 + 16 a; each step takes one tap quadruple for all four
@@ -232,7 +253,7 @@ int launch_convolve_fast(Context* c, const float* field, float* out) {
     hipLaunchKernelGGL(k_lr_cols, gc, dim3(256), lds_c, c->stream, c->d_lr_planes, field,
                        c->d_lr_col, K, r, LC, W, H);
     CAGEO_HIP_TRY(hipGetLastError());
-    dim3 gr((W + TILE - 1) / TILE, (H + TILE - 1) / TILE);
+    dim3 gr((H + TILE - 1) / TILE, (W + TILE - 1) / TILE);  // transposed
     // the taps go into LDS behind the tile when both fit
     const size_t taps_bytes = ((size_t)K * LP + 4) * sizeof(float);
     if (lds_r + taps_bytes <= LDS_LIMIT)
