@@ -75,7 +75,14 @@ __device__ inline void burn_tile(int W, int H, const uint8_t* __restrict__ sf,
     __shared__ float zt[BL_W * BL_H];
     __shared__ uint8_t ft[BL_W * BL_H];
     constexpr int ITER = (BL_W * BL_H + 255) / 256;  // loads per thread at the largest radius
-    const int x0 = blockIdx.x * BT_X, y0 = blockIdx.y * BT_Y;
+    // Blocks go round robin over the 8 XCDs; renumbering them so that each
+    // XCD takes a contiguous band of tiles keeps the neighbours whose cache
+    // lines hold this tile's halo in the same L2 (a bijection on the block
+    // numbers when their count is a multiple of 8).
+    const unsigned nb = gridDim.x * gridDim.y;
+    unsigned t = blockIdx.y * gridDim.x + blockIdx.x;
+    if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
+    const int x0 = (int)(t % gridDim.x) * BT_X, y0 = (int)(t / gridDim.x) * BT_Y;
     const int lw = BT_X + 2 * R, lh = BT_Y + 2 * R;
     // all of a thread's loads are issued before its LDS stores
     float zv[ITER];
