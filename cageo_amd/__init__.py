@@ -16,9 +16,11 @@ execute, ``flow_tracing`` <- FlowTracingAlgorithm::execute,
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -38,7 +40,24 @@ LAKE = -9998
 
 _STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error",
            -3: "device allocation failed", -4: "iteration cap reached",
-           -5: "invalid call order", -6: "input outside the supported domain"}
+           -5: "invalid call order", -6: "input outside the supported domain",
+           -7: "persistent launch abandoned (abort word or barrier time limit)"}
+ERR_ABORTED = -7
+
+# Every live handle (context, batch, surrogate), closed before the interpreter
+# tears down: a handle left to the garbage collector at exit would free device
+# memory after the HIP runtime or a profiler attached to it has begun its own
+# teardown.
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
 
 
 class CageoError(RuntimeError):
@@ -93,6 +112,10 @@ def lib():
         "cageo_write_jpeg_gray": ([ctypes.c_char_p, vp, i, i, i], i),
         "cageo_iteration_counts": ([vp, vp], i),
         "cageo_context_configure": ([vp, i, i], i),
+        "cageo_context_abort": ([vp, i], i),
+        "cageo_context_set_barrier_timeout": ([vp, ctypes.c_double], i),
+        "cageo_context_long_paths": ([vp, ctypes.POINTER(i64)], i),
+        "cageo_batch_abort": ([vp, i], i),
         "cageo_context_set_conv_mode": ([vp, i], i),
         "cageo_context_conv_terms": ([vp, vp], i),
         "cageo_filter_low_rank": ([vp, i, i, ctypes.c_double, vp, vp, vp, vp], i),
@@ -194,6 +217,7 @@ class Batch:
         _check(lib().cageo_batch_create(ctypes.byref(h), n, ws, hs, sp), "batch_create")
         self._h = h
         self._keep = []
+        _live.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -242,6 +266,16 @@ class Batch:
         _check(lib().cageo_batch_configure_tile(self._h, t, persistent_workgroups, conv_rows),
                "batch_configure_tile")
 
+    def abort(self, set_=True):
+        """Raise (or clear) the abort word of every tile's context."""
+        _check(lib().cageo_batch_abort(self._h, int(bool(set_))), "batch_abort")
+
+    def context(self, t):
+        """The raw context handle of tile t (for stage calls on one tile)."""
+        h = ctypes.c_void_p()
+        _check(lib().cageo_batch_context(self._h, t, ctypes.byref(h)), "batch_context")
+        return h
+
 
 class Surrogate:
     """The border-probability surrogate behind the C ABI (cageo_surrogate):
@@ -255,6 +289,7 @@ class Surrogate:
         _check(lib().cageo_surrogate_create(ctypes.byref(h), os.fsencode(weights_path), sp),
                "surrogate_create")
         self._h = h
+        _live.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -322,6 +357,7 @@ class Context:
         s = stream if isinstance(stream, (int, type(None))) else stream.cuda_stream
         _check(lib().cageo_context_create(ctypes.byref(h), width, height, s), "context_create")
         self._h = h
+        _live.add(self)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -414,6 +450,21 @@ class Context:
     def configure(self, persistent_workgroups=0, conv_rows=0):
         _check(lib().cageo_context_configure(self._h, persistent_workgroups, conv_rows),
                "context_configure")
+
+    def abort(self, set_=True):
+        """Raise (or clear) the abort word: persistent launches end at their
+        next barrier wait with CAGEO_ERR_ABORTED while it is raised."""
+        _check(lib().cageo_context_abort(self._h, int(bool(set_))), "context_abort")
+
+    def set_barrier_timeout(self, seconds):
+        _check(lib().cageo_context_set_barrier_timeout(self._h, float(seconds)),
+               "set_barrier_timeout")
+
+    def long_paths(self):
+        """Cells past the reference's walk guard, summed over the last call."""
+        out = ctypes.c_int64()
+        _check(lib().cageo_context_long_paths(self._h, ctypes.byref(out)), "long_paths")
+        return out.value
 
     def stage_times(self, enable=True):
         out = np.zeros(7, np.float32)

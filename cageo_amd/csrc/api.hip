@@ -9,8 +9,6 @@
 
 namespace cageo {
 
-namespace {
-
 // min <- max ordered value, max and "bad" <- 0.  A kernel rather than memsets
 // so that a captured graph holds no memset nodes.
 __global__ void k_reset_minmax(uint32_t* __restrict__ scal) {
@@ -26,6 +24,8 @@ int reset_minmax(Context* c) {
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
+
+namespace {
 
 int sync(Context* c) {
     CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -50,6 +50,7 @@ void free_all(Context* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
+    if (c->h_abort) (void)hipHostFree(c->h_abort);
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -99,9 +100,34 @@ int read_scalars(Context* c) {
     return CAGEO_OK;
 }
 
-int clear_status(Context* c) {
-    CAGEO_HIP_TRY(hipMemsetAsync(c->d_ctrl + CTL_STATUS, 0, sizeof(uint32_t), c->stream));
+__global__ void k_reset_launch_state(uint32_t* __restrict__ status, uint32_t* __restrict__ bar,
+                                     uint32_t* __restrict__ scal) {
+    // the generation word keeps counting; every other barrier word restarts
+    for (int i = threadIdx.x; i < BAR_WORDS; i += blockDim.x)
+        if (i != BAR_GEN) bar[i] = 0u;
+    if (threadIdx.x == 0) {
+        *status = 0u;
+        if (scal) scal[SC_LONGRUN] = 0u;
+    }
+}
+
+int reset_launch_state(hipStream_t s, uint32_t* status, uint32_t* bar, uint32_t* scal) {
+    hipLaunchKernelGGL(k_reset_launch_state, dim3(1), dim3(256), 0, s, status, bar, scal);
+    CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
+}
+
+int barrier_ticks(int device, double seconds, uint64_t* ticks) {
+    int khz = 0;
+    CAGEO_HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    if (khz <= 0) khz = 100000;  // the 100 MHz constant clock of CDNA parts
+    const double t = seconds * 1e3 * (double)khz;
+    *ticks = t >= 1.8e19 ? ~0ull : (uint64_t)t;
+    return CAGEO_OK;
+}
+
+int clear_status(Context* c) {
+    return reset_launch_state(c->stream, c->d_ctrl + CTL_STATUS, c->d_bar, c->d_scal);
 }
 
 // Synchronises and returns the sticky device status of the persistent kernels.
@@ -170,6 +196,7 @@ const char* cageo_status_string(int s) {
         case CAGEO_ERR_NOT_CONVERGED: return "iteration cap reached";
         case CAGEO_ERR_STATE: return "invalid call order";
         case CAGEO_ERR_RANGE: return "input outside the supported domain";
+        case CAGEO_ERR_ABORTED: return "persistent launch abandoned (abort word or barrier time limit)";
         default: return "unknown status";
     }
 }
@@ -223,6 +250,22 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
         hipHostMalloc(reinterpret_cast<void**>(&c->h_scal), cageo::SC_NSLOTS * sizeof(uint32_t)) !=
             hipSuccess)
         st = CAGEO_ERR_NOMEM;
+    // The abort word: host memory the kernels read while they wait at a
+    // barrier, so the host can raise it while a launch runs.
+    if (st == CAGEO_OK &&
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_abort), sizeof(uint32_t),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+        st = CAGEO_ERR_NOMEM;
+    if (st == CAGEO_OK) {
+        __atomic_store_n(c->h_abort, 0u, __ATOMIC_SEQ_CST);
+        if (hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_habort), c->h_abort, 0) !=
+            hipSuccess)
+            st = CAGEO_ERR_HIP;
+    }
+    if (st == CAGEO_OK) {
+        c->bar_timeout_s = cageo::BAR_DEFAULT_TIMEOUT_S;
+        st = cageo::barrier_ticks(c->device, c->bar_timeout_s, &c->bar_limit);
+    }
     if (st == CAGEO_OK &&
         (hipMemsetAsync(c->d_mark, 0, (size_t)c->ntiles * sizeof(uint32_t), c->stream) != hipSuccess ||
          hipMemsetAsync(c->d_mark_pf, 0, (size_t)c->ntiles * sizeof(uint32_t), c->stream) != hipSuccess ||
@@ -500,6 +543,30 @@ int cageo_context_configure(cageo_context* ctx, int persistent_workgroups, int c
         c->pf_grid = c->fr_grid = c->tr_grid = persistent_workgroups;
     }
     c->conv_ty = conv_rows;
+    return CAGEO_OK;
+}
+
+int cageo_context_long_paths(cageo_context* ctx, int64_t* out) {
+    if (!ctx || !out) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    CAGEO_TRY(cageo::read_scalars(c));
+    *out = (int64_t)c->h_scal[cageo::SC_LONGRUN];
+    return CAGEO_OK;
+}
+
+int cageo_context_abort(cageo_context* ctx, int set) {
+    if (!ctx || !ctx->c.h_abort) return CAGEO_ERR_ARG;
+    __atomic_store_n(ctx->c.h_abort, set ? 1u : 0u, __ATOMIC_SEQ_CST);
+    return CAGEO_OK;
+}
+
+int cageo_context_set_barrier_timeout(cageo_context* ctx, double seconds) {
+    if (!ctx || !(seconds >= 0.0)) return CAGEO_ERR_ARG;
+    Context* c = &ctx->c;
+    uint64_t t = 0;
+    CAGEO_TRY(cageo::barrier_ticks(c->device, seconds, &t));
+    c->bar_limit = t;
+    c->bar_timeout_s = seconds;
     return CAGEO_OK;
 }
 

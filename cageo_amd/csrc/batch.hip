@@ -128,7 +128,7 @@ int check_all(cageo_batch* b) {
 int clear_all(cageo_batch* b) {
     for (auto& t : b->tiles) CAGEO_TRY(clear_status(&t.ctx->c));
     if (b->multi_ready)
-        CAGEO_HIP_TRY(hipMemsetAsync(b->multi.d_ctrl + CTL_STATUS, 0, sizeof(uint32_t), b->stream));
+        CAGEO_TRY(reset_launch_state(b->stream, b->multi.d_ctrl + CTL_STATUS, b->multi.d_bar));
     return CAGEO_OK;
 }
 
@@ -236,6 +236,10 @@ int multi_setup(MultiState* m, Context* const* ctxs, int nr) {
          hipMemset(m->d_cert, 0, 3 * (size_t)nr * sizeof(uint32_t)) != hipSuccess ||
          hipMemset(m->d_bar, 0, BAR_WORDS * sizeof(uint32_t)) != hipSuccess))
         st = CAGEO_ERR_HIP;
+    // the all-tile launches share the abort word and wait bound of tile 0's
+    // context (cageo_batch_abort raises every tile's word)
+    m->d_habort = ctxs[0]->d_habort;
+    m->bar_limit = ctxs[0]->bar_limit;
     if (st == CAGEO_OK) st = pit_fill_tables(m, ctxs);
     if (st == CAGEO_OK) st = flow_routing_tables(m, ctxs);
     if (st == CAGEO_OK) st = flow_tracing_tables(m, ctxs);
@@ -409,18 +413,11 @@ int cageo_batch_tune(cageo_batch* b, float* const* d_scratch_mc, int reps, int* 
             out_config[2 * t + 1] = best_ty;
         }
     }
-    // workgroups of the all-tile launches: fractions of the smallest cap
+    // workgroups of the all-tile launches: fractions of the smallest
+    // co-resident count of the all-tile kernels themselves
     CAGEO_TRY(cageo::ensure_multi(b));
     cageo::MultiState& m = b->multi;
-    int cap = 0;
-    {
-        int caps[3];
-        Context* c = &b->tiles[0].ctx->c;
-        CAGEO_TRY(cageo::pit_fill_grid_cap(c, &caps[0]));
-        CAGEO_TRY(cageo::flow_routing_grid_cap(c, &caps[1]));
-        CAGEO_TRY(cageo::flow_tracing_grid_cap(c, &caps[2]));
-        cap = std::min(caps[0], std::min(caps[1], caps[2]));
-    }
+    const int cap = std::min(m.pf_cap, std::min(m.fr_cap, m.tr_cap));
     float best = 3.4e38f;
     int best_g = cap;
     for (int g : {cap / 4, cap / 2, (3 * cap) / 4, cap}) {
@@ -448,6 +445,12 @@ int cageo_batch_info(cageo_batch* b, int* out) {
     out[1] = b->multi.pf_grid;
     out[2] = b->multi.fr_grid;
     out[3] = b->multi.tr_grid;
+    return CAGEO_OK;
+}
+
+int cageo_batch_abort(cageo_batch* b, int set) {
+    if (!b) return CAGEO_ERR_ARG;
+    for (auto& t : b->tiles) CAGEO_TRY(cageo_context_abort(t.ctx, set));
     return CAGEO_OK;
 }
 

@@ -39,6 +39,7 @@ enum ScalarSlot : int {
     SC_CHANGED = 7,    // generic "something changed" flag
     SC_LONGPATH = 8,   // cells whose trace is longer than the walk guard
     SC_EPOCH = 9,      // unused on device; reserved
+    SC_LONGRUN = 10,   // SC_LONGPATH summed over the launches since the last clear_status
     SC_NSLOTS = 64
 };
 
@@ -58,6 +59,10 @@ struct Context {
     int* d_work3 = nullptr;         // 7 worklists of the persistent kernels
     uint32_t* d_ctrl = nullptr;     // control block of the persistent kernels
     uint32_t* d_bar = nullptr;      // grid barrier words (BAR_WORDS), see grid_sync
+    uint32_t* h_abort = nullptr;    // abort word in host-mapped memory (cageo_context_abort)
+    uint32_t* d_habort = nullptr;   // its device address
+    uint64_t bar_limit = 0;         // bound of one barrier wait, wall_clock64() ticks
+    double bar_timeout_s = 0.0;
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
     int conv_ty = 0;  // tuned row-block height of the 2-D convolution (0 = largest that fits)
     // Fast mode of the 2-D convolution: the filter's low-rank separable form.
@@ -173,16 +178,53 @@ __host__ __device__ inline bool pf_live(float v, float min_elev) {
 // last arrival resets its counter and releases before counting at the top,
 // so the reset is seen before any arrival at the next barrier (those follow
 // the generation bump, which follows the top count).
+//
+// The wait is bounded.  If co-residency is lost (another launch holds part
+// of the device for long, or workgroups of a persistent launch on another
+// stream of the same device take the places of some of this launch's), or
+// the host raises the context's abort word, the waiting workgroup gives up:
+// it raises the barrier block's abort word, records CAGEO_ERR_ABORTED in the
+// launch's sticky status word and returns false.  Every other workgroup sees
+// the abort word in its own wait (or at its next barrier) and returns false
+// too, so the whole grid leaves its loops and ends normally.  A later
+// persistent launch on the same barrier block ends at once while the abort
+// word is up (grid_aborted); the host clears the word and the counters
+// before its next call (k_reset_launch_state).
 constexpr int BAR_STRIDE = 32;  // words: one 128-byte line per word
 constexpr int BAR_GROUPS = 16;
 constexpr int BAR_COUNT = 0;
 constexpr int BAR_GEN = BAR_STRIDE;
-constexpr int BAR_GROUP = 2 * BAR_STRIDE;
-constexpr int BAR_WORDS = (2 + BAR_GROUPS) * BAR_STRIDE;
+constexpr int BAR_ABORT = 2 * BAR_STRIDE;
+constexpr int BAR_GROUP = 3 * BAR_STRIDE;
+constexpr int BAR_WORDS = (3 + BAR_GROUPS) * BAR_STRIDE;
+// Polls between two checks of the abort words and the clock.
+constexpr uint32_t BAR_CHECK_EVERY = 64;
+// Default bound of one barrier wait, in seconds.
+constexpr double BAR_DEFAULT_TIMEOUT_S = 5.0;
 
-__device__ inline void grid_sync(uint32_t* bar) {
+// What a persistent launch needs for its barriers.
+struct GridBar {
+    uint32_t* bar;                // BAR_WORDS device words
+    const uint32_t* host_abort;   // host-mapped word raised by the host (may be null)
+    uint64_t limit;               // bound of one wait, in wall_clock64() ticks
+    uint32_t* status;             // the launch's sticky status word
+};
+
+__device__ inline bool bar_should_stop(const GridBar& gb, uint64_t t0, bool host) {
+    if (__hip_atomic_load(&gb.bar[BAR_ABORT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+        return true;
+    if (wall_clock64() - t0 >= gb.limit) return true;
+    return host && gb.host_abort &&
+           __hip_atomic_load(gb.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+}
+
+// Returns false when the launch is being abandoned (then every workgroup
+// must leave its loops and end without another barrier).
+__device__ inline bool grid_sync(const GridBar& gb) {
     __syncthreads();
+    int fail = 0;
     if (threadIdx.x == 0) {
+        uint32_t* bar = gb.bar;
         uint32_t* gen = &bar[BAR_GEN];
         const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -201,11 +243,32 @@ __device__ inline void grid_sync(uint32_t* bar) {
                 __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g)
+        const uint64_t t0 = wall_clock64();
+        for (uint32_t it = 0;
+             __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g; ++it) {
+            // the first check at once (an abandoned launch is left without a
+            // full poll period), the host word every fourth check
+            if (it % BAR_CHECK_EVERY == 0 &&
+                bar_should_stop(gb, t0, it % (4 * BAR_CHECK_EVERY) == 0)) {
+                __hip_atomic_store(&bar[BAR_ABORT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicCAS(gb.status, 0u, (uint32_t)(-CAGEO_ERR_ABORTED));
+                fail = 1;
+                break;
+            }
             __builtin_amdgcn_s_sleep(2);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        if (!fail) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __syncthreads();
+    return !__syncthreads_or(fail);
+}
+
+// True (in every thread of the workgroup) when an earlier launch on this
+// barrier block was abandoned and the host has not cleared it yet: the
+// launch must end at once.
+__device__ inline bool grid_aborted(const GridBar& gb) {
+    return __syncthreads_or(threadIdx.x == 0 &&
+                            __hip_atomic_load(&gb.bar[BAR_ABORT], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT) != 0u);
 }
 
 // Words of the control block (d_ctrl), shared by the persistent kernels.
@@ -260,6 +323,9 @@ struct MultiState {
     uint32_t* d_ctrl = nullptr;       // CTL_WORDS
     uint32_t* d_cert = nullptr;       // 3 rotating counters per raster
     uint32_t* d_bar = nullptr;        // grid barrier words
+    const uint32_t* d_habort = nullptr;  // host abort word (the batch's)
+    uint64_t bar_limit = 0;
+    int pf_cap = 0, fr_cap = 0, tr_cap = 0;  // co-resident workgroups of the all-tile kernels
     void* d_pf = nullptr;             // raster tables of the three kernels
     void* d_fr = nullptr;
     void* d_tr = nullptr;
@@ -268,6 +334,19 @@ struct MultiState {
 
 // Worklist helpers --------------------------------------------------------
 int read_scalars(Context* c);
+// Barrier of a launch on the context's (or the batch's) device state.
+inline GridBar context_gridbar(Context* c) {
+    return GridBar{c->d_bar, c->d_habort, c->bar_limit, c->d_ctrl + CTL_STATUS};
+}
+inline GridBar multi_gridbar(MultiState* m) {
+    return GridBar{m->d_bar, m->d_habort, m->bar_limit, m->d_ctrl + CTL_STATUS};
+}
+// Ticks of wall_clock64() in `seconds` on `device`.
+int barrier_ticks(int device, double seconds, uint64_t* ticks);
+// Clears a launch state after any earlier launch: the sticky status word,
+// the barrier counters and the barrier abort word (one small kernel, so
+// that it may be captured into a graph).
+int reset_launch_state(hipStream_t s, uint32_t* status, uint32_t* bar, uint32_t* scal = nullptr);
 
 // Stage drivers implemented in the stage translation units -----------------
 int launch_random_field(Context* c, float* out, uint64_t seed, uint32_t realization,
@@ -292,6 +371,7 @@ int launch_stream_flags(Context* c, const float* stream, float nodata_stream, ui
 int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq);
 int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool stats_ready);
 int check_status(Context* c);
+int reset_minmax(Context* c);
 // One realization from the error field in c->d_a (stages a3..a9); with
 // `evs`, events evs[2..7] are recorded after each stage.
 int realization_body(Context* c, float* mc, hipEvent_t* evs);

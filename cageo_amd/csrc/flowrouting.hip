@@ -77,7 +77,7 @@ struct FrArgs {
     uint32_t* mark; // per-block epoch stamps
     uint32_t* rows; // per-block dirty-row masks: 2 pass parities x 64 bits
     uint32_t* ctrl;
-    uint32_t* bar;  // grid barrier words
+    GridBar gb;     // grid barrier
     int max_passes;
 };
 
@@ -294,13 +294,14 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
     __shared__ FrShared S;
+    if (grid_aborted(a.gb)) return;
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH_FR]);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_COUNT + 0] = 0;
         a.ctrl[CTL_COUNT + 1] = 0;
         a.ctrl[CTL_COUNT + 2] = 0;
     }
-    grid_sync(a.bar);
+    if (!grid_sync(a.gb)) return;
 
     // D8 block by block, from the block's halo tile of elevations staged in
     // LDS (one batch of loads); blocks holding flats open list 0.
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         if (__syncthreads_or(has_flat) && threadIdx.x == 0)
             a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
     }
-    grid_sync(a.bar);
+    if (!grid_sync(a.gb)) return;
 
     int li = 0;
     bool converged = false;
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         int* wout = a.work + (size_t)cout * a.total;
         for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
             bfs_tile(S, a, fr_block<MULTI>(a, win[i]), wout, &a.ctrl[CTL_COUNT + cout], epoch, pass);
-        grid_sync(a.bar);
+        if (!grid_sync(a.gb)) return;
         li = cout;
     }
     // Nothing below writes a control word that the pass loop reads, so no
@@ -460,7 +461,7 @@ int flow_routing(Context* c, const float* z, int8_t* fd) {
     a.mark = c->d_mark;
     a.rows = c->d_rows;
     a.ctrl = c->d_ctrl;
-    a.bar = c->d_bar;
+    a.gb = context_gridbar(c);
     a.max_passes = 1 << 20;
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
     hipLaunchKernelGGL(k_fr_persistent<false>, dim3(c->fr_grid), dim3(TILE_THREADS), 0, c->stream,
@@ -475,15 +476,16 @@ int flow_routing_tables(MultiState* m, Context* const* ctxs) {
     for (int r = 0; r < m->nr; ++r) t[r] = fr_raster(ctxs[r], ctxs[r]->d_w, ctxs[r]->d_fd);
     CAGEO_HIP_TRY(hipMalloc(&m->d_fr, t.size() * sizeof(FrRaster)));
     CAGEO_HIP_TRY(hipMemcpy(m->d_fr, t.data(), t.size() * sizeof(FrRaster), hipMemcpyHostToDevice));
-    if (m->fr_grid <= 0) {
+    {
         int per_cu = 0;
         CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void*>(k_fr_persistent<true>), TILE_THREADS, 0));
         hipDeviceProp_t prop;
         CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, ctxs[0]->device));
         if (per_cu < 1) return CAGEO_ERR_HIP;
-        m->fr_grid = per_cu * prop.multiProcessorCount;
+        m->fr_cap = per_cu * prop.multiProcessorCount;
     }
+    if (m->fr_grid <= 0) m->fr_grid = m->fr_cap;
     return CAGEO_OK;
 }
 
@@ -498,7 +500,7 @@ int flow_routing_multi(MultiState* m, hipStream_t s) {
     a.mark = m->d_mark_fr;
     a.rows = m->d_rows;
     a.ctrl = m->d_ctrl;
-    a.bar = m->d_bar;
+    a.gb = multi_gridbar(m);
     a.max_passes = 1 << 20;
     hipLaunchKernelGGL(k_fr_persistent<true>, dim3(m->fr_grid), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());

@@ -192,7 +192,7 @@ struct TrArgs {
     int nr, total;
     int max_rounds;
     uint32_t* ctrl;
-    uint32_t* bar;  // grid barrier words
+    GridBar gb;     // grid barrier
 };
 
 struct TrBlock {
@@ -241,6 +241,7 @@ __device__ inline TrUnit tr_unit(const TrArgs& a, long long u) {
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
     __shared__ uint64_t L[TILE * TILE];
+    if (grid_aborted(a.gb)) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_FLAG + 0] = 0;
         a.ctrl[CTL_FLAG + 1] = 0;
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         if (b.lb == 0 && threadIdx.x == 0) R.scal[SC_LONGPATH] = 0;
         trace_tile(L, R.fd, R.sf, R.slabel, R.ent, R.W, R.H, R.ntx, b.lb);
     }
-    grid_sync(a.bar);
+    if (!grid_sync(a.gb)) return;
 
     const long long units = (long long)a.total * UNITS_PER_BLOCK;
     int rounds = 0;
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         }
         if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&a.ctrl[CTL_FLAG + r % 3], 1u);
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_FLAG + (r + 1) % 3] = 0;
-        grid_sync(a.bar);
+        if (!grid_sync(a.gb)) return;
         if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
     }
 
@@ -303,11 +304,14 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         }
         if (__any(longp)) {
             for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
-            if ((threadIdx.x & 63) == 0) atomicAdd(&t.R.scal[SC_LONGPATH], longp);
+            if ((threadIdx.x & 63) == 0) {
+                atomicAdd(&t.R.scal[SC_LONGPATH], longp);
+                atomicAdd(&t.R.scal[SC_LONGRUN], longp);
+            }
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
-    grid_sync(a.bar);
+    if (!grid_sync(a.gb)) return;
     // performConnectFlowPath: label[0] is final and never changes here.
     for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
         const TrUnit t = tr_unit<MULTI>(a, u);
@@ -373,7 +377,7 @@ int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int3
     a.total = c->ntiles;
     a.max_rounds = max_rounds_for(c->n);
     a.ctrl = c->d_ctrl;
-    a.bar = c->d_bar;
+    a.gb = context_gridbar(c);
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
     hipLaunchKernelGGL(k_tr_persistent<false>, dim3(c->tr_grid), dim3(TILE_THREADS), 0, c->stream,
                        a);
@@ -395,15 +399,16 @@ int flow_tracing_tables(MultiState* m, Context* const* ctxs) {
     }
     CAGEO_HIP_TRY(hipMalloc(&m->d_tr, t.size() * sizeof(TrRaster)));
     CAGEO_HIP_TRY(hipMemcpy(m->d_tr, t.data(), t.size() * sizeof(TrRaster), hipMemcpyHostToDevice));
-    if (m->tr_grid <= 0) {
+    {
         int per_cu = 0;
         CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void*>(k_tr_persistent<true>), TILE_THREADS, 0));
         hipDeviceProp_t prop;
         CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, ctxs[0]->device));
         if (per_cu < 1) return CAGEO_ERR_HIP;
-        m->tr_grid = per_cu * prop.multiProcessorCount;
+        m->tr_cap = per_cu * prop.multiProcessorCount;
     }
+    if (m->tr_grid <= 0) m->tr_grid = m->tr_cap;
     return CAGEO_OK;
 }
 
@@ -416,7 +421,7 @@ int flow_tracing_multi(MultiState* m, hipStream_t s) {
     a.total = m->total;
     a.max_rounds = max_rounds_for((size_t)m->max_cells);
     a.ctrl = m->d_ctrl;
-    a.bar = m->d_bar;
+    a.gb = multi_gridbar(m);
     hipLaunchKernelGGL(k_tr_persistent<true>, dim3(m->tr_grid), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;

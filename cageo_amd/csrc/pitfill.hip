@@ -64,6 +64,9 @@ struct PfShared {
     uint32_t steps; // row steps of the tile's solve, all waves
 };
 
+// run_passes result of a launch being abandoned at a barrier.
+constexpr int PF_ABANDONED = -2;
+
 // A solve of at least this many row steps (all waves; one sweep is 64)
 // ranks its tile heavy for the next pass (see Ranked).
 constexpr uint32_t PF_HEAVY = 256;
@@ -139,7 +142,7 @@ struct PfArgs {
     uint32_t* rows;       // per-block dirty-row masks: 2 pass parities x 64 bits
     uint32_t* ctrl;       // control block (CTL_*)
     uint32_t* cert;       // per raster: 3 rotating per-round fixed-cell counters
-    uint32_t* bar;        // grid barrier words
+    GridBar gb;           // grid barrier
     int max_passes;
 };
 
@@ -558,9 +561,10 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
 
 // Runs the passes of phase `phase` (kind 0 fill, 1 certify) until a pass
 // lists nothing; returns the index of the (empty) rotating list it stopped
-// at, or -1 when the pass cap is hit.  The first pass reads phase list
-// phase % 3 (and plays the part of rotating list `li`, the empty one the
-// last phase stopped at); its still-open tiles form phase list
+// at, -1 when the pass cap is hit, or PF_ABANDONED when a barrier gave up.
+// The first pass reads phase list phase % 3 (and plays the part of rotating
+// list `li`, the empty one the last phase stopped at); its still-open tiles
+// form phase list
 // (phase + 1) % 3; list (phase + 2) % 3, last read before the barrier that
 // ended the phase before, is cleared for the phase after.  Fixed cells
 // count into counter `slot` of their raster.
@@ -613,7 +617,7 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
             i = S.next;
             __syncthreads();
         }
-        grid_sync(a.bar);
+        if (!grid_sync(a.gb)) return PF_ABANDONED;
         li = cout;
     }
     return -1;
@@ -646,6 +650,7 @@ __device__ inline void compact_open(const PfArgs& a, int from) {
 template <bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pf_persistent(PfArgs a) {
     __shared__ PfShared S;
+    if (grid_aborted(a.gb)) return;
 
     // One raster: its minimum and domain flag, once.
     float mn1 = 0.0f;
@@ -731,7 +736,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
-    grid_sync(a.bar);
+    if (!grid_sync(a.gb)) return;
 
     // Phases alternate fill and certification.  Round r counts fixed cells
     // into counter slot r % 3 of each raster; its fill's first pass reads
@@ -741,17 +746,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (;; ++round) {
         const int slot = round % 3;
         li = run_passes<0, MULTI>(S, a, li, phase, slot, round, round > 0, epoch, mn1, ok1);
+        if (li == PF_ABANDONED) return;
         if (li < 0) break;
         ++phase;
         if (blockIdx.x == 0)
             for (int r = threadIdx.x; r < a.nr; r += blockDim.x) a.cert[3 * r + (round + 1) % 3] = 0;
         if (ld(&a.ctrl[CTL_PHASE + phase % 3]) + ld(&a.ctrl[CTL_PHASEB + phase % 3]) == 0) break;
         li = run_passes<1, MULTI>(S, a, li, phase, slot, round, false, epoch, mn1, ok1);
+        if (li == PF_ABANDONED) return;
         if (li < 0) break;
         ++phase;
         if (ld(&a.ctrl[CTL_PHASE + phase % 3]) + ld(&a.ctrl[CTL_PHASEB + phase % 3]) == 0) break;
         compact_open(a, phase % 3);
-        grid_sync(a.bar);
+        if (!grid_sync(a.gb)) return;
         if (ld(&a.ctrl[CTL_COMPACT]) + ld(&a.ctrl[CTL_COMPACTB]) == 0) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -817,7 +824,7 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.rows = c->d_rows;
     a.ctrl = c->d_ctrl;
     a.cert = c->d_ctrl + CTL_CERT;
-    a.bar = c->d_bar;
+    a.gb = context_gridbar(c);
     a.max_passes = 1 << 20;
     // An ordinary launch: the grid barrier needs its workgroups co-resident,
     // which the occupancy cap guarantees (work of other kernels on the device
@@ -834,9 +841,9 @@ int pit_fill_tables(MultiState* m, Context* const* ctxs) {
     for (int r = 0; r < m->nr; ++r) t[r] = pf_raster(ctxs[r], ctxs[r]->d_z, ctxs[r]->d_w, ctxs[r]->nodata_dem);
     CAGEO_HIP_TRY(hipMalloc(&m->d_pf, t.size() * sizeof(PfRaster)));
     CAGEO_HIP_TRY(hipMemcpy(m->d_pf, t.data(), t.size() * sizeof(PfRaster), hipMemcpyHostToDevice));
-    if (m->pf_grid <= 0)
-        CAGEO_TRY(occupancy_cap(reinterpret_cast<const void*>(k_pf_persistent<true>),
-                                ctxs[0]->device, &m->pf_grid));
+    CAGEO_TRY(occupancy_cap(reinterpret_cast<const void*>(k_pf_persistent<true>),
+                            ctxs[0]->device, &m->pf_cap));
+    if (m->pf_grid <= 0) m->pf_grid = m->pf_cap;
     return CAGEO_OK;
 }
 
@@ -855,7 +862,7 @@ int pit_fill_multi(MultiState* m, hipStream_t s) {
     a.rows = m->d_rows;
     a.ctrl = m->d_ctrl;
     a.cert = m->d_cert;
-    a.bar = m->d_bar;
+    a.gb = multi_gridbar(m);
     a.max_passes = 1 << 20;
     hipLaunchKernelGGL(k_pf_persistent<true>, dim3(m->pf_grid), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());

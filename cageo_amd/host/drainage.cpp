@@ -565,6 +565,11 @@ int run(int argc, char** argv) {
             check(cageo_run_realizations(e.ctx, monteCarloBorder.getDeviceData(), chunk,
                                          (uint64_t)o.seed, (uint32_t)iter),
                   "Monte Carlo realizations");
+            int64_t longPaths = 0;
+            check(cageo_context_long_paths(e.ctx, &longPaths), "long paths");
+            if (longPaths > 0)
+                std::cout << longPaths << " cells with flow paths longer than 100001 steps in "
+                          << chunk << " realizations" << std::endl;
             // one interval covers `chunk` realizations
             mcTimer.add(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
                             .count(),

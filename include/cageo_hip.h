@@ -34,6 +34,9 @@ extern "C" {
 #define CAGEO_ERR_NOT_CONVERGED (-4) /* an iteration cap was reached         */
 #define CAGEO_ERR_STATE (-5)         /* call order (e.g. no filter set)      */
 #define CAGEO_ERR_RANGE (-6)         /* input outside the reference's domain */
+#define CAGEO_ERR_ABORTED (-7)       /* a persistent launch was abandoned: the
+                                        host raised the abort word, or a grid
+                                        barrier waited past its time limit   */
 
 /* ---- constants shared with the reference ------------------------------- */
 #define CAGEO_FILTER_EXPONENTIAL 0 /* ConvolutionCommon.h:18-21 */
@@ -154,12 +157,36 @@ int cageo_mc_setup(cageo_context* ctx, const float* d_dem, const float* d_stream
 int cageo_run_realizations(cageo_context* ctx, float* d_mc, int n, uint64_t seed,
                            uint32_t first_realization);
 
+/* Cells, summed over the realizations of the last cageo_run_realizations,
+ * cageo_realization_from_noise or cageo_flow_tracing call on this context
+ * (or the last cageo_batch_run, for a tile's context), whose flow path is
+ * longer than the reference's 100 001-step walk guard
+ * (FlowTracingAlgorithmCPUOpenMP.h:68-75, where the reference prints
+ * "Infinite loop?" and its label depends on its raster order). */
+int cageo_context_long_paths(cageo_context* ctx, int64_t* out);
+
 /* Run one realization from an injected white-noise field (stages a3..a9),
  * leaving the intermediate rasters readable for parity checks.  d_noise is
  * consumed.  Any of the optional outputs may be NULL. */
 int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc,
                                  float* d_filled_out, int8_t* d_flowdir_out,
                                  int32_t* d_labels_out);
+
+/* The persistent kernels (pit fill, flow routing, flow tracing) wait at grid
+ * barriers that need all their workgroups resident at once.  Each wait is
+ * bounded: past `seconds` (default 5 s; 0 gives up at the first wait) the
+ * launch is abandoned, every workgroup ends normally, and the call returns
+ * CAGEO_ERR_ABORTED instead of hanging the device.  This replaces the
+ * reference's host-side convergence loop over reduceAnd (Reduce.cuh:48-89),
+ * which cannot deadlock but costs a host round trip per iteration.
+ * Persistent launches of two contexts on concurrent streams of one device
+ * can take each other's places on the device; run them one after another. */
+int cageo_context_set_barrier_timeout(cageo_context* ctx, double seconds);
+/* Raises (set != 0) or clears the context's abort word.  While it is raised,
+ * every persistent launch of the context ends at its next barrier wait and
+ * the call returns CAGEO_ERR_ABORTED.  The only call that may be made from
+ * another host thread while a call on the context runs. */
+int cageo_context_abort(cageo_context* ctx, int set);
 
 /* Per-stage device time (ms, hipEvent) of the last cageo_run_realizations
  * call, summed over its realizations: random field, convolution, realistic +
@@ -219,6 +246,9 @@ int cageo_batch_tune(cageo_batch* batch, float* const* d_scratch_mc, int reps, i
 /* out[4]: 64x64 blocks over all tiles, and the workgroup counts of the
  * all-tile pit-fill, flow-routing and flow-tracing launches. */
 int cageo_batch_info(cageo_batch* batch, int* out);
+/* Raises or clears the abort word of every tile's context (see
+ * cageo_context_abort); the all-tile launches read tile 0's. */
+int cageo_batch_abort(cageo_batch* batch, int set);
 /* Set one tile's settings by hand: persistent_workgroups (0 = full chip) and
  * conv_rows in {0 (auto), 4, 8, 16}. */
 int cageo_batch_configure_tile(cageo_batch* batch, int tile, int persistent_workgroups,

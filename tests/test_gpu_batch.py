@@ -134,3 +134,49 @@ def test_batch_tile_outside_domain(torch_cuda, tiles):
                 single = torch.zeros((h, w), dtype=torch.float32, device="cuda")
                 ctx.run_realizations(single, 2, 8, 0)
             assert torch.equal(mcs[1], single), mode
+
+
+def test_batch_64_uneven_tiles_graph_vs_oracle(torch_cuda):
+    """The batched workload: 64 tiles with sides drawn U(256, 2048), error
+    RMSE U(0.5, 3) m and practical range U(30, 200) m (Gaussian filter), as
+    the benchmark draws them; two realizations replayed as one hipGraph of
+    all-tile launches.  Every tile's counter raster must equal the oracle's
+    two realizations of that tile."""
+    import cageo_amd as ca
+    from concurrent.futures import ThreadPoolExecutor
+    from helpers import oracle_counts_parallel
+    torch = torch_cuda
+    seed = 20240601
+    r = np.random.default_rng(seed + 3)
+    n = 64
+    shapes = [tuple(int(v) for v in r.integers(256, 2049, size=2)) for _ in range(n)]
+    rmse = r.uniform(0.5, 3.0, n)
+    pr = r.uniform(30.0, 200.0, n)
+    announce("batch 64 tiles U(256,2048): generating DEMs")
+    with ThreadPoolExecutor(16) as ex:
+        dems = list(ex.map(lambda k: ca.synth_dem(shapes[k][1], shapes[k][0], seed + 100 + k,
+                                                  10.0, 2000, float(NODATA)), range(n)))
+    reals = 2
+    announce("batch 64 tiles U(256,2048): graph mode x2, cells", sum(h * w for h, w in shapes))
+    with ca.Batch(shapes) as b:
+        dev = [(torch.from_numpy(d).cuda(), torch.from_numpy(s).cuda()) for d, s in dems]
+        for t, (dd, ds) in enumerate(dev):
+            b.setup_tile(t, dd, ds, float(NODATA), float(NODATA), 2, ca.GAUSSIAN, float(pr[t]),
+                         10.0, 0.0, float(rmse[t]), seed + 1000 + t)
+        mcs = [torch.zeros(s, dtype=torch.float32, device="cuda") for s in shapes]
+        b.run(mcs, reals, 0, ca.BATCH_GRAPH)
+        got = [m.cpu().numpy() for m in mcs]
+        info = b.info()
+        print("all-tile launch", info, flush=True)
+        assert info["blocks"] == sum(((h + 63) // 64) * ((w + 63) // 64) for h, w in shapes)
+    announce("batch 64 tiles: oracle in worker processes")
+    jobs = []
+    for t in range(n):
+        rad = ca.filter_radius(float(pr[t]), 10.0)
+        taps = ca.create_filter(ca.GAUSSIAN, float(pr[t]), 10.0, rad)
+        jobs.append((dems[t][0], dems[t][1], taps, rad, ca.GAUSSIAN, float(rmse[t]),
+                     seed + 1000 + t, 0, reals))
+    refs = oracle_counts_parallel(jobs)
+    for t in range(n):
+        assert np.array_equal(got[t], refs[t]), \
+            f"tile {t} {shapes[t]}: {int((got[t] != refs[t]).sum())} counters differ"
