@@ -3,16 +3,19 @@
 // this generator instead:
 //   DEM  = 100 m + 0.005*cell*x + 40 Gaussian hills + N(0, 0.3^2)
 //   streams: D8 flow accumulation > threshold on the 5x5 box-smoothed DEM,
-//            filled by priority flood with epsilon steps (Barnes, Lehman and
+//            filled with epsilon steps (the result of Barnes, Lehman and
 //            Mulla 2014, "Priority-flood: an optimal depression-filling and
-//            watershed-labeling algorithm", the +epsilon variant), one float
-//            id >= 1 per stream link (a new id at every channel head and
-//            confluence), nodata elsewhere.
+//            watershed-labeling algorithm", +epsilon variant, computed by the
+//            relaxation of Planchon and Darboux 2001, "A fast, simple and
+//            versatile algorithm to fill the depressions of digital elevation
+//            models"), one float id >= 1 per stream link (a new id at every
+//            channel head and confluence, in decreasing elevation order),
+//            nodata elsewhere.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <queue>
 #include <thread>
 #include <vector>
 
@@ -56,6 +59,32 @@ void parallel_rows(int rows, F body) {
                 body(r);
         });
     for (auto& x : th) x.join();
+}
+
+// Row bands [r0, r1) of `rows` over the host's hardware threads.
+template <class F>
+void parallel_bands(int rows, F body) {
+    int nt = (int)std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    nt = std::max(1, std::min(nt, rows / 8));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([=, &body]() {
+            body((int)((int64_t)rows * t / nt), (int)((int64_t)rows * (t + 1) / nt));
+        });
+    for (auto& x : th) x.join();
+}
+
+// Relaxed atomic access to a float shared between the band threads.
+inline float load_relaxed(const float* p) {
+    const uint32_t b = __atomic_load_n(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED);
+    float f;
+    std::memcpy(&f, &b, sizeof(f));
+    return f;
+}
+inline void store_relaxed(float* p, float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, sizeof(b));
+    __atomic_store_n(reinterpret_cast<uint32_t*>(p), b, __ATOMIC_RELAXED);
 }
 
 const int DX8[8] = {0, 1, 1, 1, 0, -1, -1, -1};
@@ -127,32 +156,60 @@ extern "C" int cageo_synth_dem(int W, int H, uint64_t seed, float cell, int acc_
         }
     });
 
-    // Priority-flood + epsilon from the grid edge: every interior cell ends
-    // strictly above the neighbour that reached it.
-    typedef std::pair<float, uint32_t> Item;
-    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
-    std::vector<uint8_t> seen(n, 0);
-    for (int y = 0; y < H; ++y)
-        for (int x = 0; x < W; ++x)
-            if (x == 0 || y == 0 || x == W - 1 || y == H - 1) {
-                size_t i = (size_t)y * W + x;
-                seen[i] = 1;
-                pq.push(Item(sm[i], (uint32_t)i));
+    // Depression filling with epsilon steps from the grid edge: every interior
+    // cell ends strictly above a neighbour.  The result is that of the
+    // priority flood with epsilon steps (Barnes, Lehman and Mulla 2014, the
+    // +epsilon variant: cells leave a priority queue in (elevation, index)
+    // order and a cell takes max(z, nextafter(z of the neighbour that reached
+    // it))), which is the greatest fixed point of
+    //   F(n) = max(sm(n), nextafter(min over the 8 neighbours c of F(c)))
+    // with the edge cells at sm: the neighbour that reaches n first is its
+    // lowest one, and by induction over the queue order no fixed point lies
+    // above it.  That fixed point is computed here by the Planchon and
+    // Darboux (2001) relaxation from +infinity: sweeps in four raster
+    // orders, each over bands of rows in parallel, until a round of sweeps
+    // changes nothing.  Values only decrease and the map is monotone, so the
+    // order of the updates (and reads of a neighbour band's rows while it
+    // sweeps) does not change the limit.
+    {
+        std::vector<float> w(n, INFINITY);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x)
+                if (x == 0 || y == 0 || x == W - 1 || y == H - 1) w[(size_t)y * W + x] = sm[(size_t)y * W + x];
+        for (;;) {
+            std::atomic<long long> changed{0};
+            for (int dir = 0; dir < 4; ++dir) {
+                const bool down = dir < 2, right = (dir & 1) == 0;
+                parallel_bands(H - 2, [&](int r0, int r1) {
+                    long long ch = 0;
+                    for (int k = 0; k < r1 - r0; ++k) {
+                        const int y = 1 + (down ? r0 + k : r1 - 1 - k);
+                        for (int j = 0; j < W - 2; ++j) {
+                            const int x = right ? 1 + j : W - 2 - j;
+                            const size_t i = (size_t)y * W + x;
+                            const float cur = load_relaxed(&w[i]);
+                            if (cur == sm[i]) continue;  // cannot go lower
+                            float m = load_relaxed(&w[i - W - 1]);
+                            m = std::min(m, load_relaxed(&w[i - W]));
+                            m = std::min(m, load_relaxed(&w[i - W + 1]));
+                            m = std::min(m, load_relaxed(&w[i - 1]));
+                            m = std::min(m, load_relaxed(&w[i + 1]));
+                            m = std::min(m, load_relaxed(&w[i + W - 1]));
+                            m = std::min(m, load_relaxed(&w[i + W]));
+                            m = std::min(m, load_relaxed(&w[i + W + 1]));
+                            const float v = std::max(sm[i], std::nextafter(m, INFINITY));
+                            if (v < cur) {
+                                store_relaxed(&w[i], v);
+                                ++ch;
+                            }
+                        }
+                    }
+                    changed += ch;
+                });
             }
-    while (!pq.empty()) {
-        Item it = pq.top();
-        pq.pop();
-        int cx = (int)(it.second % W), cy = (int)(it.second / W);
-        for (int k = 0; k < 8; ++k) {
-            int nx = cx + DX8[k], ny = cy + DY8[k];
-            if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
-            size_t ni = (size_t)ny * W + nx;
-            if (seen[ni]) continue;
-            seen[ni] = 1;
-            float lift = std::nextafter(it.first, INFINITY);
-            if (sm[ni] < lift) sm[ni] = lift;
-            pq.push(Item(sm[ni], (uint32_t)ni));
+            if (changed.load() == 0) break;
         }
+        sm.swap(w);
     }
 
     // D8 steepest descent on the filled surface; edge cells drain out.
@@ -176,17 +233,36 @@ extern "C" int cageo_synth_dem(int W, int H, uint64_t seed, float cell, int acc_
         }
     });
 
-    // Flow accumulation in decreasing elevation order.
-    std::vector<uint32_t> order(n);
-    for (size_t i = 0; i < n; ++i) order[i] = (uint32_t)i;
+    // Flow accumulation: a cell passes its total downstream once all its
+    // donors have (a topological order of the D8 tree; the sums do not depend
+    // on which order).
+    std::vector<uint32_t> acc(n, 1);
+    {
+        std::vector<uint8_t> donors(n, 0);
+        for (size_t i = 0; i < n; ++i)
+            if (down[i] >= 0) ++donors[down[i]];
+        std::vector<uint32_t> ready;
+        ready.reserve(1 << 16);
+        for (size_t i = 0; i < n; ++i)
+            if (donors[i] == 0) ready.push_back((uint32_t)i);
+        while (!ready.empty()) {
+            const uint32_t i = ready.back();
+            ready.pop_back();
+            const int32_t d = down[i];
+            if (d < 0) continue;
+            acc[d] += acc[i];
+            if (--donors[d] == 0) ready.push_back((uint32_t)d);
+        }
+    }
+
+    // Stream cells and link ids (upstream first): the stream cells in
+    // decreasing elevation order, ties by index.
+    std::vector<uint32_t> order;
+    for (size_t i = 0; i < n; ++i)
+        if (acc[i] > (uint32_t)acc_threshold) order.push_back((uint32_t)i);
     std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         return sm[a] > sm[b] || (sm[a] == sm[b] && a < b);
     });
-    std::vector<uint32_t> acc(n, 1);
-    for (uint32_t i : order)
-        if (down[i] >= 0) acc[down[i]] += acc[i];
-
-    // Stream cells and link ids (upstream first).
     std::vector<uint8_t> nup(n, 0);
     for (size_t i = 0; i < n; ++i)
         if (acc[i] > (uint32_t)acc_threshold && down[i] >= 0 &&
@@ -195,7 +271,6 @@ extern "C" int cageo_synth_dem(int W, int H, uint64_t seed, float cell, int acc_
     std::vector<float> id(n, 0.0f);
     float next_id = 1.0f;
     for (uint32_t i : order) {
-        if (acc[i] <= (uint32_t)acc_threshold) continue;
         if (nup[i] != 1) id[i] = next_id++;  // head or confluence
         if (down[i] >= 0 && acc[down[i]] > (uint32_t)acc_threshold && nup[down[i]] == 1)
             id[down[i]] = id[i];
