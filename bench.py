@@ -84,10 +84,14 @@ def parse():
     p.add_argument("--batch-tiles", type=int, default=64,
                    help="tiles of the batched workload (0: skip)")
     p.add_argument("--batch-realizations", type=int, default=100)
-    p.add_argument("--large", action="store_true",
-                   help="also measure the large-tile configuration (16384^2, exponential PR "
-                        "200 m, RMSE 2.5 m): Monte Carlo vs surrogate + refinement (minutes)")
+    p.add_argument("--large-size", type=int, default=16384,
+                   help="side of the large-tile configuration (exponential PR 200 m, RMSE "
+                        "2.5 m): Monte Carlo vs surrogate + windowed check + refinement; 0 skips")
     p.add_argument("--large-realizations", type=int, default=1000)
+    p.add_argument("--large-window", type=int, default=1024)
+    p.add_argument("--large-check", type=int, default=8,
+                   help="realizations of the large tile's windowed check")
+    p.add_argument("--large-refine", type=int, default=200)
     p.add_argument("--large-weights",
                    default=os.path.join(ROOT, "cageo_amd", "weights", "surrogate_exp_b32.bin"),
                    help="surrogate weights for the exponential filter (.bin)")
@@ -95,10 +99,8 @@ def parse():
                    help="Monte Carlo size the surrogate is judged against (0: skip)")
     p.add_argument("--traffic-out", default=None,
                    help="keep the counter passes' CSV files in this directory")
-    p.add_argument("--traffic", action="store_true",
-                   help="run the two rocprofv3 counter passes for roofline.traffic (off by "
-                        "default: the profiler can crash in its own exit path, so the passes "
-                        "are normally run separately and summarized in profiles/)")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 counter passes behind roofline.traffic")
     p.add_argument("--probe", action="store_true",
                    help="internal: the short run profiled by the counter passes")
     return p.parse_args()
@@ -179,7 +181,7 @@ def counter_pass(counter, args, outdir):
            "--seed", str(args.seed)]
     try:
         # rocprofv3 may crash at its own exit after writing complete results
-        res = subprocess.run(cmd, capture_output=True, text=True, timeout=60, check=False)
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=120, check=False)
     except subprocess.TimeoutExpired:
         return None, f"{counter}: profiler pass timed out"
     files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -279,32 +281,33 @@ def batched_leg(torch, ca, args):
 
 
 def large_leg(torch, ca, args):
-    """Large tile, both growth patterns: a 16384^2 DEM (the generator's
-    8192^2 DEM, seed --seed, mirrored across x and y: continuous heights and
-    streams at the seams), exponential filter PR 200 m, RMSE 2.5 m.
-    - Monte Carlo: --large-realizations realizations with the fast
-      convolution (within 1e-6 of the exact field), run in chunks of 50 (one
-      progress line each); its frequency map is the reference.  The exact
-      order is timed on 10 realizations beside.
-    - Surrogate (exponential weights) + check on 1024^2 windows with 4
-      realizations; windows whose L1 exceeds the sampling noise + 0.01 are
-      refined by a 200-realization Monte Carlo of the whole DEM.
+    """Large tile, both growth patterns: the generator's --large-size^2 DEM
+    (seed --seed), exponential filter PR 200 m, RMSE 2.5 m.
+    - Monte Carlo: --large-realizations realizations in the fast
+      convolution mode (the FFT of the torus, within 1e-6 of the exact
+      field), in chunks of 50 with one progress line each; its frequency map
+      is the reference.  The exact order is timed on 10 realizations beside.
+    - Surrogate (exponential weights) through the C ABI: prediction, a
+      --large-check realization check on --large-window windows (unbiased
+      mean signed error <= 0.01 and RMS error <= 1.2 x 2.36 x 0.02 per
+      window), and refinement of the failing windows by a --large-refine
+      realization Monte Carlo of the whole DEM.
     Times are wall times with the device synchronised."""
     import numpy as np
+    from cageo_amd import surrogate as sg
     log = lambda m: print(f"large: {m}", file=sys.stderr, flush=True)
-    half = 8192
+    size = args.large_size
     t0 = time.perf_counter()
-    d0, s0 = ca.synth_dem(half, half, args.seed, 10.0, 2000, NODATA)
-    log(f"synthetic {half}^2 in {time.perf_counter() - t0:.1f} s")
-    dem = np.block([[d0, d0[:, ::-1]], [d0[::-1, :], d0[::-1, ::-1]]])
-    st = np.block([[s0, s0[:, ::-1]], [s0[::-1, :], s0[::-1, ::-1]]])
-    del d0, s0
+    dem, st = ca.synth_dem(size, size, args.seed, 10.0, 2000, NODATA)
+    gen_s = time.perf_counter() - t0
+    log(f"synthetic {size}^2 in {gen_s:.1f} s")
     H, W = dem.shape
-    ddem, dst = torch.from_numpy(np.ascontiguousarray(dem)).cuda(), torch.from_numpy(np.ascontiguousarray(st)).cuda()
+    ddem, dst = torch.from_numpy(dem).cuda(), torch.from_numpy(st).cuda()
+    del dem, st
     r = ca.filter_radius(200.0, 10.0)
     taps = ca.create_filter(ca.EXPONENTIAL, 200.0, 10.0, r)
-    out = {"dem": f"{W}x{H} (8192^2 generator DEM mirrored)", "filter": "exponential PR 200 m",
-           "error_std_m": 2.5}
+    out = {"dem": f"{W}x{H}", "generator_s": round(gen_s, 1), "filter": "exponential PR 200 m",
+           "error_std_m": 2.5, "dem_seed": args.seed}
     seed = args.seed + 11
     n = args.large_realizations
     with ca.Context(W, H, torch.cuda.current_stream()) as ctx:
@@ -321,6 +324,7 @@ def large_leg(torch, ca, args):
                                  "est_1000_s": round(1000 * exact_per, 1), "timed": 10}
         log(f"exact order {1 / exact_per:.2f} realizations/s")
         ctx.set_conv_mode(ca.CONV_FAST)
+        terms = ctx.conv_terms()
         mc = torch.zeros((H, W), dtype=torch.float32, device="cuda")
         ctx.run_realizations(scratch, 1, seed, 0)  # warm-up
         torch.cuda.synchronize()
@@ -331,12 +335,19 @@ def large_leg(torch, ca, args):
             log(f"Monte Carlo {min(k + 50, n)}/{n} ({time.perf_counter() - t:.1f} s)")
         mc_s = time.perf_counter() - t
         ref = mc / float(n)
-        out["mc"] = {"realizations": n, "convolution": f"fast ({ctx.conv_terms()} terms)",
+        out["mc"] = {"realizations": n,
+                     "convolution": "fast: FFT of the torus" if terms == -1 else
+                                    f"fast: low-rank, {terms} terms",
+                     "note": "the 1000-realization reference runs in the fast convolution "
+                             "mode (within 1e-6 of the exact field); the exact order is "
+                             "timed beside",
                      "s": round(mc_s, 2), "realizations_per_s": round(n / mc_s, 3),
                      "mcells_per_s": round(W * H * n / mc_s / 1e6, 1)}
-        if args.large_weights and os.path.exists(args.large_weights):
-            K, win, tol = 4, 1024, 0.01
-            with ca.Surrogate(args.large_weights, torch.cuda.current_stream()) as s:
+        weights = os.path.join(ROOT, "cageo_amd", "weights", "surrogate_exp_b32.bin")
+        if os.path.exists(weights):
+            win = args.large_window
+            bias_tol, rms_tol = sg.check_tolerances(0.02)
+            with ca.Surrogate(weights, torch.cuda.current_stream()) as s:
                 p = torch.empty((H, W), dtype=torch.float32, device="cuda")
                 s.predict(ddem[:1024, :1024].contiguous(), dst[:1024, :1024].contiguous(), NODATA,
                           NODATA, 2.5, 200.0, p[:1024, :1024].contiguous())  # warm-up
@@ -345,91 +356,96 @@ def large_leg(torch, ca, args):
                 s.predict(ddem, dst, NODATA, NODATA, 2.5, 200.0, p)
                 torch.cuda.synchronize()
                 pred_s = time.perf_counter() - t
-                cnt = torch.zeros((H, W), dtype=torch.float32, device="cuda")
-                ctx.run_realizations(cnt, K, seed + 1, 0)
-                f = cnt / K
-                qd = p.double()
-                noise = sum(math.comb(K, j) * qd ** j * (1 - qd) ** (K - j) * (j / K - qd).abs()
-                            for j in range(K + 1))
-                l1w = (p - f).abs().reshape(H // win, win, W // win, win).mean(dim=(1, 3))
-                nw = noise.reshape(H // win, win, W // win, win).mean(dim=(1, 3))
-                bad = (l1w > nw + tol)
-                torch.cuda.synchronize()
-                check_s = time.perf_counter() - t - pred_s
-                nbad = int(bad.sum())
-                final = p.clone()
-                refine_s = 0.0
-                if nbad:
-                    t2 = time.perf_counter()
-                    c200 = torch.zeros((H, W), dtype=torch.float32, device="cuda")
-                    ctx.run_realizations(c200, 200, seed + 2, 0)
-                    m = bad.repeat_interleave(win, 0).repeat_interleave(win, 1)
-                    final = torch.where(m, c200 / 200.0, p)
-                    torch.cuda.synchronize()
-                    refine_s = time.perf_counter() - t2
-                sur_s = pred_s + check_s + refine_s
-                log(f"surrogate {sur_s:.2f} s, {nbad} of {bad.numel()} windows refined")
-                out["surrogate"] = {
-                    "weights": os.path.relpath(os.path.abspath(args.large_weights), ROOT),
-                    "predict_s": round(pred_s, 3), "check_s": round(check_s, 3),
-                    "windows": int(bad.numel()), "windows_refined": nbad,
-                    "refine_s": round(refine_s, 2), "total_s": round(sur_s, 2),
-                    "speedup_vs_mc": round(mc_s / sur_s, 1),
-                    "l1_prediction": round(float((p - ref).abs().mean()), 5),
-                    "l1_final": round(float((final - ref).abs().mean()), 5),
-                    "max_dev_final": round(float((final - ref).abs().max()), 4)}
+            l1_pred = float((p - ref).abs().mean())
+            t = time.perf_counter()
+            acc, bias, rms = ctx.check_probability_map(p, args.large_check, seed + 1, bias_tol,
+                                                       rms_tol, scratch, window=win)
+            check_s = time.perf_counter() - t
+            nbad = int((~acc).sum())
+            t = time.perf_counter()
+            ctx.refine_probability_map(p, acc, args.large_refine, seed + 2, scratch, window=win)
+            torch.cuda.synchronize()
+            refine_s = time.perf_counter() - t
+            sur_s = pred_s + check_s + refine_s
+            log(f"surrogate {sur_s:.2f} s, {nbad} of {acc.size} windows refined")
+            dev = (p - ref).abs()
+            wl1 = dev.reshape(H // win, win, W // win, win).mean(dim=(1, 3)).cpu().numpy()
+            out["surrogate"] = {
+                "weights": os.path.relpath(weights, ROOT), "window": win,
+                "check_realizations": args.large_check, "bias_tol": bias_tol,
+                "rms_tol": round(rms_tol, 5), "predict_s": round(pred_s, 3),
+                "check_s": round(check_s, 3), "windows": int(acc.size),
+                "windows_refined": nbad, "refine_realizations": args.large_refine,
+                "refine_s": round(refine_s, 2), "total_s": round(sur_s, 2),
+                "speedup_vs_mc": round(mc_s / sur_s, 1),
+                "l1_prediction": round(l1_pred, 5), "l1_final": round(float(dev.mean()), 5),
+                "max_window_l1_final": round(float(wl1.max()), 5),
+                "max_dev_final": round(float(dev.max()), 4),
+                "check_rms_max": round(float(rms.max()), 5),
+                "check_bias_absmax": round(float(np.abs(bias).max()), 5)}
     return out
 
 
-def surrogate_leg(torch, ca, dem, stream, args):
-    """Checked surrogate prediction against an N-realization Monte Carlo of the
-    same DEM and error model: mean absolute and tail per-cell deviation of the
-    border probability, and the wall time of each (device work bracketed by
-    synchronisations; the Monte Carlo is the batched graph form)."""
+def surrogate_leg(torch, ca, dem, stream, tdem, tst, args):
+    """The packaged network (C-ABI inference from the resident DEM) with its
+    Monte Carlo check, against an N-realization Monte Carlo of the same DEM
+    and error model: mean absolute and tail per-cell deviation of the border
+    probability, and the wall time of each (device work bracketed by
+    synchronisations; both with the context already set up)."""
     from cageo_amd import surrogate as sg
     n = args.surrogate_realizations
-    net = sg.load_net(torch)
-    # warm-up: kernels, allocator and graph code paths on a 512^2 crop
-    sg.checked_prediction(torch, net, dem[:512, :512], stream[:512, :512], 1.8, 60.0, 1,
-                          check_realizations=2, fallback_realizations=2)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    p_mc = sg.mc_frequency(torch, [dem], [stream], [1.8], [60.0], n, args.seed,
-                           on_device=True)[0]
-    torch.cuda.synchronize()
-    mc_s = time.perf_counter() - t0
-    q, info = sg.checked_prediction(torch, net, dem, stream, 1.8, 60.0, args.seed + 7,
-                                    fallback_realizations=n)
-    sur_s = info["predict_s"] + info["check_s"] + info.get("fallback_s", 0.0)
-    # the network alone: forward passes timed with events on its stream
-    h, w = dem.shape
-    x = sg.device_inputs(torch, dem, stream, 1.8, 60.0)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 5
-    net.predict(x)
-    e0.record()
-    for _ in range(reps):
-        net.predict(x)
-    e1.record()
-    torch.cuda.synchronize()
-    fwd_ms = e0.elapsed_time(e1) / reps
-    tflops = sg.forward_flops(h, w) / (fwd_ms * 1e-3) / 1e12
-    dev = (q - p_mc).abs()
+    size = dem.shape[0]
+    sh = torch.cuda.current_stream()
+    r = ca.filter_radius(60.0, 10.0)
+    with ca.Context(size, size, sh) as ctx, ca.Surrogate(sg.DEFAULT_WEIGHTS, sh) as s:
+        ctx.set_filter(ca.GAUSSIAN, r, ca.create_filter(ca.GAUSSIAN, 60.0, 10.0, r))
+        ctx.mc_setup(tdem, tst, NODATA, NODATA, 2, 0.0, 1.8)
+        cnt = torch.zeros((size, size), dtype=torch.float32, device="cuda")
+        ctx.run_realizations(cnt, 2, args.seed, 0)  # warm-up
+        cnt.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.run_realizations(cnt, n, args.seed, 0)
+        torch.cuda.synchronize()
+        mc_s = time.perf_counter() - t0
+        p_mc = cnt / float(n)
+        q = torch.empty((size, size), dtype=torch.float32, device="cuda")
+        s.predict(tdem, tst, NODATA, NODATA, 1.8, 60.0, q)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.predict(tdem, tst, NODATA, NODATA, 1.8, 60.0, q)
+        torch.cuda.synchronize()
+        pred_s = time.perf_counter() - t0
+        out, info = sg.checked_prediction(torch, None, dem, stream, 1.8, 60.0, args.seed + 7,
+                                          fallback_realizations=n, prediction=q, ctx=ctx)
+        sur_s = pred_s + info["check_s"] + info.get("fallback_s", 0.0)
+        # the prediction alone, timed with events on its stream
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record(sh)
+        for _ in range(reps):
+            s.predict(tdem, tst, NODATA, NODATA, 1.8, 60.0, q)
+        e1.record(sh)
+        torch.cuda.synchronize()
+        fwd_ms = e0.elapsed_time(e1) / reps
+    tflops = sg.forward_flops(size, size) / (fwd_ms * 1e-3) / 1e12
+    dev = (out - p_mc).abs()
     border = p_mc > 0
     return {
         "mc_realizations": n, "mc_s": round(mc_s, 4), "surrogate_s": round(sur_s, 4),
-        "speedup_vs_mc": round(mc_s / sur_s, 1),
+        "predict_s": round(pred_s, 4), "speedup_vs_mc": round(mc_s / sur_s, 1),
         "l1": round(float(dev.mean()), 5),
         "l1_border_cells": round(float(dev[border].mean()), 5) if bool(border.any()) else 0.0,
         "p999_dev": round(float(torch.quantile(dev.flatten()[::16].float(), 0.999)), 4),
         "max_dev": round(float(dev.max()), 4),
         "check": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in info.items()},
         "weights": os.path.relpath(sg.DEFAULT_WEIGHTS, ROOT),
-        "network_forward_ms": round(fwd_ms, 3),
+        "predict_ms": round(fwd_ms, 3),
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 5),
-                     "flops_per_forward": sg.forward_flops(h, w),
-                     "note": "BF16 MFMA convolutions of one 4096x4096 forward pass"},
+                     "flops_per_forward": sg.forward_flops(size, size),
+                     "note": "BF16 MFMA convolutions of one 4096x4096 prediction (input planes "
+                             "and crop included in the time)"},
     }
 
 
@@ -511,8 +527,9 @@ def main():
         kernel = STAGE_KERNEL[key]
         bytes_per_launch = STAGE_BYTES[key] * cells
         achieved = bytes_per_launch / (stages[key] * 1e-3) / 1e9
-        traffic, why = measure_traffic(args, kernel) if args.traffic else (None, "not requested")
-        if traffic is None and args.traffic:
+        traffic, why = (measure_traffic(args, kernel) if not args.no_traffic
+                        else (None, "not requested"))
+        if traffic is None and not args.no_traffic:
             print(f"bench.py: traffic not measured: {why}", file=sys.stderr)
         line["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
@@ -526,6 +543,19 @@ def main():
         if traffic:
             line["roofline"]["traffic_detail"] = {
                 k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in traffic.items()}
+        # per stage and end to end: algorithmic bytes of every stage per
+        # realization over the measured time (stage events; whole step)
+        line["roofline_stages"] = {
+            k: {"gb_s": round(STAGE_BYTES[k] * cells / (stages[k] * 1e-3) / 1e9, 1),
+                "frac": round(STAGE_BYTES[k] * cells / (stages[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+            for k in STAGE_NAMES if stages.get(k)}
+    e2e_bytes = sum(STAGE_BYTES.values()) * cells
+    e2e = e2e_bytes / (dt / args.steps) / 1e9
+    line["roofline_end_to_end"] = {
+        "bound": "hbm", "achieved": round(e2e, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(e2e / HBM_PEAK_GBS, 5), "bytes_per_realization": e2e_bytes,
+        "note": "80 B/cell per realization (one pass over every stage's boundary I/O at the "
+                "reference's data types, fused stages counted unfused) over the whole step"}
     if args.exp_steps > 0:
         dte, st_e, _ = run(ca.EXPONENTIAL, 200.0, args.exp_steps, 1, not args.no_stage_times)
         line["exponential_pr200"] = {
@@ -560,8 +590,8 @@ def main():
     if args.batch_tiles > 0:
         line["batched"] = batched_leg(torch, ca, args)
     if args.surrogate_realizations > 0:
-        line["surrogate"] = surrogate_leg(torch, ca, dem, stream, args)
-    if args.large:
+        line["surrogate"] = surrogate_leg(torch, ca, dem, stream, tdem, tst, args)
+    if args.large_size > 0:
         line["large_tile"] = large_leg(torch, ca, args)
     if args.cpu_sample > 0 and oracle_ok:
         cps, secs = cpu_baseline(size, args.cpu_sample, args.seed)

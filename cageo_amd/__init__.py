@@ -131,9 +131,13 @@ def lib():
         "cageo_surrogate_destroy": ([vp], i),
         "cageo_surrogate_forward": ([vp, vp, i, i, vp], i),
         "cageo_surrogate_predict": ([vp, vp, vp, i, i, f, f, f, f, vp], i),
-        "cageo_surrogate_check": ([vp, vp, vp, i, u64, ctypes.c_double, vp,
+        "cageo_surrogate_check": ([vp, vp, vp, i, u64, ctypes.c_double, ctypes.c_double, vp,
                                    ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)], i),
+        "cageo_check_windows": ([vp, i, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "cageo_check_probability_map": ([vp, vp, i, u64, i, ctypes.c_double, ctypes.c_double,
+                                         vp, vp, vp, vp], i),
+        "cageo_refine_probability_map": ([vp, vp, i, vp, i, u64, vp], i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -319,13 +323,14 @@ class Surrogate:
                                              nodata_stream, rmse, practical_range, _ptr(out)),
                "surrogate_predict")
 
-    def check(self, ctx, prob, k, seed, tolerance, scratch):
-        """(accepted, l1, noise_l1) of a k-realization Monte Carlo check."""
-        acc, l1, noise = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
-        _check(lib().cageo_surrogate_check(self._h, ctx._h, _ptr(prob), k, seed, tolerance,
-                                           _ptr(scratch), ctypes.byref(acc), ctypes.byref(l1),
-                                           ctypes.byref(noise)), "surrogate_check")
-        return bool(acc.value), l1.value, noise.value
+    def check(self, ctx, prob, k, seed, bias_tol, rms_tol, scratch):
+        """(accepted, bias, rms) of a k-realization Monte Carlo check of the
+        whole map (see Context.check_probability_map)."""
+        acc, b, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().cageo_surrogate_check(self._h, ctx._h, _ptr(prob), k, seed, bias_tol,
+                                           rms_tol, _ptr(scratch), ctypes.byref(acc),
+                                           ctypes.byref(b), ctypes.byref(r)), "surrogate_check")
+        return bool(acc.value), b.value, r.value
 
 
 def read_header(path: str) -> dict:
@@ -450,6 +455,37 @@ class Context:
     def configure(self, persistent_workgroups=0, conv_rows=0):
         _check(lib().cageo_context_configure(self._h, persistent_workgroups, conv_rows),
                "context_configure")
+
+    def check_windows(self, window=0):
+        """(windows along x, windows along y) of a window size."""
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        _check(lib().cageo_check_windows(self._h, window, ctypes.byref(nx), ctypes.byref(ny)),
+               "check_windows")
+        return nx.value, ny.value
+
+    def check_probability_map(self, prob, k, seed, bias_tol, rms_tol, scratch, window=0):
+        """Monte Carlo check of a probability map with k realizations of this
+        context: per window (row-major; window 0 = the whole raster) the
+        unbiased mean signed error and RMS error estimates and the verdict
+        |bias| <= bias_tol and rms <= rms_tol.  Returns numpy arrays
+        (accepted, bias, rms) of shape (windows along y, windows along x)."""
+        nx, ny = self.check_windows(window)
+        acc = np.zeros(nx * ny, np.int32)
+        b = np.zeros(nx * ny, np.float64)
+        r = np.zeros(nx * ny, np.float64)
+        _check(lib().cageo_check_probability_map(self._h, _ptr(prob), k, seed, window, bias_tol,
+                                                 rms_tol, _ptr(scratch), acc.ctypes.data,
+                                                 b.ctypes.data, r.ctypes.data),
+               "check_probability_map")
+        return acc.reshape(ny, nx).astype(bool), b.reshape(ny, nx), r.reshape(ny, nx)
+
+    def refine_probability_map(self, prob, accepted, n, seed, scratch, window=0):
+        """Replace the windows of `prob` whose flag in `accepted` is False by
+        the frequency of an n-realization Monte Carlo of the whole DEM."""
+        acc = np.ascontiguousarray(np.asarray(accepted).ravel(), np.int32)
+        _check(lib().cageo_refine_probability_map(self._h, _ptr(prob), window, acc.ctypes.data,
+                                                  n, seed, _ptr(scratch)),
+               "refine_probability_map")
 
     def abort(self, set_=True):
         """Raise (or clear) the abort word: persistent launches end at their

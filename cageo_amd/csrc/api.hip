@@ -45,10 +45,11 @@ int alloc(T** p, size_t count) {
 void free_all(Context* c) {
     void* ptrs[] = {c->d_mark, c->d_mark_pf, c->d_topen, c->d_tcost, c->d_rows, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_filter,
                     c->d_sflags, c->d_slabel, c->d_off, c->d_factor, c->d_a, c->d_b, c->d_z,
-                    c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr,
+                    c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr, c->d_bent,
                     c->d_lr_col, c->d_lr_row, c->d_lr_planes};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    release_fft(c);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_abort) (void)hipHostFree(c->h_abort);
     for (hipEvent_t& e : c->ev)
@@ -145,6 +146,17 @@ int check_status(Context* c) {
 // column taps padded to whole register-ring turns, row taps lambda_k v_k.
 int prepare_low_rank(Context* c) {
     if (c->model != CAGEO_FILTER_EXPONENTIAL || c->h_taps.empty()) return CAGEO_OK;
+    // the FFT form where the grid suits it; the low-rank form otherwise
+    const int fst = prepare_fft(c);
+    if (fst == CAGEO_OK) {
+        if (c->d_lr_col) (void)hipFree(c->d_lr_col);
+        if (c->d_lr_row) (void)hipFree(c->d_lr_row);
+        if (c->d_lr_planes) (void)hipFree(c->d_lr_planes);
+        c->d_lr_col = c->d_lr_row = c->d_lr_planes = nullptr;
+        c->lr_k = 0;
+        return CAGEO_OK;
+    }
+    if (fst != CAGEO_ERR_RANGE) return fst;
     const int r = c->radius, L = 2 * r + 1, KMAX = 32;
     int LC, LP, S;
     size_t lc, lr;
@@ -246,6 +258,7 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
     A(cageo::alloc(&c->d_dist, n));
     A(cageo::alloc(&c->d_lab, n));
     A(cageo::alloc(&c->d_ptr, n));
+    A(cageo::alloc(&c->d_bent, 256 * (size_t)c->ntiles));
     if (st == CAGEO_OK &&
         hipHostMalloc(reinterpret_cast<void**>(&c->h_scal), cageo::SC_NSLOTS * sizeof(uint32_t)) !=
             hipSuccess)
@@ -349,7 +362,8 @@ int cageo_context_set_conv_mode(cageo_context* ctx, int mode) {
 int cageo_context_conv_terms(cageo_context* ctx, int* terms) {
     if (!ctx || !terms) return CAGEO_ERR_ARG;
     Context* c = &ctx->c;
-    *terms = (c->conv_fast && c->model == CAGEO_FILTER_EXPONENTIAL) ? c->lr_k : 0;
+    *terms = (c->conv_fast && c->model == CAGEO_FILTER_EXPONENTIAL) ? (c->fft_ready ? -1 : c->lr_k)
+                                                                     : 0;
     return CAGEO_OK;
 }
 

@@ -64,6 +64,7 @@ struct Context {
     uint64_t bar_limit = 0;         // bound of one barrier wait, wall_clock64() ticks
     double bar_timeout_s = 0.0;
     int pf_grid = 0, fr_grid = 0, tr_grid = 0;
+    int pf_res_grid = 0;  // co-resident workgroups of the pit fill's resident rounds
     int conv_ty = 0;  // tuned row-block height of the 2-D convolution (0 = largest that fits)
     // Fast mode of the 2-D convolution: the filter's low-rank separable form.
     int conv_fast = 0;
@@ -72,6 +73,11 @@ struct Context {
     float* d_lr_row = nullptr;        // K x LP row taps (lambda_k v_k)
     float* d_lr_planes = nullptr;     // K column-pass planes of W x H
     std::vector<float> h_taps;        // the 2-D filter (host copy)
+    // ... or its FFT form on power-of-two grids (convfft.hip)
+    bool fft_ready = false;
+    float2* d_fft_spec = nullptr;     // H x (W/2 + 1) half spectra
+    float* d_fft_q = nullptr;         // (W/2 + 1) x (H/2 + 1) filter spectrum (scaled)
+    float2* d_fft_tw = nullptr;       // exp(-2 pi i k / max(W, H))
 
     // scalars
     uint32_t* d_scal = nullptr;
@@ -106,6 +112,7 @@ struct Context {
     int32_t* d_dist = nullptr;
     int32_t* d_lab = nullptr;
     uint64_t* d_ptr = nullptr;
+    uint64_t* d_bent = nullptr;  // flow tracing: 256 border-table slots per tile
 
     // timing
     bool timing = false;
@@ -219,8 +226,11 @@ __device__ inline bool bar_should_stop(const GridBar& gb, uint64_t t0, bool host
 }
 
 // Returns false when the launch is being abandoned (then every workgroup
-// must leave its loops and end without another barrier).
-__device__ inline bool grid_sync(const GridBar& gb) {
+// must leave its loops and end without another barrier).  The barrier is
+// over workgroups 0 .. n-1 (all of the grid by default; a launch may go on
+// with its first n workgroups once the others have ended).
+__device__ inline bool grid_sync(const GridBar& gb, uint32_t n = 0) {
+    if (n == 0) n = gridDim.x;
     __syncthreads();
     int fail = 0;
     if (threadIdx.x == 0) {
@@ -228,9 +238,9 @@ __device__ inline bool grid_sync(const GridBar& gb) {
         uint32_t* gen = &bar[BAR_GEN];
         const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t groups = gridDim.x < (uint32_t)BAR_GROUPS ? gridDim.x : (uint32_t)BAR_GROUPS;
+        const uint32_t groups = n < (uint32_t)BAR_GROUPS ? n : (uint32_t)BAR_GROUPS;
         const uint32_t grp = blockIdx.x % groups;
-        const uint32_t members = (gridDim.x - grp + groups - 1) / groups;
+        const uint32_t members = (n - grp + groups - 1) / groups;
         uint32_t* gc = &bar[BAR_GROUP + grp * BAR_STRIDE];
         if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
             __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -289,7 +299,9 @@ enum : int {
     CTL_COUNTB = 23,    // 3 back counts of the pit fill's two-ended pass lists
     CTL_PHASEB = 26,    // 3 back counts of its phase lists
     CTL_COMPACTB = 29,  // back count of its compacted fill list
-    CTL_WORDS = 30
+    CTL_RESIDENT = 30,  // pit fill: open tiles handed to the resident rounds (0: none)
+    CTL_RROUND = 31,    // pit fill: the round the resident rounds go on from
+    CTL_WORDS = 32
 };
 
 // Several rasters in one persistent launch: raster r owns the global 64x64
@@ -360,6 +372,10 @@ int launch_convolve_fast(Context* c, const float* field, float* out);
 int lr_cols_geometry(int r, int* LC, size_t* lds);
 int lr_rows_geometry(int r, int* LP, int* S, size_t* lds);
 int prepare_low_rank(Context* c);
+bool fft_supported(int W, int H, int r);
+int prepare_fft(Context* c);
+void release_fft(Context* c);
+int launch_convolve_fft(Context* c, const float* field, float* out);
 constexpr size_t LDS_LIMIT = 160 * 1024;
 int conv_2d_config(int r, int* ty, int* S, size_t* lds_bytes);
 int launch_realistic(Context* c, const float* dem, float* field, float nodata);
@@ -378,6 +394,7 @@ int realization_body(Context* c, float* mc, hipEvent_t* evs);
 // Its first part: stages a3..a5 and the quantisation into c->d_z.
 int realization_pre(Context* c, hipEvent_t* evs);
 int pit_fill_grid_cap(Context* c, int* cap);
+int pit_fill_resident_cap(Context* c, int* cap);
 int flow_routing_grid_cap(Context* c, int* cap);
 int flow_tracing_grid_cap(Context* c, int* cap);
 int clear_status(Context* c);

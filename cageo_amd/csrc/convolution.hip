@@ -305,8 +305,9 @@ int launch_convolve_to(Context* c, float* field, float* buf, float** result) {
         *result = field;
         return CAGEO_OK;
     }
-    if (c->conv_fast) {  // low-rank separable form (within 1e-5 of the exact order)
-        CAGEO_TRY(launch_convolve_fast(c, field, buf));
+    if (c->conv_fast) {  // FFT or low-rank separable form (within 1e-5 of the exact order)
+        if (c->fft_ready) CAGEO_TRY(launch_convolve_fft(c, field, buf));
+        else CAGEO_TRY(launch_convolve_fast(c, field, buf));
         *result = buf;
         return CAGEO_OK;
     }

@@ -10,20 +10,26 @@
 //   1. per 64x64 tile, in LDS: every cell's pointer jumps along the in-tile
 //      part of its path (12 doubling rounds cover 4096 cells) and ends as a
 //      terminal label, "stuck" (its path cycles inside the tile, e.g. a cell
-//      without a direction), or an exit: the first cell outside the tile;
-//   2. over the grid: exits jump to the entries of their exit cells, in
-//      place, until no exit is left; a chain still moving after
-//      ceil(log2 n)+1 rounds is a cycle through several tiles (stuck).
-//      In place is safe because every entry of a cell, old or rewritten,
-//      names a cell on that cell's own path with the step count to it, so a
-//      hop through either is correct, and a rewritten one is further along:
-//      each round advances every exit at least as far as a round over a
-//      copy would, and only the exits are written.
+//      without a direction), or an exit: the first cell outside the tile.
+//      Cells that end in their tile get their label at once.  An exit lands
+//      on a border cell of the neighbouring tile, so the tile also writes the
+//      entries of its own 252 border cells into a compact border table
+//      (256 slots per tile), and each exit cell keeps the border slot it
+//      exits to and its step count;
+//   2. over the border table only: exits jump through the entries of the
+//      slots they name, in place, until no exit is left; a chain still moving
+//      after ceil(log2 slots)+2 rounds is a cycle through several tiles
+//      (stuck).  In place is safe because every entry of a slot, old or
+//      rewritten, names a slot on that cell's own path with the step count to
+//      it, so a hop through either is correct, and a rewritten one is further
+//      along;
+//   3. every exit cell takes the label its border slot resolved to.
 // Entries are 64 bits: kind (2 bits) | steps (30 bits) | value (32 bits), so
 // the step count to the terminal is known and cells whose terminal is more
 // than 100 001 steps away (where the reference's result depends on its
 // raster order) are counted.  performConnectFlowPath (:115-134) then maps a
-// label >= 0 to label[label] when that is negative.
+// label >= 0 (only 0 occurs: labels are -stream <= 0 or the negative
+// constants) to label[0] when that is negative; it is applied in step 3.
 #include <vector>
 
 #include "common.hpp"
@@ -34,6 +40,21 @@ namespace {
 
 constexpr uint32_t K_LOCAL = 0, K_RESOLVED = 1, K_EXIT = 2, K_STUCK = 3;
 constexpr uint64_t STEP_MASK = (1ull << 30) - 1;
+// Label of an exit cell between steps 1 and 3 (labels are -stream ids, at
+// least -2^31 + 1, or the negative constants, or 0).
+constexpr int32_t EXIT_MARK = INT32_MIN;
+constexpr int BORDER_SLOTS = 256;
+
+// Slot of tile cell (lx, ly) in its tile's border table: row 0 -> 0..63,
+// row 63 -> 64..127, column 0 (rows 1..62) -> 129..190, column 63 (rows
+// 1..62) -> 193..254; -1 inside.  Slots 128, 191, 192 and 255 stay unused.
+__device__ inline int border_slot(int lx, int ly) {
+    if (ly == 0) return lx;
+    if (ly == TILE - 1) return TILE + lx;
+    if (lx == 0) return 2 * TILE + ly;
+    if (lx == TILE - 1) return 3 * TILE + ly;
+    return -1;
+}
 
 __device__ inline uint64_t pack(uint32_t kind, uint32_t steps, int32_t value) {
     return ((uint64_t)kind << 62) | ((uint64_t)(steps & STEP_MASK) << 32) |
@@ -54,7 +75,8 @@ __device__ inline uint64_t hop(uint64_t e, uint64_t f) {
 
 __device__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
                            const uint8_t* __restrict__ sf, const int32_t* __restrict__ slabel,
-                           uint64_t* __restrict__ ent, int W, int H, int ntx, int t) {
+                           uint64_t* __restrict__ ent, int32_t* __restrict__ lab,
+                           uint64_t* __restrict__ bent, int W, int H, int ntx, int t) {
     const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
@@ -146,16 +168,43 @@ __device__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
         }
     }
     __syncthreads();
+    // Outputs: the labels of the cells whose path ends in the tile; for an
+    // exit cell, the border slot it exits to (in the entry raster) and the
+    // mark EXIT_MARK; the tile's border table, every slot written (cells
+    // outside the grid and the four unused slots stuck).
+    uint64_t* tb = bent + (size_t)t * BORDER_SLOTS;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         const int ly = strip * STRIP + k;
         const int gy = y0 + ly;
+        uint64_t e = pack(K_STUCK, 0, 0);
         if (gx < W && gy < H) {
-            uint64_t e = L[ly * TILE + lx];
-            if (kind_of(e) == K_LOCAL) e = pack(K_STUCK, 0, 0);  // cycle inside the tile
-            ent[(size_t)gy * W + gx] = e;
+            e = L[ly * TILE + lx];
+            const uint32_t kind = kind_of(e);
+            const size_t i = (size_t)gy * W + gx;
+            int32_t label = 0;
+            if (kind == K_RESOLVED) {
+                label = value_of(e);
+            } else if (kind == K_EXIT) {
+                // the first cell outside the tile: a border cell of the next tile
+                const uint32_t ti = (uint32_t)value_of(e);
+                const int ny = (int)(ti / (uint32_t)W), nx = (int)(ti - (uint32_t)ny * W);
+                const int slot = ((ny / TILE) * ntx + nx / TILE) * BORDER_SLOTS +
+                                 border_slot(nx % TILE, ny % TILE);
+                e = pack(K_EXIT, steps_of(e), slot);
+                ent[i] = e;
+                label = EXIT_MARK;
+            } else {
+                e = pack(K_STUCK, 0, 0);  // K_LOCAL: a cycle inside the tile
+            }
+            lab[i] = label;
         }
+        const int slot = border_slot(lx, ly);
+        if (slot >= 0) tb[slot] = e;
     }
+    if (threadIdx.x < 4)
+        tb[threadIdx.x < 2 ? 2 * TILE + (TILE - 1) * threadIdx.x
+                           : 3 * TILE + (TILE - 1) * (threadIdx.x - 2)] = pack(K_STUCK, 0, 0);
     __syncthreads();
 }
 
@@ -177,7 +226,8 @@ struct TrRaster {
     const int8_t* fd;
     const uint8_t* sf;
     const int32_t* slabel;
-    uint64_t* ent;  // one 64-bit entry per cell
+    uint64_t* ent;  // one 64-bit entry per cell (written for exit cells)
+    uint64_t* bent; // border table: BORDER_SLOTS entries per tile
     int32_t* lab;
     int W, H, ntx;
     uint32_t* scal;
@@ -232,12 +282,20 @@ __device__ inline TrUnit tr_unit(const TrArgs& a, long long u) {
     return t;
 }
 
-// Tile pass, exit jumping, labels and the connect step in one persistent
-// launch over every raster.  Sweeps over all cells go by units of UNIT_ROWS
-// x 64 cells, one wave each, with each unit's loads issued before its
-// stores.  Round r of the exit jumping raises flag r % 3 while exits remain
-// and clears flag (r+1) % 3, whose last read (end of round r-2) is
-// separated from the clear by barriers.
+// Final label of cell 0 of a raster (between steps 2 and 3 of the launch).
+__device__ inline int32_t label_of_cell0(const TrRaster& R) {
+    const int32_t v = R.lab[0];
+    if (v != EXIT_MARK) return v;
+    const uint64_t f = ld64(&R.bent[value_of(R.ent[0])]);
+    return kind_of(f) == K_RESOLVED ? value_of(f) : 0;
+}
+
+// The tile pass, the border-table jumping, the exit cells' labels and the
+// connect step in one persistent launch over every raster.  Round r of the
+// jumping raises flag r % 3 while exits remain and clears flag (r+1) % 3,
+// whose last read (end of round r-2) is separated from the clear by
+// barriers.  The exit cells' pass goes by units of UNIT_ROWS x 64 cells, one
+// wave each, with each unit's loads issued before its stores.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
     __shared__ uint64_t L[TILE * TILE];
@@ -251,31 +309,28 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         const TrBlock b = tr_block<MULTI>(a, g);
         const TrRaster& R = b.R;
         if (b.lb == 0 && threadIdx.x == 0) R.scal[SC_LONGPATH] = 0;
-        trace_tile(L, R.fd, R.sf, R.slabel, R.ent, R.W, R.H, R.ntx, b.lb);
+        trace_tile(L, R.fd, R.sf, R.slabel, R.ent, R.lab, R.bent, R.W, R.H, R.ntx, b.lb);
     }
     if (!grid_sync(a.gb)) return;
 
-    const long long units = (long long)a.total * UNITS_PER_BLOCK;
+    // Jumping over the border tables: thread s takes slot s % BORDER_SLOTS
+    // of global block s / BORDER_SLOTS (the blocks of one raster are
+    // consecutive, so a wave's 64 slots lie in one raster).
+    const long long slots = (long long)a.total * BORDER_SLOTS;
     int rounds = 0;
     for (int r = 0; r < a.max_rounds; ++r) {
         rounds = r + 1;
         uint32_t pending = 0;
-        for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
-            const TrUnit t = tr_unit<MULTI>(a, u);
-            uint64_t* ent = t.R.ent;
-            uint64_t e[UNIT_ROWS], f[UNIT_ROWS];
-#pragma unroll
-            for (int k = 0; k < UNIT_ROWS; ++k)  // only this lane writes its own cells
-                e[k] = k < t.rows ? ent[(size_t)(t.y0 + k) * t.R.W + t.x] : pack(K_STUCK, 0, 0);
-#pragma unroll
-            for (int k = 0; k < UNIT_ROWS; ++k)
-                f[k] = kind_of(e[k]) == K_EXIT ? ld64(&ent[value_of(e[k])]) : 0;
-#pragma unroll
-            for (int k = 0; k < UNIT_ROWS; ++k) {
-                if (k >= t.rows || kind_of(e[k]) != K_EXIT) continue;
-                e[k] = hop(e[k], f[k]);
-                pending |= kind_of(e[k]) == K_EXIT;
-                st64(&ent[(size_t)(t.y0 + k) * t.R.W + t.x], e[k]);
+        for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < slots;
+             s += (long long)gridDim.x * blockDim.x) {
+            const TrBlock b = tr_block<MULTI>(a, (int)(s / BORDER_SLOTS));
+            uint64_t* tb = b.R.bent;
+            const size_t own = (size_t)b.lb * BORDER_SLOTS + (size_t)(s % BORDER_SLOTS);
+            const uint64_t e = ld64(&tb[own]);
+            if (kind_of(e) == K_EXIT) {
+                const uint64_t ne = hop(e, ld64(&tb[value_of(e)]));
+                pending |= kind_of(ne) == K_EXIT;
+                st64(&tb[own], ne);
             }
         }
         if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&a.ctrl[CTL_FLAG + r % 3], 1u);
@@ -283,24 +338,41 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         if (!grid_sync(a.gb)) return;
         if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
 
+    // Exit cells take their slot's label; performConnectFlowPath maps a
+    // label 0 to label[0] when that is negative.  An exit whose chain is
+    // still moving (a cycle through tiles) or ends stuck stays 0.
+    const long long units = (long long)a.total * UNITS_PER_BLOCK;
     for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
         const TrUnit t = tr_unit<MULTI>(a, u);
-        const uint64_t* fin = t.R.ent;
+        const int32_t l0 = label_of_cell0(t.R);
+        int32_t v[UNIT_ROWS];
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k)
+            v[k] = k < t.rows ? t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] : 1;
         uint64_t e[UNIT_ROWS];
 #pragma unroll
         for (int k = 0; k < UNIT_ROWS; ++k)
-            e[k] = k < t.rows ? fin[(size_t)(t.y0 + k) * t.R.W + t.x] : pack(K_STUCK, 0, 0);
+            e[k] = v[k] == EXIT_MARK ? t.R.ent[(size_t)(t.y0 + k) * t.R.W + t.x] : 0;
+        uint64_t f[UNIT_ROWS];
+#pragma unroll
+        for (int k = 0; k < UNIT_ROWS; ++k)
+            f[k] = v[k] == EXIT_MARK ? ld64(&t.R.bent[value_of(e[k])]) : 0;
         uint32_t longp = 0;
 #pragma unroll
         for (int k = 0; k < UNIT_ROWS; ++k) {
             if (k >= t.rows) continue;
-            int32_t v = 0;
-            if (kind_of(e[k]) == K_RESOLVED) {
-                v = value_of(e[k]);
-                if (steps_of(e[k]) > 100001u) ++longp;  // past the reference's walk guard
+            int32_t w = v[k];
+            if (w == EXIT_MARK) {
+                w = 0;
+                if (kind_of(f[k]) == K_RESOLVED) {
+                    w = value_of(f[k]);
+                    if (steps_of(e[k]) + steps_of(f[k]) > 100001u) ++longp;  // past the walk guard
+                }
             }
-            t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] = v;
+            if (w == 0 && l0 < 0) w = l0;
+            if (w != v[k]) t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] = w;
         }
         if (__any(longp)) {
             for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
@@ -309,23 +381,6 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
                 atomicAdd(&t.R.scal[SC_LONGRUN], longp);
             }
         }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
-    if (!grid_sync(a.gb)) return;
-    // performConnectFlowPath: label[0] is final and never changes here.
-    for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
-        const TrUnit t = tr_unit<MULTI>(a, u);
-        const size_t n = (size_t)t.R.W * t.R.H;
-        int32_t v[UNIT_ROWS], w[UNIT_ROWS];
-#pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k)
-            v[k] = k < t.rows ? t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] : -1;
-#pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k) w[k] = (v[k] >= 0 && (size_t)v[k] < n) ? t.R.lab[v[k]] : 0;
-#pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k)
-            if (k < t.rows && v[k] >= 0 && (size_t)v[k] < n && w[k] < 0)
-                t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] = w[k];
     }
 }
 
@@ -342,6 +397,7 @@ TrRaster tr_raster(Context* c, const int8_t* fd, const uint8_t* sflags, const in
     R.sf = sflags;
     R.slabel = slabel;
     R.ent = c->d_ptr;
+    R.bent = c->d_bent;
     R.lab = lab;
     R.W = c->W;
     R.H = c->H;

@@ -62,6 +62,7 @@ struct PfShared {
     uint32_t any;
     uint32_t next;  // the workgroup's next tile index in the pass
     uint32_t steps; // row steps of the tile's solve, all waves
+    uint32_t ringrows[2];  // resident rounds: tile rows next to changed halo cells
 };
 
 // run_passes result of a launch being abandoned at a barrier.
@@ -144,6 +145,7 @@ struct PfArgs {
     uint32_t* cert;       // per raster: 3 rotating per-round fixed-cell counters
     GridBar gb;           // grid barrier
     int max_passes;
+    int resident_cap;     // co-resident workgroups of k_pf_resident (0: no resident rounds)
 };
 
 // The raster of global block g, g's number within it, and the raster's
@@ -272,9 +274,11 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
         unsigned long long w64, e64;
         edge_columns_get(S.wcol, S.ecol, w64, e64);
         const Ranked rank{a.tcost, PF_HEAVY, count_back, a.total};
+        // a neighbour without OPEN cells (its open flag 0, for good: OPEN
+        // cells never reappear) cannot change and is not listed
         activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
                        work_out, count_out, a.mark, epoch, a.rows, (pass + 1) & 1, w64, e64,
-                       &rank);
+                       &rank, a.topen);
     }
     __syncthreads();
 }
@@ -297,59 +301,15 @@ __device__ inline void keep_quantized(const PfRaster& R, int lb) {
     }
 }
 
-// Fill relaxation of one tile (with immediate fixing at Z).  Cell state
-// lives in LDS (relays in rel, Z in zt); each thread keeps bit masks of its
-// strip's cells only, so the row loop can run in either direction without
-// per-thread arrays.
-// The first pass of a round is its restart: OPEN cells start from +inf and
-// all of the tile's are rewritten (in round 0 they already hold +inf).  In
-// it, a raster whose last round fixed nothing is finished instead.
-__device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
-                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
-                          uint32_t* count_back, uint32_t epoch, int pass, int round,
-                          int phase_out) {
-    const PfRaster& a = b.R;
-    const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
-    if (dirty == 0) return;  // uniform
-    const bool restart = pass == 0;
-    if (restart && round > 0 &&
-        __builtin_amdgcn_readfirstlane(ld(&args.cert[3 * b.r + (round + 2) % 3])) == 0u) {
-        keep_quantized(a, b.lb);
-        return;
-    }
-    const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    load_tile(S, a, x0, y0, restart);
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
-    const int gx = x0 + lx;
-    uint32_t open = 0, zlm = 0;  // OPEN cells; OPEN cells whose Z is live
-#pragma unroll
-    for (int k0 = 0; k0 < STRIP; k0 += STRIP / 2) {  // two batches of independent loads
-        uint8_t sv[STRIP / 2];
-        float zv[STRIP / 2];
-#pragma unroll
-        for (int j = 0; j < STRIP / 2; ++j) {
-            const int gy = y0 + strip * STRIP + k0 + j;
-            sv[j] = PF_NODATA;
-            zv[j] = 0.0f;
-            if (gx < a.W && gy < a.H) {
-                const size_t gi = (size_t)gy * a.W + gx;
-                sv[j] = a.st[gi];
-                zv[j] = a.zq[gi];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < STRIP / 2; ++j) {
-            if (sv[j] == PF_OPEN) {
-                open |= 1u << (k0 + j);
-                if (pf_live(zv[j], mn)) zlm |= 1u << (k0 + j);
-            }
-            S.zt[(strip * STRIP + k0 + j) * TILE + lx] = zv[j];
-        }
-    }
-    __syncthreads();
-    clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
-    const uint32_t rewrite = restart && round > 0 ? open : 0u;  // stale values of the last round
-
+// Fill sweeps of a tile whose relays (S.rel) and Z (S.zt) are in LDS, over
+// the rows of `dirty` first (a 64-bit row mask) and then every row a change
+// reaches.  open / zlm: the thread's OPEN cells and those whose Z is live
+// (bit k = row k of its strip); lowered / fixed / fixed_live collect its
+// cells whose value dropped, that were fixed, that were fixed live; nrow
+// counts the wave's row steps.
+__device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float mn,
+                                   uint32_t& open, uint32_t zlm, uint32_t& lowered,
+                                   uint32_t& fixed, uint32_t& fixed_live, uint32_t& nrow) {
     // Row step (row ly, all 64 columns at once):
     //  1. Z-rule: an OPEN cell with a live FIXED neighbour no higher than Z
     //     is fixed at Z and, if Z is live, relays Z; along the row this
@@ -360,9 +320,8 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     //     (their value if live, nothing if dead).  The fixed point of the row,
     //     W(x) = max(Z, min(c, W(x-1), W(x+1))), is min(L(x), R(x)) with L, R
     //     the map chains from the left and right halo cells.
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int hl = 0, hr = HALO_W - 1;  // halo columns
-    uint32_t lowered = 0, fixed = 0, fixed_live = 0;
-    uint32_t nrow = 0;  // rows solved by this wave
     uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
@@ -438,6 +397,64 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
         if (!__syncthreads_or(changed)) break;
     }
+}
+
+// Fill relaxation of one tile (with immediate fixing at Z).  Cell state
+// lives in LDS (relays in rel, Z in zt); each thread keeps bit masks of its
+// strip's cells only, so the row loop can run in either direction without
+// per-thread arrays.
+// The first pass of a round is its restart: OPEN cells start from +inf and
+// all of the tile's are rewritten (in round 0 they already hold +inf).  In
+// it, a raster whose last round fixed nothing is finished instead.
+__device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
+                          uint32_t* count_back, uint32_t epoch, int pass, int round,
+                          int phase_out) {
+    const PfRaster& a = b.R;
+    const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
+    if (dirty == 0) return;  // uniform
+    const bool restart = pass == 0;
+    if (restart && round > 0 &&
+        __builtin_amdgcn_readfirstlane(ld(&args.cert[3 * b.r + (round + 2) % 3])) == 0u) {
+        keep_quantized(a, b.lb);
+        return;
+    }
+    const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
+    load_tile(S, a, x0, y0, restart);
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int gx = x0 + lx;
+    uint32_t open = 0, zlm = 0;  // OPEN cells; OPEN cells whose Z is live
+#pragma unroll
+    for (int k0 = 0; k0 < STRIP; k0 += STRIP / 2) {  // two batches of independent loads
+        uint8_t sv[STRIP / 2];
+        float zv[STRIP / 2];
+#pragma unroll
+        for (int j = 0; j < STRIP / 2; ++j) {
+            const int gy = y0 + strip * STRIP + k0 + j;
+            sv[j] = PF_NODATA;
+            zv[j] = 0.0f;
+            if (gx < a.W && gy < a.H) {
+                const size_t gi = (size_t)gy * a.W + gx;
+                sv[j] = a.st[gi];
+                zv[j] = a.zq[gi];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < STRIP / 2; ++j) {
+            if (sv[j] == PF_OPEN) {
+                open |= 1u << (k0 + j);
+                if (pf_live(zv[j], mn)) zlm |= 1u << (k0 + j);
+            }
+            S.zt[(strip * STRIP + k0 + j) * TILE + lx] = zv[j];
+        }
+    }
+    __syncthreads();
+    clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
+    const uint32_t rewrite = restart && round > 0 ? open : 0u;  // stale values of the last round
+
+    uint32_t lowered = 0, fixed = 0, fixed_live = 0;
+    uint32_t nrow = 0;  // rows solved by this wave
+    fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow);
     if ((threadIdx.x & 63) == 0 && nrow) atomicAdd(&S.steps, nrow);
 
     uint32_t bits = 0, nfixed = 0;
@@ -458,49 +475,19 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
                 count_back, epoch, pass, phase_out);
 }
 
-// Certification of one tile after the fill has converged.
-__device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
-                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
-                          uint32_t* count_back, uint32_t epoch, int pass, int phase_out) {
-    const PfRaster& a = b.R;
-    const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
-    if (dirty == 0) return;  // uniform
-    if (pass == 0 && __builtin_amdgcn_readfirstlane(ld(&args.topen[b.gbase + b.lb])) == 0u)
-        return;  // closed since it joined the phase list
-    const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    load_tile(S, a, x0, y0, false);
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
-    const int gx = x0 + lx;
-    uint32_t open = 0, open_inf = 0;
-    uint8_t sv[STRIP];
-#pragma unroll
-    for (int k = 0; k < STRIP; ++k) {
-        const int gy = y0 + strip * STRIP + k;
-        sv[k] = PF_NODATA;
-        if (gx < a.W && gy < a.H) sv[k] = a.st[(size_t)gy * a.W + gx];
-    }
-    __syncthreads();  // the relays of load_tile are in LDS
-    clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
-    uint32_t lm = 0;  // OPEN cells whose (converged, fixed here) value is live
-#pragma unroll
-    for (int k = 0; k < STRIP; ++k) {
-        if (sv[k] == PF_OPEN) {
-            const float wk = S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1].x;
-            if (wk < PF_INF) {
-                open |= 1u << k;
-                if (pf_live(wk, mn)) lm |= 1u << k;
-            } else {
-                open_inf |= 1u << k;
-            }
-        }
-    }
-
+// Certification sweeps of a tile whose converged fill values (S.rel.x) are
+// in LDS, over the rows of `dirty` first and then every row a change
+// reaches.  open / lm: the thread's OPEN cells with a finite value and those
+// whose value is live; fixed / fixed_live collect the cells it fixes (live);
+// nrow counts the wave's row steps.
+__device__ inline void cert_sweeps(PfShared& S, unsigned long long dirty, uint32_t& open,
+                                   uint32_t lm, uint32_t& fixed, uint32_t& fixed_live,
+                                   uint32_t& nrow) {
     // Row step: seeds are OPEN cells with a live FIXED neighbour no higher
     // than their value; along the row, fixing spreads from a cell fixed live
     // to an OPEN neighbour whose value is not lower (segmented propagation
     // over the lane masks).
-    uint32_t fixed = 0, fixed_live = 0;
-    uint32_t nrow = 0;  // rows solved by this wave
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
@@ -542,6 +529,48 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         }
         if (!__syncthreads_or(changed)) break;
     }
+}
+
+// Certification of one tile after the fill has converged.
+__device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, float mn,
+                          uint32_t* cert_ctr, int* work_out, uint32_t* count_out,
+                          uint32_t* count_back, uint32_t epoch, int pass, int phase_out) {
+    const PfRaster& a = b.R;
+    const unsigned long long dirty = tile_dirty_rows(args.rows, b.gbase + b.lb, pass);
+    if (dirty == 0) return;  // uniform
+    if (pass == 0 && __builtin_amdgcn_readfirstlane(ld(&args.topen[b.gbase + b.lb])) == 0u)
+        return;  // closed since it joined the phase list
+    const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
+    load_tile(S, a, x0, y0, false);
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int gx = x0 + lx;
+    uint32_t open = 0, open_inf = 0;
+    uint8_t sv[STRIP];
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) {
+        const int gy = y0 + strip * STRIP + k;
+        sv[k] = PF_NODATA;
+        if (gx < a.W && gy < a.H) sv[k] = a.st[(size_t)gy * a.W + gx];
+    }
+    __syncthreads();  // the relays of load_tile are in LDS
+    clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
+    uint32_t lm = 0;  // OPEN cells whose (converged, fixed here) value is live
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) {
+        if (sv[k] == PF_OPEN) {
+            const float wk = S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1].x;
+            if (wk < PF_INF) {
+                open |= 1u << k;
+                if (pf_live(wk, mn)) lm |= 1u << k;
+            } else {
+                open_inf |= 1u << k;
+            }
+        }
+    }
+
+    uint32_t fixed = 0, fixed_live = 0;
+    uint32_t nrow = 0;  // rows solved by this wave
+    cert_sweeps(S, dirty, open, lm, fixed, fixed_live, nrow);
     if ((threadIdx.x & 63) == 0 && nrow) atomicAdd(&S.steps, nrow);
 
     uint32_t bits = 0, nfixed = 0;
@@ -645,6 +674,264 @@ __device__ inline void compact_open(const PfArgs& a, int from) {
     }
 }
 
+// ---- resident rounds ---------------------------------------------------------
+// After round 0 only a few tiles still hold OPEN cells (some 450 of the 4096
+// of a 4096^2 raster), but the rounds after it take dozens of barrier-
+// separated passes that each solve a handful of tiles, so their time is the
+// latency of a tile's list traffic, loads and writes.  When every such tile
+// can have a workgroup of its own, the launch goes on with those workgroups
+// only (the others end): each keeps its tile in LDS for all the remaining
+// rounds and per pass reloads only the 260 halo cells around it, solving the
+// rows next to halo cells that changed.  A pass ends at a barrier over these
+// workgroups, and a phase ends after a pass in which no tile changed a
+// border cell (then no halo changes and every tile is at its fixed point).
+// Passes raise rotating flags CTL_FLAG + q % 3 (q counts the resident
+// passes); workgroup 0 clears flag (q + 1) % 3 during pass q, whose last read
+// ended pass q - 2.  The rules of each phase are those of the classic
+// passes: the same sweeps, write-back and counters.
+
+// Reloads the halo ring (the restart rule as in load_tile); returns the
+// 64-bit mask of tile rows next to ring cells whose relay or live value
+// changed (uniform).
+__device__ inline unsigned long long load_ring(PfShared& S, const PfRaster& a, int x0, int y0,
+                                               bool restart) {
+    constexpr int RING = 4 * HALO_W - 4;  // 260 cells
+    constexpr int IT = (RING + TILE_THREADS - 1) / TILE_THREADS;
+    if (threadIdx.x < 2) S.ringrows[threadIdx.x] = 0u;
+    uint8_t sv[IT];
+    float wv[IT];
+    int li[IT];
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+        const int k = threadIdx.x + j * TILE_THREADS;
+        sv[j] = PF_NODATA;
+        wv[j] = PF_INF;
+        li[j] = -1;
+        if (k < RING) {
+            int hx, hy;
+            if (k < HALO_W) {
+                hy = 0;
+                hx = k;
+            } else if (k < 2 * HALO_W) {
+                hy = HALO_W - 1;
+                hx = k - HALO_W;
+            } else {
+                const int m = k - 2 * HALO_W;
+                hy = 1 + (m >> 1);
+                hx = (m & 1) ? HALO_W - 1 : 0;
+            }
+            li[j] = hy * HALO_W + hx;
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+            if (gx >= 0 && gy >= 0 && gx < a.W && gy < a.H) {
+                const size_t gi = (size_t)gy * a.W + gx;
+                sv[j] = a.st[gi];
+                wv[j] = a.w[gi];
+            }
+        }
+    }
+    __syncthreads();  // ringrows cleared
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+        if (li[j] < 0) continue;
+        const uint8_t st = sv[j];
+        const float r = (st == PF_FIXED_LIVE || (st == PF_OPEN && !restart)) ? wv[j] : PF_INF;
+        const float2 nv = make_float2(r, st == PF_FIXED_LIVE ? r : PF_INF);
+        const float2 ov = S.rel[li[j]];
+        if (nv.x != ov.x || nv.y != ov.y) {
+            S.rel[li[j]] = nv;
+            const int hy = li[j] / HALO_W;
+            int r0, r1;  // tile rows whose cells see this halo cell
+            if (hy == 0) {
+                r0 = r1 = 0;
+            } else if (hy == HALO_W - 1) {
+                r0 = r1 = TILE - 1;
+            } else {
+                r0 = max(hy - 2, 0);
+                r1 = min(hy, TILE - 1);
+            }
+            for (int r2 = r0; r2 <= r1; ++r2) {
+                if (r2 < 32) lo |= 1u << r2;
+                else hi |= 1u << (r2 - 32);
+            }
+        }
+    }
+    if (lo) atomicOr(&S.ringrows[0], lo);
+    if (hi) atomicOr(&S.ringrows[1], hi);
+    __syncthreads();
+    return ((unsigned long long)S.ringrows[1] << 32) | S.ringrows[0];
+}
+
+// Sum over the workgroup of a per-thread count below 32 and OR of 8-bit
+// border masks, through LDS words (S.steps / S.bits); uniform results.
+__device__ inline void resident_reduce(PfShared& S, uint32_t nfixed, uint32_t bits,
+                                       uint32_t* cert_ctr, uint32_t& any_border) {
+    if (threadIdx.x == 0) {
+        S.bits = 0;
+        S.steps = 0;
+    }
+    __syncthreads();
+    bits = wave_or8(bits);
+    nfixed = wave_sum5(nfixed);
+    if ((threadIdx.x & 63) == 0) {
+        if (bits) atomicOr(&S.bits, bits);
+        if (nfixed) atomicAdd(&S.steps, nfixed);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && S.steps) atomicAdd(cert_ctr, S.steps);
+    any_border = S.bits;
+    __syncthreads();
+}
+
+// The rounds after round `round` for the tile of workgroup `blockIdx.x`
+// (one raster; workgroups 0 .. n-1 take the n tiles of the compacted list).
+// Returns the round it ended in, -1 when a phase hit the pass cap, or
+// PF_ABANDONED.
+__device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round, uint32_t n) {
+    const PfRaster& R = a.one;
+    const uint32_t nf = ld(&a.ctrl[CTL_COMPACT]);
+    const int t = ranked_entry(a.work + (size_t)6 * a.total, a.total, nf, blockIdx.x);
+    const int x0 = (t % R.ntx) * TILE, y0 = (t / R.ntx) * TILE;
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int gx = x0 + lx;
+    // the whole tile once, as a first pass of a round loads it
+    load_tile(S, R, x0, y0, true);
+    uint32_t open = 0, zlm = 0;
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) {
+        const int gy = y0 + strip * STRIP + k;
+        uint8_t sv = PF_NODATA;
+        float zv = 0.0f;
+        if (gx < R.W && gy < R.H) {
+            const size_t gi = (size_t)gy * R.W + gx;
+            sv = R.st[gi];
+            zv = R.zq[gi];
+        }
+        if (sv == PF_OPEN) {
+            open |= 1u << k;
+            if (pf_live(zv, mn)) zlm |= 1u << k;
+        }
+        S.zt[(strip * STRIP + k) * TILE + lx] = zv;
+    }
+    __syncthreads();
+    uint32_t q = 0;  // resident pass count
+    bool first_round = true;
+    for (++round;; ++round) {
+        const int slot = round % 3;
+        // a round after one that fixed nothing: the OPEN cells are never
+        // reached and keep Z
+        if (__builtin_amdgcn_readfirstlane(ld(&a.cert[(round + 2) % 3])) == 0u) {
+#pragma unroll
+            for (int k = 0; k < STRIP; ++k) {
+                const int gy = y0 + strip * STRIP + k;
+                if ((open >> k) & 1u) R.w[(size_t)gy * R.W + gx] = S.zt[(strip * STRIP + k) * TILE + lx];
+            }
+            return round;
+        }
+        // ---- fill: OPEN cells restart from +inf and are all rewritten
+        if (!first_round) {
+#pragma unroll
+            for (int k = 0; k < STRIP; ++k)
+                if ((open >> k) & 1u)
+                    S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1] = make_float2(PF_INF, PF_INF);
+        }
+        first_round = false;
+        uint32_t rewrite = open;
+        for (int pass = 0;; ++pass, ++q) {
+            if (pass >= a.max_passes) return -1;
+            unsigned long long dirty = ~0ull;
+            const unsigned long long ring = load_ring(S, R, x0, y0, pass == 0);
+            if (pass > 0) dirty = ring;
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.ctrl[CTL_FLAG + (q + 1) % 3] = 0;
+                atomicAdd(&a.ctrl[CTL_PASSES], 1u);
+            }
+            uint32_t any_border = 0;
+            if (dirty) {
+                if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
+                __syncthreads();
+                uint32_t lowered = 0, fixed = 0, fixed_live = 0, nrow = 0;
+                fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow);
+                uint32_t bits = 0, nfixed = 0;
+#pragma unroll
+                for (int k = 0; k < STRIP; ++k) {
+                    const int ly = strip * STRIP + k;
+                    const size_t gi = (size_t)(y0 + ly) * R.W + gx;
+                    if ((lowered | rewrite) & (1u << k))
+                        R.w[gi] = (fixed & (1u << k)) ? S.zt[ly * TILE + lx]
+                                                      : S.rel[(ly + 1) * HALO_W + lx + 1].x;
+                    if (fixed & (1u << k)) {
+                        R.st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+                        ++nfixed;
+                    }
+                    if ((lowered | fixed) & (1u << k)) bits |= border_bits(lx, ly);
+                }
+                rewrite = 0;
+                resident_reduce(S, nfixed, bits, &a.cert[slot], any_border);
+            }
+            if (any_border && threadIdx.x == 0) atomicOr(&a.ctrl[CTL_FLAG + q % 3], 1u);
+            if (!grid_sync(a.gb, n)) return PF_ABANDONED;
+            if (!ld(&a.ctrl[CTL_FLAG + q % 3])) {
+                ++q;
+                break;
+            }
+        }
+        // the next round's fixed-cell counter: last read at this round's start
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.cert[(round + 1) % 3] = 0;
+        // ---- certification of the converged fill
+        uint32_t copen = 0, lm = 0, open_inf = 0;
+#pragma unroll
+        for (int k = 0; k < STRIP; ++k) {
+            if (!((open >> k) & 1u)) continue;
+            const float wk = S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1].x;
+            if (wk < PF_INF) {
+                copen |= 1u << k;
+                if (pf_live(wk, mn)) lm |= 1u << k;
+            } else {
+                open_inf |= 1u << k;
+            }
+        }
+        for (int pass = 0;; ++pass, ++q) {
+            if (pass >= a.max_passes) return -1;
+            const unsigned long long ring = load_ring(S, R, x0, y0, false);
+            const unsigned long long dirty = pass == 0 ? ~0ull : ring;
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.ctrl[CTL_FLAG + (q + 1) % 3] = 0;
+                atomicAdd(&a.ctrl[CTL_PASSES], 1u);
+            }
+            uint32_t any_border = 0;
+            if (dirty) {
+                if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
+                __syncthreads();
+                uint32_t fixed = 0, fixed_live = 0, nrow = 0;
+                cert_sweeps(S, dirty, copen, lm, fixed, fixed_live, nrow);
+                uint32_t bits = 0, nfixed = 0;
+#pragma unroll
+                for (int k = 0; k < STRIP; ++k) {
+                    if (!(fixed & (1u << k))) continue;
+                    const int ly = strip * STRIP + k;
+                    const bool live = (fixed_live >> k) & 1u;
+                    R.st[(size_t)(y0 + ly) * R.W + gx] = live ? PF_FIXED_LIVE : PF_FIXED_DEAD;
+                    // a cell fixed dead relays nothing from now on
+                    if (!live)
+                        S.rel[(ly + 1) * HALO_W + lx + 1] = make_float2(PF_INF, PF_INF);
+                    ++nfixed;
+                    if (live) bits |= border_bits(lx, ly);
+                }
+                open &= ~fixed;
+                resident_reduce(S, nfixed, bits, &a.cert[slot], any_border);
+            }
+            if (any_border && threadIdx.x == 0) atomicOr(&a.ctrl[CTL_FLAG + q % 3], 1u);
+            if (!grid_sync(a.gb, n)) return PF_ABANDONED;
+            if (!ld(&a.ctrl[CTL_FLAG + q % 3])) {
+                ++q;
+                break;
+            }
+        }
+        (void)open_inf;
+    }
+}
+
 // MULTI: a table of rasters (else the one raster of the arguments; two
 // instantiations keep the one-raster code free of table lookups).
 template <bool MULTI>
@@ -733,6 +1020,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         a.ctrl[CTL_PHASEB + 1] = 0;
         a.ctrl[CTL_PHASEB + 2] = 0;
         a.ctrl[CTL_COMPACTB] = 0;
+        a.ctrl[CTL_FLAG + 0] = 0;  // pass flags of the resident rounds
+        a.ctrl[CTL_FLAG + 1] = 0;
+        a.ctrl[CTL_FLAG + 2] = 0;
+        a.ctrl[CTL_RESIDENT] = 0;
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH]);
@@ -759,12 +1050,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (ld(&a.ctrl[CTL_PHASE + phase % 3]) + ld(&a.ctrl[CTL_PHASEB + phase % 3]) == 0) break;
         compact_open(a, phase % 3);
         if (!grid_sync(a.gb)) return;
-        if (ld(&a.ctrl[CTL_COMPACT]) + ld(&a.ctrl[CTL_COMPACTB]) == 0) break;
+        const uint32_t nopen = ld(&a.ctrl[CTL_COMPACT]) + ld(&a.ctrl[CTL_COMPACTB]);
+        if (nopen == 0) break;
+        if (!MULTI && nopen <= (uint32_t)a.resident_cap) {
+            // every tile still open can have a workgroup of its own: the
+            // rounds go on in k_pf_resident, launched after this kernel
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.ctrl[CTL_RESIDENT] = nopen;
+                a.ctrl[CTL_RROUND] = (uint32_t)round;
+                a.ctrl[CTL_EPOCH] = epoch;
+            }
+            return;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_EPOCH] = epoch;
         a.ctrl[CTL_ROUNDS] = (uint32_t)round + 1;
         if (li < 0) a.ctrl[CTL_STATUS] = (uint32_t)(-CAGEO_ERR_NOT_CONVERGED);
+    }
+}
+
+// The rounds after the one k_pf_persistent stopped at, when it handed them
+// over (ctrl[CTL_RESIDENT] open tiles, one workgroup each; the other
+// workgroups end at once).  Nothing to do otherwise.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pf_resident(PfArgs a) {
+    __shared__ PfShared S;
+    const uint32_t n = __builtin_amdgcn_readfirstlane(ld(&a.ctrl[CTL_RESIDENT]));
+    if (blockIdx.x >= n) return;
+    if (grid_aborted(a.gb)) return;
+    float mn = 0.0f;
+    pf_raster_ok(a.one, mn);  // in the domain: the first kernel checked it
+    const int round = (int)__builtin_amdgcn_readfirstlane(ld(&a.ctrl[CTL_RROUND]));
+    const int rr = resident_rounds(S, a, mn, round, n);
+    if (rr == PF_ABANDONED) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl[CTL_ROUNDS] = (uint32_t)(rr < 0 ? round : rr) + 1;
+        if (rr < 0) a.ctrl[CTL_STATUS] = (uint32_t)(-CAGEO_ERR_NOT_CONVERGED);
     }
 }
 
@@ -801,6 +1122,10 @@ int pit_fill_grid_cap(Context* c, int* cap) {
     return occupancy_cap(reinterpret_cast<const void*>(k_pf_persistent<false>), c->device, cap);
 }
 
+int pit_fill_resident_cap(Context* c, int* cap) {
+    return occupancy_cap(reinterpret_cast<const void*>(k_pf_resident), c->device, cap);
+}
+
 int pit_fill_persistent_grid(Context* c, int* grid) {
     if (c->pf_grid <= 0) CAGEO_TRY(pit_fill_grid_cap(c, &c->pf_grid));
     *grid = c->pf_grid;
@@ -826,11 +1151,16 @@ int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool /*sta
     a.cert = c->d_ctrl + CTL_CERT;
     a.gb = context_gridbar(c);
     a.max_passes = 1 << 20;
-    // An ordinary launch: the grid barrier needs its workgroups co-resident,
-    // which the occupancy cap guarantees (work of other kernels on the device
+    if (c->pf_res_grid <= 0) CAGEO_TRY(pit_fill_resident_cap(c, &c->pf_res_grid));
+    a.resident_cap = std::min(c->pf_res_grid, grid);
+    // Ordinary launches: the grid barrier needs its workgroups co-resident,
+    // which the occupancy caps guarantee (work of other kernels on the device
     // can only delay the start of some workgroups).  Cooperative launches are
-    // not used: the runtime allows them to one process at a time.
+    // not used: the runtime allows them to one process at a time.  The second
+    // launch goes on with the rounds the first one hands over, if any.
     hipLaunchKernelGGL(k_pf_persistent<false>, dim3(grid), dim3(TILE_THREADS), 0, c->stream, a);
+    CAGEO_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_pf_resident, dim3(a.resident_cap), dim3(TILE_THREADS), 0, c->stream, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
@@ -864,6 +1194,7 @@ int pit_fill_multi(MultiState* m, hipStream_t s) {
     a.cert = m->d_cert;
     a.gb = multi_gridbar(m);
     a.max_passes = 1 << 20;
+    a.resident_cap = 0;  // the batch keeps every round in the all-tile launch
     hipLaunchKernelGGL(k_pf_persistent<true>, dim3(m->pf_grid), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;

@@ -122,41 +122,6 @@ __global__ void k_crop(const float* __restrict__ in, int wp, float* __restrict__
     }
 }
 
-// Check statistics, one pair per workgroup: sum |q - c/k| and the expected
-// sum E|B/k - q| with B ~ Binomial(k, q) (the distance sampling noise alone
-// would give if q were exact).
-__global__ void k_check_partials(const float* __restrict__ q, const float* __restrict__ cnt,
-                                 long long n, int k, double* __restrict__ part) {
-    __shared__ double s[2][256];
-    double l1 = 0.0, ex = 0.0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const double qi = q[i];
-        l1 += fabs(qi - (double)cnt[i] / k);
-        // sum_j C(k, j) q^j (1-q)^(k-j) |j/k - q|
-        double e = 0.0, comb = 1.0;
-        for (int j = 0; j <= k; ++j) {
-            e += comb * pow(qi, j) * pow(1.0 - qi, k - j) * fabs((double)j / k - qi);
-            comb = comb * (k - j) / (j + 1);
-        }
-        ex += e;
-    }
-    s[0][threadIdx.x] = l1;
-    s[1][threadIdx.x] = ex;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            s[0][threadIdx.x] += s[0][threadIdx.x + off];
-            s[1][threadIdx.x] += s[1][threadIdx.x + off];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        part[(size_t)blockIdx.x * 2] = s[0][0];
-        part[(size_t)blockIdx.x * 2 + 1] = s[1][0];
-    }
-}
-
 constexpr int RED_BLOCKS = 1024;
 
 struct Layer {
@@ -411,31 +376,11 @@ int cageo_surrogate_predict(cageo_surrogate* s, const float* d_dem, const float*
 }
 
 int cageo_surrogate_check(cageo_surrogate* s, cageo_context* ctx, const float* d_prob, int k,
-                          uint64_t seed, double tolerance, float* d_scratch, int* accepted,
-                          double* l1, double* noise_l1) {
-    if (!s || !ctx || !d_prob || !d_scratch || k < 1 || k > 1000) return CAGEO_ERR_ARG;
-    Context* c = &ctx->c;
-    if (!c->mc_ready) return CAGEO_ERR_STATE;
-    CAGEO_HIP_TRY(hipMemsetAsync(d_scratch, 0, c->n * sizeof(float), c->stream));
-    CAGEO_TRY(cageo_run_realizations(ctx, d_scratch, k, seed, 0));
-    hipLaunchKernelGGL(cageo::k_check_partials, dim3(cageo::RED_BLOCKS), dim3(256), 0, c->stream,
-                       d_prob, d_scratch, (long long)c->n, k, s->part);
-    CAGEO_HIP_TRY(hipGetLastError());
-    std::vector<double> part((size_t)cageo::RED_BLOCKS * 2);
-    CAGEO_HIP_TRY(hipMemcpyAsync(part.data(), s->part, part.size() * sizeof(double),
-                                 hipMemcpyDeviceToHost, c->stream));
-    CAGEO_HIP_TRY(hipStreamSynchronize(c->stream));
-    double a = 0.0, e = 0.0;
-    for (int i = 0; i < cageo::RED_BLOCKS; ++i) {
-        a += part[(size_t)i * 2];
-        e += part[(size_t)i * 2 + 1];
-    }
-    a /= (double)c->n;
-    e /= (double)c->n;
-    if (l1) *l1 = a;
-    if (noise_l1) *noise_l1 = e;
-    if (accepted) *accepted = a <= e + tolerance;
-    return CAGEO_OK;
+                          uint64_t seed, double bias_tol, double rms_tol, float* d_scratch,
+                          int* accepted, double* bias, double* rms) {
+    if (!s) return CAGEO_ERR_ARG;
+    return cageo_check_probability_map(ctx, d_prob, k, seed, 0, bias_tol, rms_tol, d_scratch,
+                                       accepted, bias, rms);
 }
 
 }  // extern "C"

@@ -290,12 +290,16 @@ __device__ inline int ranked_entry(const int* lst, int cap, uint32_t nfront, uin
 // that pass, and the other parity is the one being read in this pass.
 // With `rank` set, the list is two-ended (see Ranked): a neighbour whose
 // last solve took at least `rank->heavy` row steps goes to the front.
+// With `open` set, a neighbour whose word there is 0 (it can no longer
+// change) is not listed; the word is read with the mark exchange, so the
+// check adds no round trip.
 __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty, int gbase,
                                       int* __restrict__ work_out, uint32_t* count_out,
                                       uint32_t* __restrict__ mark, uint32_t epoch,
                                       uint32_t* rows = nullptr, int rpar = 0,
                                       unsigned long long w64 = 0, unsigned long long e64 = 0,
-                                      const Ranked* rank = nullptr) {
+                                      const Ranked* rank = nullptr,
+                                      const uint32_t* open = nullptr) {
     const int lane = threadIdx.x & 63;
     bool push = false, heavy = true;
     int t = 0;
@@ -303,15 +307,16 @@ __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, in
         const int nx = tx + TDX[lane], ny = ty + TDY[lane];
         if (nx >= 0 && ny >= 0 && nx < ntx && ny < nty) {
             t = gbase + ny * ntx + nx;
-            uint32_t c = 0;
+            uint32_t c = 0, o = 1;
             if (rank) c = rank->cost[t];  // in flight with the atomics below
+            if (open) o = __hip_atomic_load(&open[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (rows) {
                 const unsigned long long m = neighbour_rows(lane, w64, e64);
                 uint32_t* p = rows + 4 * (size_t)t + 2 * rpar;
                 if ((uint32_t)m) atomicOr(p, (uint32_t)m);
                 if ((uint32_t)(m >> 32)) atomicOr(p + 1, (uint32_t)(m >> 32));
             }
-            push = atomicExch(&mark[t], epoch) != epoch;
+            push = atomicExch(&mark[t], epoch) != epoch && o != 0u;
             if (rank) heavy = c >= rank->heavy;
         }
     }

@@ -70,8 +70,14 @@ struct Options {
     int accThreshold = 2000;
     bool fastConvolution = false;
     std::string surrogate;
-    int surrogateCheck = 4;
-    double surrogateTolerance = 0.01;
+    int surrogateCheck = 8;
+    double surrogateL1 = 0.02;
+    double surrogateRmsRatio = 2.36;
+    int surrogateWindow = 0;
+    int surrogateRefine = 200;
+    std::string batchFile;
+    std::string batchMode = "graph";
+    bool batchTune = false;
 };
 
 const char* kHelp =
@@ -106,8 +112,24 @@ const char* kHelp =
     "                                      predict, check with a short Monte Carlo, and\n"
     "                                      write probability x iterations; the full Monte\n"
     "                                      Carlo runs if the check fails\n"
-    "  --surrogate-check arg (=4)          Realizations of the check\n"
-    "  --surrogate-tolerance arg (=0.01)   Allowed L1 above the check's sampling noise\n";
+    "  --surrogate-check arg (=8)          Realizations of the check (>= 2)\n"
+    "  --surrogate-l1 arg (=0.02)          Mean absolute error the map may have: the\n"
+    "                                      check accepts a mean signed error up to half\n"
+    "                                      of it and an RMS error up to 1.2 x ratio x it\n"
+    "  --surrogate-rms-ratio arg (=2.36)   RMS / mean absolute error of the network's\n"
+    "                                      errors (measured for the packaged weights)\n"
+    "  --surrogate-window arg (=0)         Check windows of N x N cells; failing windows\n"
+    "                                      take a --surrogate-refine Monte Carlo (0: one\n"
+    "                                      window, the full Monte Carlo on failure)\n"
+    "  --surrogate-refine arg (=200)       Realizations that refine failing windows\n"
+    "  --batch arg                         File of tiles, one per line: DEM name, stream\n"
+    "                                      name [, error std, practical range [, filter\n"
+    "                                      type]] (rasters under --dem-path; missing\n"
+    "                                      fields from the options above); all tiles run\n"
+    "                                      --iterations realizations together, tile t\n"
+    "                                      seeded seed + t; output <name>_<DEM>.bin|.jpg\n"
+    "  --batch-mode arg (=graph)           graph | stream | per-tile\n"
+    "  --batch-tune                        Autotune the launch settings first\n";
 
 [[noreturn]] void fail(const std::string& msg) {
     std::cout << msg << std::endl;
@@ -147,7 +169,7 @@ Options parseArgs(int argc, char** argv) {
     const std::map<std::string, std::string> shortNames = {{"i", "iterations"}, {"o", "output-file"}};
     const std::map<std::string, bool> implicitBool = {{"gpu", true}, {"cpu", true}, {"staged", true},
                                                       {"timing", true}, {"help", true},
-                                                      {"fast-convolution", true}};
+                                                      {"fast-convolution", true}, {"batch-tune", true}};
     for (int k = 1; k < argc; ++k) {
         std::string a = argv[k], name, value;
         bool hasValue = false;
@@ -174,7 +196,9 @@ Options parseArgs(int argc, char** argv) {
             "help", "dem", "stream", "dem-path", "stream-path", "filter-type", "practical-range",
             "error-mean", "error-standard-deviation", "iterations", "gpu", "cpu", "staged", "timing",
             "output-file", "output-option", "seed", "checkpoint", "checkpoint-every", "resume", "synth",
-            "fast-convolution", "surrogate", "surrogate-check", "surrogate-tolerance"};
+            "fast-convolution", "surrogate", "surrogate-check", "surrogate-l1",
+            "surrogate-rms-ratio", "surrogate-window", "surrogate-refine", "batch", "batch-mode",
+            "batch-tune"};
         bool isKnown = false;
         for (const char* kname : known) isKnown = isKnown || name == kname;
         if (!isKnown) fail("unrecognised option '--" + name + "'");
@@ -208,7 +232,13 @@ Options parseArgs(int argc, char** argv) {
         else if (name == "fast-convolution") o.fastConvolution = hasValue ? parseBool(value) : true;
         else if (name == "surrogate") o.surrogate = value;
         else if (name == "surrogate-check") o.surrogateCheck = parseNum<int>(name, value);
-        else if (name == "surrogate-tolerance") o.surrogateTolerance = parseNum<double>(name, value);
+        else if (name == "surrogate-l1") o.surrogateL1 = parseNum<double>(name, value);
+        else if (name == "surrogate-rms-ratio") o.surrogateRmsRatio = parseNum<double>(name, value);
+        else if (name == "surrogate-window") o.surrogateWindow = parseNum<int>(name, value);
+        else if (name == "surrogate-refine") o.surrogateRefine = parseNum<int>(name, value);
+        else if (name == "batch") o.batchFile = value;
+        else if (name == "batch-mode") o.batchMode = value;
+        else if (name == "batch-tune") o.batchTune = hasValue ? parseBool(value) : true;
         else if (name == "synth") {
             int w = 0, h = 0;
             unsigned long long s = 0;
@@ -244,8 +274,17 @@ Options parseArgs(int argc, char** argv) {
     if (o.useCPU) o.useGPU = false;
     if (o.iterations < 0) fail("The number of iterations must not be negative");
     if (o.checkpointEvery < 0) fail("--checkpoint-every must not be negative");
-    if (o.surrogateCheck < 1 || o.surrogateCheck > 1000) fail("--surrogate-check must be in 1..1000");
+    if (o.surrogateCheck < 2 || o.surrogateCheck > 1000) fail("--surrogate-check must be in 2..1000");
+    if (o.surrogateL1 < 0 || o.surrogateRmsRatio < 0) fail("--surrogate-l1 and --surrogate-rms-ratio must not be negative");
+    if (o.surrogateWindow < 0) fail("--surrogate-window must not be negative");
+    if (o.surrogateRefine < 1) fail("--surrogate-refine must be positive");
     if (!o.surrogate.empty() && o.staged) fail("--surrogate runs on the fused path, not with --staged");
+    if (o.batchMode != "graph" && o.batchMode != "stream" && o.batchMode != "per-tile")
+        fail("Unknown batch mode: " + o.batchMode + " (graph, stream or per-tile)");
+    if (!o.batchFile.empty() && (o.staged || !o.surrogate.empty() || !o.resume.empty() ||
+                                 !o.checkpoint.empty() || o.outputOption == Options::OUT_FRAMES))
+        fail("--batch runs the fused batched path only (no --staged, --surrogate, --resume, "
+             "--checkpoint or frames output)");
     return o;
 }
 
@@ -321,18 +360,17 @@ void writeOutput(const Options& o, int iter, CatchmentBorder_t* mc,
     }
 }
 
-// Surrogate prediction with its Monte Carlo check (cageo_surrogate_*).  On
-// acceptance `mc` holds probability x iterations (the expected border count
-// of the run) and true is returned; otherwise the caller runs the Monte
-// Carlo.
+// Surrogate prediction with its Monte Carlo check (cageo_surrogate_*,
+// cageo_check_probability_map).  With one window: on acceptance `mc` holds
+// probability x iterations (the expected border count of the run) and true is
+// returned, otherwise the caller runs the Monte Carlo.  With windows, failing
+// windows take the frequency of a --surrogate-refine Monte Carlo and true is
+// returned.
 bool runSurrogate(const Options& o, cageo_context* ctx, int W, int H, RasterDEM_t& dem,
                   StreamData_t& streams, CatchmentBorder_t* mc) {
     const auto t0 = std::chrono::steady_clock::now();
     cageo_surrogate* s = nullptr;
     check(cageo_surrogate_create(&s, o.surrogate.c_str(), nullptr), "surrogate weights");
-    if (o.filterModel != GAUSSIAN)
-        std::cout << "Surrogate: note, the packaged weights are trained for the Gaussian filter"
-                  << std::endl;
     float *dProb = nullptr, *dScratch = nullptr;
     hip_check(hipMalloc(&dProb, (size_t)W * H * sizeof(float)), "hipMalloc");
     hip_check(hipMalloc(&dScratch, (size_t)W * H * sizeof(float)), "hipMalloc");
@@ -340,18 +378,40 @@ bool runSurrogate(const Options& o, cageo_context* ctx, int W, int H, RasterDEM_
                                   (float)G::noDataValueDEM, (float)G::noDataValueStream, o.errorStd,
                                   o.practicalRange, dProb),
           "surrogate prediction");
-    int accepted = 0;
-    double l1 = 0.0, noise = 0.0;
-    check(cageo_surrogate_check(s, ctx, dProb, o.surrogateCheck, (uint64_t)o.seed ^ 0x5eedull,
-                                o.surrogateTolerance, dScratch, &accepted, &l1, &noise),
+    const double biasTol = 0.5 * o.surrogateL1, rmsTol = 1.2 * o.surrogateRmsRatio * o.surrogateL1;
+    int nwx = 1, nwy = 1;
+    check(cageo_check_windows(ctx, o.surrogateWindow, &nwx, &nwy), "check windows");
+    std::vector<int> accepted((size_t)nwx * nwy);
+    std::vector<double> bias(accepted.size()), rms(accepted.size());
+    check(cageo_check_probability_map(ctx, dProb, o.surrogateCheck, (uint64_t)o.seed ^ 0x5eedull,
+                                      o.surrogateWindow, biasTol, rmsTol, dScratch, accepted.data(),
+                                      bias.data(), rms.data()),
           "surrogate check");
+    int failed = 0;
+    for (int a : accepted) failed += a == 0;
+    std::cout << "Surrogate: " << o.surrogateCheck << "-realization check of " << accepted.size()
+              << " window(s): ";
+    if (accepted.size() == 1)
+        std::cout << "mean error " << std::setprecision(5) << bias[0] << " (|.| <= " << biasTol
+                  << "), RMS error " << rms[0] << " (<= " << rmsTol << ") -> "
+                  << (failed ? "rejected" : "accepted");
+    else
+        std::cout << failed << " failed";
+    bool done = failed == 0;
+    if (failed && o.surrogateWindow > 0) {
+        check(cageo_refine_probability_map(ctx, dProb, o.surrogateWindow, accepted.data(),
+                                           o.surrogateRefine, (uint64_t)o.seed ^ 0x7e1eull,
+                                           dScratch),
+              "surrogate refinement");
+        std::cout << ", refined by a " << o.surrogateRefine << "-realization Monte Carlo";
+        done = true;
+    } else if (failed) {
+        std::cout << ", running the Monte Carlo";
+    }
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    std::cout << "Surrogate: " << o.surrogateCheck << "-realization check L1 " << std::setprecision(5)
-              << l1 << " vs sampling noise " << noise << " + " << o.surrogateTolerance << " -> "
-              << (accepted ? "accepted" : "rejected, running the Monte Carlo") << " ("
-              << std::setprecision(1) << std::fixed << ms << " ms)" << std::endl;
+    std::cout << " (" << std::setprecision(1) << std::fixed << ms << " ms)" << std::endl;
     std::cout.unsetf(std::ios::fixed);
-    if (accepted) {
+    if (done) {
         std::vector<float> p((size_t)W * H);
         hip_check(hipMemcpy(p.data(), dProb, p.size() * sizeof(float), hipMemcpyDeviceToHost),
                   "hipMemcpy");
@@ -361,7 +421,7 @@ bool runSurrogate(const Options& o, cageo_context* ctx, int W, int H, RasterDEM_
     (void)hipFree(dProb);
     (void)hipFree(dScratch);
     cageo_surrogate_destroy(s);
-    return accepted != 0;
+    return done;
 }
 
 void synthesize(const Options& o) {
@@ -383,6 +443,133 @@ void synthesize(const Options& o) {
               << o.synthSeed << ")" << std::endl;
 }
 
+/* ---- batched form: many tiles x realizations in one run -------------- */
+struct BatchTileSpec {
+    std::string dem, stream;
+    float errorStd, practicalRange;
+    FilterModel model;
+};
+
+std::vector<BatchTileSpec> readBatchFile(const Options& o) {
+    std::ifstream f(o.batchFile.c_str());
+    if (!f.is_open()) fail("Unable to open the batch file \"" + o.batchFile + "\"");
+    std::vector<BatchTileSpec> tiles;
+    std::string line;
+    int lineNo = 0;
+    while (std::getline(f, line)) {
+        ++lineNo;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        std::istringstream is(line);
+        BatchTileSpec t{"", "", o.errorStd, o.practicalRange, o.filterModel};
+        if (!(is >> t.dem)) continue;  // blank line
+        std::string filter;
+        if (!(is >> t.stream)) fail("Batch file line " + std::to_string(lineNo) + ": no stream name");
+        if (is >> t.errorStd) {
+            if (!(is >> t.practicalRange))
+                fail("Batch file line " + std::to_string(lineNo) + ": error std without practical range");
+            if (is >> filter) {
+                if (filter == "Exponential") t.model = EXPONENTIAL;
+                else if (filter == "Gaussian") t.model = GAUSSIAN;
+                else fail("Batch file line " + std::to_string(lineNo) + ": unknown filter type " + filter);
+            }
+        }
+        tiles.push_back(t);
+    }
+    if (tiles.empty()) fail("The batch file lists no tiles");
+    return tiles;
+}
+
+int runBatch(Options& o) {
+    const std::vector<BatchTileSpec> spec = readBatchFile(o);
+    const int T = (int)spec.size();
+    if (!o.seedGiven) o.seed = (uint32_t)std::time(nullptr);
+    std::cout << "Batch: " << T << " tiles, " << o.iterations << " realizations each, mode "
+              << o.batchMode << ", seed " << o.seed << std::endl;
+    std::vector<int> widths(T), heights(T);
+    std::vector<DemInfo> info(T);
+    std::vector<float*> dDem(T, nullptr), dStream(T, nullptr), dMc(T, nullptr);
+    for (int t = 0; t < T; ++t) {
+        DemInfo si;
+        loadHeaderData(o.demPath + "/" + spec[t].dem + ".hdr", &info[t]);
+        loadHeaderData(o.demPath + "/" + spec[t].stream + ".hdr", &si);
+        if (info[t].fileCols != si.fileCols || info[t].fileRows != si.fileRows)
+            fail("The sizes of the DEM and stream data of tile " + spec[t].dem + " don't match");
+        if (info[t].cellSize != si.cellSize)
+            fail("The cell sizes of the DEM and stream data of tile " + spec[t].dem + " don't match");
+        widths[t] = info[t].fileCols;
+        heights[t] = info[t].fileRows;
+        const size_t n = (size_t)widths[t] * heights[t];
+        std::vector<float> dem(n), st(n);
+        readRaster(o.demPath + "/" + spec[t].dem + ".bin", dem.data(), n);
+        readRaster(o.demPath + "/" + spec[t].stream + ".bin", st.data(), n);
+        hip_check(hipMalloc(&dDem[t], n * sizeof(float)), "hipMalloc");
+        hip_check(hipMalloc(&dStream[t], n * sizeof(float)), "hipMalloc");
+        hip_check(hipMalloc(&dMc[t], n * sizeof(float)), "hipMalloc");
+        hip_check(hipMemcpy(dDem[t], dem.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(dStream[t], st.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemset(dMc[t], 0, n * sizeof(float)), "hipMemset");
+    }
+    cageo_batch* b = nullptr;
+    check(cageo_batch_create(&b, T, widths.data(), heights.data(), nullptr), "batch");
+    for (int t = 0; t < T; ++t) {
+        DemInfo si;
+        loadHeaderData(o.demPath + "/" + spec[t].stream + ".hdr", &si);
+        check(cageo_batch_setup_tile(b, t, dDem[t], dStream[t], info[t].noDataValue, si.noDataValue,
+                                     o.bevelRadius, spec[t].model, spec[t].practicalRange,
+                                     info[t].cellSize, o.errorMean, spec[t].errorStd,
+                                     (uint64_t)o.seed + (uint64_t)t),
+              ("batch tile " + spec[t].dem).c_str());
+    }
+    if (o.batchTune) {
+        std::vector<float*> scratch(T);
+        for (int t = 0; t < T; ++t) {
+            hip_check(hipMalloc(&scratch[t], (size_t)widths[t] * heights[t] * sizeof(float)), "hipMalloc");
+            hip_check(hipMemset(scratch[t], 0, (size_t)widths[t] * heights[t] * sizeof(float)), "hipMemset");
+        }
+        check(cageo_batch_tune(b, scratch.data(), 1, nullptr), "batch tune");
+        for (float* p : scratch) (void)hipFree(p);
+    }
+    const int mode = o.batchMode == "graph" ? CAGEO_BATCH_GRAPH
+                     : o.batchMode == "stream" ? CAGEO_BATCH_STREAM : CAGEO_BATCH_PER_TILE;
+    const auto t0 = std::chrono::steady_clock::now();
+    check(cageo_batch_run(b, dMc.data(), o.iterations, 0u, mode), "batch run");
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    double cells = 0.0;
+    for (int t = 0; t < T; ++t) cells += (double)widths[t] * heights[t];
+    std::cout << "Realizations: " << T << " tiles x " << o.iterations << " in " << std::fixed
+              << std::setprecision(1) << ms << " ms (" << std::setprecision(2)
+              << (ms > 0 ? 1000.0 * T * o.iterations / ms : 0.0) << " tile realizations/s, "
+              << (ms > 0 ? 1e-3 * cells * o.iterations / ms : 0.0) << " Mcells/s)" << std::endl;
+    std::cout.unsetf(std::ios::fixed);
+    for (int t = 0; t < T; ++t) {
+        const size_t n = (size_t)widths[t] * heights[t];
+        const std::string name = o.outputName + "_" + spec[t].dem;
+        if (o.jpeg) {
+            cageo_context* ctx = nullptr;
+            check(cageo_batch_context(b, t, &ctx), "batch context");
+            uint8_t* dBytes = nullptr;
+            hip_check(hipMalloc(&dBytes, n), "hipMalloc");
+            check(cageo_to_byte_array(ctx, dBytes, dMc[t]), "byte array");
+            std::vector<uint8_t> bytes(n);
+            hip_check(hipMemcpy(bytes.data(), dBytes, n, hipMemcpyDeviceToHost), "hipMemcpy");
+            (void)hipFree(dBytes);
+            writeJpegGray(name + ".jpg", bytes.data(), widths[t], heights[t]);
+        } else {
+            std::vector<float> mc(n);
+            hip_check(hipMemcpy(mc.data(), dMc[t], n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+            writeRaster(name + ".bin", mc.data(), n);
+        }
+    }
+    cageo_batch_destroy(b);
+    for (int t = 0; t < T; ++t) {
+        (void)hipFree(dDem[t]);
+        (void)hipFree(dStream[t]);
+        (void)hipFree(dMc[t]);
+    }
+    return 0;
+}
+
 int run(int argc, char** argv) {
     Options o = parseArgs(argc, argv);
     if (!o.useGPU) {
@@ -402,6 +589,7 @@ int run(int argc, char** argv) {
               << std::endl;
 
     if (o.synthW > 0) synthesize(o);
+    if (!o.batchFile.empty()) return runBatch(o);
 
     // main.cpp:87-108; the stream files are read from the DEM path
     DemInfo demInfo, streamInfo;

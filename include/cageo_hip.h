@@ -343,15 +343,34 @@ int cageo_surrogate_forward(cageo_surrogate* s, const uint16_t* d_x, int h, int 
 int cageo_surrogate_predict(cageo_surrogate* s, const float* d_dem, const float* d_stream, int W,
                             int H, float nodata_dem, float nodata_stream, float rmse,
                             float practical_range, float* d_prob);
-/* Monte Carlo check of a probability map: k realizations of `ctx` (set up
- * with cageo_mc_setup on the same DEM; counts into d_scratch, W x H) give
- * l1 = mean |p - count/k| and noise_l1 = mean E|B/k - p|, B ~ Binomial(k,
- * p) (the distance sampling alone would give if p were exact).  accepted =
- * l1 <= noise_l1 + tolerance; on rejection the caller runs the Monte Carlo
- * instead. */
+/* Monte Carlo check of a probability map d_prob (W x H floats of the
+ * context's grid), no reference counterpart.  k >= 2 realizations of `ctx`
+ * (set up with cageo_mc_setup and cageo_set_filter; counts into d_scratch,
+ * W x H floats) give f = count / k per cell, and per window
+ *   bias = mean(f - q)                             (expectation mean(p - q))
+ *   rms  = sqrt(max(mean((f - q)^2 - f (1 - f) / (k - 1)), 0))
+ *          (the mean of an unbiased estimate of (p - q)^2 per cell)
+ * with p the true border probability: neither is inflated by the sampling
+ * noise of a small k.  accepted = |bias| <= bias_tol and rms <= rms_tol.
+ * Windows are `window` x `window` cells in row-major order, ragged at the
+ * right and bottom edges; window <= 0 makes the whole raster one window.
+ * accepted, bias, rms (each optional) hold one entry per window:
+ * cageo_check_windows gives their count. */
+int cageo_check_windows(cageo_context* ctx, int window, int* nwx, int* nwy);
+int cageo_check_probability_map(cageo_context* ctx, const float* d_prob, int k, uint64_t seed,
+                                int window, double bias_tol, double rms_tol, float* d_scratch,
+                                int* accepted, double* bias, double* rms);
+/* Refinement of the windows that failed the check: n realizations of the
+ * Monte Carlo over the whole DEM (a window's labels depend on flow paths that
+ * leave it) into d_scratch, whose frequency count / n replaces d_prob in every
+ * window with accepted[w] == 0 (host array, one flag per window as above).
+ * Nothing runs when every window was accepted. */
+int cageo_refine_probability_map(cageo_context* ctx, float* d_prob, int window,
+                                 const int* accepted, int n, uint64_t seed, float* d_scratch);
+/* The check above on the whole raster (the surrogate's own map). */
 int cageo_surrogate_check(cageo_surrogate* s, cageo_context* ctx, const float* d_prob, int k,
-                          uint64_t seed, double tolerance, float* d_scratch, int* accepted,
-                          double* l1, double* noise_l1);
+                          uint64_t seed, double bias_tol, double rms_tol, float* d_scratch,
+                          int* accepted, double* bias, double* rms);
 
 #ifdef __cplusplus
 }

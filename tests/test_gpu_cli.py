@@ -150,7 +150,9 @@ def test_cli_surrogate_accept_and_fallback(inputs, torch_cuda):
     """--surrogate: an accepted prediction is written as probability x
     iterations (the C-ABI prediction of the same inputs); a rejected one
     leaves the output to the Monte Carlo, equal to a run without the
-    surrogate.  Acceptance and rejection are forced by the tolerance."""
+    surrogate.  Acceptance and rejection are forced by the L1 target (10:
+    any map passes; 0: only an exact one would).  A windowed run refines
+    its failing windows with a Monte Carlo instead of rejecting."""
     import cageo_amd as ca
     from cageo_amd import surrogate as sg
     torch = torch_cuda
@@ -158,7 +160,7 @@ def test_cli_surrogate_accept_and_fallback(inputs, torch_cuda):
     announce("cli surrogate accept / fallback")
     common = ["--dem-path", ".", "-i", "6", "--seed", "9", "--error-standard-deviation", "1.5",
               "--surrogate", sg.DEFAULT_WEIGHTS]
-    r = run_cli(d, *common, "--surrogate-tolerance", "10", "-o", "sur_ok.bin")
+    r = run_cli(d, *common, "--surrogate-l1", "10", "-o", "sur_ok.bin")
     assert "accepted" in r.stdout, r.stdout
     got = read_bin(os.path.join(d, "sur_ok.bin"))
     dem = torch.from_numpy(read_bin(os.path.join(d, "dem10.bin"))).cuda()
@@ -168,12 +170,20 @@ def test_cli_surrogate_accept_and_fallback(inputs, torch_cuda):
         s.predict(dem, st, -9999.0, -9999.0, 1.5, 60.0, p)
     torch.cuda.synchronize()
     assert np.array_equal(got, p.cpu().numpy() * np.float32(6))
-    r = run_cli(d, *common, "--surrogate-tolerance", "-1", "-o", "sur_no.bin")
+    r = run_cli(d, *common, "--surrogate-l1", "0", "-o", "sur_no.bin")
     assert "rejected" in r.stdout, r.stdout
     run_cli(d, "--dem-path", ".", "-i", "6", "--seed", "9", "--error-standard-deviation", "1.5",
             "-o", "plain.bin")
     assert np.array_equal(read_bin(os.path.join(d, "sur_no.bin")),
                           read_bin(os.path.join(d, "plain.bin")))
+    # windows of 64 cells, all failing at target 0: every window takes the
+    # refinement's frequency (count / 16) x iterations
+    r = run_cli(d, *common, "--surrogate-l1", "0", "--surrogate-window", "64",
+                "--surrogate-refine", "16", "-o", "sur_win.bin")
+    assert "failed" in r.stdout and "refined" in r.stdout, r.stdout
+    win = read_bin(os.path.join(d, "sur_win.bin"))
+    counts = win / np.float32(6) * np.float32(16)
+    assert np.allclose(counts, np.round(counts), atol=1e-3) and counts.max() <= 16
 
 
 def test_cli_fast_convolution(inputs, torch_cuda):
@@ -188,3 +198,44 @@ def test_cli_fast_convolution(inputs, torch_cuda):
     a, b = read_bin(os.path.join(d, "exact.bin")), read_bin(os.path.join(d, "fast.bin"))
     assert b.min() >= 0 and b.max() <= 3
     assert abs(float(b.sum()) - float(a.sum())) <= 0.01 * float(a.sum())
+
+
+def test_cli_batch(tmp_path, torch_cuda):
+    """--batch: three tiles of different sizes and error models listed in a
+    file run together (graph mode, tuned); tile t's counter raster equals
+    the fused single-tile run of that DEM with seed + t, and the oracle's
+    loop.  A tile line may omit the error model (then the options give it)."""
+    d = str(tmp_path)
+    announce("cli batch")
+    sizes = [(200, 130, 5), (96, 160, 6), (128, 128, 7)]
+    for k, (w, h, s) in enumerate(sizes):
+        run_cli(d, "--synth", f"{w},{h},{s}", "--dem-path", ".", "--dem", f"t{k}",
+                "--stream", f"t{k}s", "-i", "0")
+    with open(os.path.join(d, "tiles.txt"), "w") as f:
+        f.write("# dem stream [error std, practical range, filter]\n")
+        f.write("t0 t0s 1.5 60 Gaussian\n")
+        f.write("t1 t1s 2.5 150 Exponential\n")
+        f.write("t2 t2s   # error model from the options\n")
+    r = run_cli(d, "--batch", "tiles.txt", "--batch-tune", "--dem-path", ".", "-i", "3",
+                "--seed", "40", "--error-standard-deviation", "1.0", "--practical-range", "80",
+                "-o", "bt.bin")
+    assert "3 tiles x 3" in r.stdout, r.stdout
+    models = [(1.5, 60.0, 1), (2.5, 150.0, 0), (1.0, 80.0, 1)]
+    for k, ((w, h, _), (std, pr, model)) in enumerate(zip(sizes, models)):
+        got = np.fromfile(os.path.join(d, f"bt_t{k}.bin"), np.float32).reshape(h, w)
+        ftype = "Gaussian" if model == 1 else "Exponential"
+        run_cli(d, "--dem-path", ".", "--dem", f"t{k}", "--stream", f"t{k}s", "-i", "3",
+                "--seed", str(40 + k), "--error-standard-deviation", str(std),
+                "--practical-range", str(pr), "--filter-type", ftype, "-o", f"one{k}.bin")
+        one = np.fromfile(os.path.join(d, f"one{k}.bin"), np.float32).reshape(h, w)
+        assert np.array_equal(got, one), f"tile {k}"
+        dem = np.fromfile(os.path.join(d, f"t{k}.bin"), np.float32).reshape(h, w)
+        st = np.fromfile(os.path.join(d, f"t{k}s.bin"), np.float32).reshape(h, w)
+        import cageo_amd as ca
+        rad = oracle.filter_radius(pr, 10.0)
+        taps = ca.create_filter(model, pr, 10.0, rad)
+        ref = np.zeros((h, w), np.float32)
+        for i in range(3):
+            oracle.lib().or_realization(dem, st, w, h, taps, rad, model, 0.0, std, 40 + k, i,
+                                        -9999.0, -9999.0, 2, ref)
+        assert np.array_equal(got, ref), f"tile {k} vs oracle"

@@ -73,10 +73,13 @@ def test_convolution_bit_exact(ca, torch_cuda, shape, pr):
 
 
 @pytest.mark.parametrize("shape,pr", [((256, 256), 200), ((96, 130), 200), ((200, 300), 60),
-                                      ((1024, 1024), 200), ((17, 17), 30)])
+                                      ((1024, 1024), 200), ((17, 17), 30), ((512, 128), 200),
+                                      ((64, 2048), 30), ((2048, 1024), 260)])
 def test_convolution_fast_mode(ca, torch_cuda, shape, pr):
-    """Fast mode (low-rank separable form of the exponential filter, fused
-    multiply-adds): within 1e-5 of the exact convolution relative to its
+    """Fast mode of the exponential filter: on power-of-two grids the
+    periodic correlation through the FFT of the torus (conv_terms -1),
+    elsewhere the low-rank separable form with fused multiply-adds (1..32
+    terms).  Either is within 1e-5 of the exact convolution relative to its
     largest magnitude; the Gaussian filter, separable already, stays
     bit-exact.  The exact GPU convolution is the reference (it equals the
     oracle bit for bit, test above)."""
@@ -102,7 +105,11 @@ def test_convolution_fast_mode(ca, torch_cuda, shape, pr):
         if model == oracle.GAUSSIAN:
             assert terms == 0 and np.array_equal(fast, exact)
         else:
-            assert 1 <= terms <= 32
+            h2, w2 = (h & (h - 1)) == 0, (w & (w - 1)) == 0
+            if h2 and w2 and w >= 64 and h >= 32 and r <= min(w, h) // 2:
+                assert terms == -1, terms
+            else:
+                assert 1 <= terms <= 32, terms
             err = np.abs(fast.astype(np.float64) - exact).max() / np.abs(exact).max()
             assert err <= 1e-5, err
 

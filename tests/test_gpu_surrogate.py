@@ -237,7 +237,9 @@ def test_native_surrogate_matches_python(env):
     bit-identical forward pass on the same bf16 input; predictions from the
     DEM within the rounding of the input planes (mean / deviation in double
     there, float32 here); the Monte Carlo check statistics equal to the
-    Python check's for the same seed (float vs double sums)."""
+    check statistics (unbiased mean signed error and RMS error, whole map and
+    64 x 64 windows) equal to a numpy evaluation of the batched Monte Carlo's
+    counts for the same seed."""
     import cageo_amd as ca
     torch, sg, L = env
     announce("native surrogate vs python 256x200")
@@ -261,9 +263,21 @@ def test_native_surrogate_matches_python(env):
             ctx.set_filter(ca.GAUSSIAN, r, ca.create_filter(ca.GAUSSIAN, 60.0, 10.0, r))
             ctx.mc_setup(ddem, dst, -9999.0, -9999.0, 2, 0.0, 1.8)
             scratch = torch.empty((256, 200), dtype=torch.float32, device="cuda")
-            acc, l1, noise = s.check(ctx, p, 4, 21, 0.01, scratch)
+            acc, bias, rms = s.check(ctx, p, 4, 21, 0.01, 0.05, scratch)
+            wacc, wbias, wrms = ctx.check_probability_map(p, 4, 21, 0.01, 0.05, scratch,
+                                                          window=64)
+        # the same statistics from the batched Monte Carlo's 4 realizations
         chk = sg.mc_frequency(torch, [dem], [st], [1.8], [60.0], 4, 21, on_device=True)[0]
-        l1_py = float((p - chk).abs().mean())
-        noise_py = sg.expected_l1_of_sample(p.double(), 4)
-        assert abs(l1 - l1_py) <= 1e-6 and abs(noise - noise_py) <= 1e-6, (l1, l1_py, noise, noise_py)
-        assert acc == (l1 <= noise + 0.01)
+        f, q = chk.double().cpu().numpy(), p.double().cpu().numpy()
+        u = (f - q) ** 2 - f * (1 - f) / 3.0
+        b_py, r_py = (f - q).mean(), np.sqrt(max(u.mean(), 0.0))
+        assert abs(bias - b_py) <= 1e-9 and abs(rms - r_py) <= 1e-9, (bias, b_py, rms, r_py)
+        assert acc == (abs(b_py) <= 0.01 and r_py <= 0.05)
+        assert wacc.shape == (4, 4)  # 256 / 64 rows, ceil(200 / 64) columns
+        for wy in range(4):
+            for wx in range(4):
+                sl = (slice(64 * wy, 64 * wy + 64), slice(64 * wx, 64 * wx + 64))
+                bw = (f - q)[sl].mean()
+                rw = np.sqrt(max(u[sl].mean(), 0.0))
+                assert abs(wbias[wy, wx] - bw) <= 1e-9 and abs(wrms[wy, wx] - rw) <= 1e-9
+                assert wacc[wy, wx] == (abs(bw) <= 0.01 and rw <= 0.05)
