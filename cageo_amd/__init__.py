@@ -116,6 +116,7 @@ def lib():
         "cageo_context_set_barrier_timeout": ([vp, ctypes.c_double], i),
         "cageo_context_long_paths": ([vp, ctypes.POINTER(i64)], i),
         "cageo_batch_abort": ([vp, i], i),
+        "cageo_pit_fill_noalloc": ([vp, vp, i, i, f, vp], i),
         "cageo_context_set_conv_mode": ([vp, i], i),
         "cageo_context_conv_terms": ([vp, vp], i),
         "cageo_filter_low_rank": ([vp, i, i, ctypes.c_double, vp, vp, vp, vp], i),
@@ -331,6 +332,16 @@ class Surrogate:
                                            rms_tol, _ptr(scratch), ctypes.byref(acc),
                                            ctypes.byref(b), ctypes.byref(r)), "surrogate_check")
         return bool(acc.value), b.value, r.value
+
+
+def pit_fill_noalloc(filled, dem, nodata_dem, stream=None) -> None:
+    """cageo_pit_fill_noalloc: the pit fill with no context and no device
+    allocation (workspace in the library's own device array)."""
+    h, w = dem.shape
+    sp = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream")
+                                      else stream)
+    _check(lib().cageo_pit_fill_noalloc(_ptr(filled), _ptr(dem), w, h, nodata_dem, sp),
+           "pit_fill_noalloc")
 
 
 def read_header(path: str) -> dict:

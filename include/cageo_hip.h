@@ -121,6 +121,23 @@ int cageo_stream_burn(cageo_context* ctx, float* d_z, const float* d_stream,
  * to d_filled. */
 int cageo_pit_fill(cageo_context* ctx, float* d_filled, float* d_dem, float nodata_dem);
 
+/* Allocation-free variant of cageo_pit_fill.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same computation and result as cageo_pit_fill; no context.  Its workspace
+ * is a file-scope device array of CAGEO_NOALLOC_SCRATCH_BYTES that every
+ * device running the library holds until the process ends; width * height
+ * must be at most CAGEO_NOALLOC_MAX_CELLS and the 64 x 64 tiles covering the
+ * grid at most CAGEO_NOALLOC_MAX_TILES (else CAGEO_ERR_ARG).  Enqueued on
+ * hip_stream (a hipStream_t; NULL = the null stream); returns when done.
+ * Every call on one device shares that device's array, so calls that overlap
+ * on one device race: serialise them. */
+#define CAGEO_NOALLOC_MAX_CELLS (4096 * 4096)
+#define CAGEO_NOALLOC_MAX_TILES 16384
+#define CAGEO_NOALLOC_SCRATCH_BYTES (18 * 1024 * 1024)
+int cageo_pit_fill_noalloc(float* d_filled, float* d_dem, int width, int height,
+                           float nodata_dem, void* hip_stream);
+
 /* FlowRoutingAlgorithm<T,U>::execute(flowDir, filled), FlowRoutingAlgorithm.h:
  * 69-94 (D8 FlowRoutingAlgorithmCPUOpenMP.h:53-151, flats :153-249).
  * d_flowdir: 2*W*H int8 (char2 {x, y}).  nodata_dem is accepted for the

@@ -98,3 +98,13 @@ def test_filter_low_rank_refuses_asymmetric_and_short_budgets():
         ca.filter_low_rank(g, r)
     with pytest.raises(ca.CageoError):  # two terms cannot reach 1e-6
         ca.filter_low_rank(f, r, kmax=2)
+
+
+def test_pit_fill_noalloc_refuses_grids_beyond_its_workspace():
+    """cageo_pit_fill_noalloc checks the grid against its fixed workspace
+    before any device call: too many cells, or too many 64 x 64 tiles for a
+    skinny grid, give CAGEO_ERR_ARG (the pointers are never touched)."""
+    lib = ca.lib()
+    for w, h in ((4097, 4096), (1 << 21, 8), (0, 10), (10, -1)):
+        assert lib.cageo_pit_fill_noalloc(16, 32, w, h, -9999.0, None) == -1, (w, h)
+    assert lib.cageo_pit_fill_noalloc(16, 16, 8, 8, -9999.0, None) == -1  # filled aliases dem
