@@ -14,8 +14,9 @@ Prints one JSON line.  `value` is realizations/s of the Gaussian workload on
 one GPU.  Per-stage device times come from HIP events that the library
 records on its stream between the stages of every timed realization (no host
 synchronisation inside the timed region).  `roofline` covers the dominant
-kernel: algorithmic bytes per launch / its average launch time.  With
---traffic, `traffic` is the HBM bytes per launch of that kernel from two
+stage: algorithmic bytes per launch / its average launch time (the pit
+fill is two kernels, k_pf_persistent and k_pf_resident).  Unless
+--no-traffic, `traffic` is the HBM bytes per launch of those kernels from two
 rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over a short probe
 run (`--probe`), scaled by what the same counters read for an elementwise
 kernel that reads and writes 512 MiB; otherwise it is null.
@@ -61,11 +62,12 @@ STAGE_BYTES = {
 }
 STAGE_NAMES = ["random_field", "convolution", "realistic_burn", "pit_fill", "flow_routing",
                "flow_tracing", "border"]
-# Stages that are exactly one kernel launch (the events bracket that launch).
+# The kernel launches each stage's events bracket (the pit fill's resident
+# kernel ends at once when the first kernel converged alone).
 STAGE_KERNEL = {
-    "random_field": "k_random_field", "realistic_burn": "k_realistic_burn_quant",
-    "pit_fill": "k_pf_persistent", "flow_routing": "k_fr_persistent",
-    "flow_tracing": "k_tr_persistent", "border": "k_border",
+    "random_field": ("k_random_field",), "realistic_burn": ("k_realistic_burn_quant",),
+    "pit_fill": ("k_pf_persistent", "k_pf_resident"), "flow_routing": ("k_fr_persistent",),
+    "flow_tracing": ("k_tr_persistent",), "border": ("k_border",),
 }
 
 
@@ -197,10 +199,12 @@ def counter_pass(counter, args, outdir):
     return vals, None
 
 
-def measure_traffic(args, kernel):
-    """HBM bytes per launch of `kernel` (read and write), each counter scaled
-    by what it reads for the calibration kernel (an elementwise multiply that
-    reads and writes CALIB_BYTES).  Returns (result or None, reason)."""
+def measure_traffic(args, kernels):
+    """HBM bytes per launch of the stage whose launches are `kernels` (read
+    and write; the per-launch averages of its kernels summed), each counter
+    scaled by what it reads for the calibration kernel (an elementwise
+    multiply that reads and writes CALIB_BYTES).  Returns (result or None,
+    reason)."""
     if not shutil.which("rocprofv3"):
         return None, "rocprofv3 not found"
     outdir = args.traffic_out or tempfile.mkdtemp(prefix="cageo_pmc_")
@@ -212,11 +216,13 @@ def measure_traffic(args, kernel):
             vals, err = counter_pass(counter, args, outdir)
             if vals is None:
                 return None, err
-            k = [v for name, v in vals.items() if kernel in name]
             cal = [v for name, v in vals.items() if "AUnaryFunctor" in name]
-            if not k or not cal:
-                return None, f"{counter}: kernels not found in {sorted(vals)[:8]}"
-            per_launch_kib = sum(k[0]) / len(k[0])
+            per_launch_kib = 0.0
+            for kernel in kernels:
+                k = [v for name, v in vals.items() if kernel in name]
+                if not k or not cal:
+                    return None, f"{counter}: {kernel} not found in {sorted(vals)[:8]}"
+                per_launch_kib += sum(k[0]) / len(k[0])
             cal_kib = max(cal[0])
             factor = CALIB_BYTES / (cal_kib * 1024.0)
             res[counter] = {"kib_per_launch": per_launch_kib, "calibration_kib": cal_kib,
@@ -449,6 +455,10 @@ def surrogate_leg(torch, ca, dem, stream, tdem, tst, args):
     }
 
 
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -496,6 +506,7 @@ def main():
         ctx.close()
         return dt, stages, total
 
+    log(f"Gaussian PR 60 m, {args.steps} realizations")
     dt, stages, total = run(ca.GAUSSIAN, 60.0, args.steps, args.warmup,
                             not args.no_stage_times)
     real_s = args.steps / dt
@@ -524,10 +535,11 @@ def main():
     if stages:
         line["stage_ms"] = stages
         key = max(STAGE_KERNEL, key=lambda k: stages[k])
-        kernel = STAGE_KERNEL[key]
+        kernels = STAGE_KERNEL[key]
         bytes_per_launch = STAGE_BYTES[key] * cells
         achieved = bytes_per_launch / (stages[key] * 1e-3) / 1e9
-        traffic, why = (measure_traffic(args, kernel) if not args.no_traffic
+        log("counter passes" if not args.no_traffic else "counter passes skipped")
+        traffic, why = (measure_traffic(args, kernels) if not args.no_traffic
                         else (None, "not requested"))
         if traffic is None and not args.no_traffic:
             print(f"bench.py: traffic not measured: {why}", file=sys.stderr)
@@ -536,7 +548,7 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": (round(traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"])
                         if traffic else None),
-            "kernel": kernel, "launch_ms": stages[key],
+            "kernel": " + ".join(kernels), "launch_ms": stages[key],
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "bytes_per_cell": STAGE_BYTES[key],
         }
@@ -557,6 +569,7 @@ def main():
         "note": "80 B/cell per realization (one pass over every stage's boundary I/O at the "
                 "reference's data types, fused stages counted unfused) over the whole step"}
     if args.exp_steps > 0:
+        log(f"exponential PR 200 m, {args.exp_steps} realizations, exact and fast")
         dte, st_e, _ = run(ca.EXPONENTIAL, 200.0, args.exp_steps, 1, not args.no_stage_times)
         line["exponential_pr200"] = {
             "realizations_per_s": round(args.exp_steps / dte, 3),
@@ -575,10 +588,22 @@ def main():
                            ca.CONV_FAST)
         fast = {"realizations_per_s": round(args.exp_steps / dtf, 3),
                 "mcells_per_s": round(cells * args.exp_steps / dtf / 1e6, 2),
-                "terms": run.terms, "stage_ms": st_f,
-                "note": "low-rank separable form of the filter with fused multiply-adds, within "
+                "form": "fft" if run.terms < 0 else "low-rank", "stage_ms": st_f,
+                "note": "periodic correlation through FFTs of the W x H torus (power-of-two "
+                        "sides), within 1e-5 of the exact convolution (not bit-exact)"
+                        if run.terms < 0 else
+                        "low-rank separable form of the filter with fused multiply-adds, within "
                         "1e-5 of the exact convolution (not bit-exact)"}
-        if st_f:
+        if st_f and run.terms < 0:
+            b = 24 * cells  # three passes, each reading and writing 4 B per cell
+            gbs = b / (st_f["convolution"] * 1e-3) / 1e9
+            fast["convolution_roofline"] = {
+                "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": 24,
+                "note": "k_fft_rows_fwd + k_fft_cols + k_fft_rows_inv: row FFTs to the half "
+                        "spectrum, column FFT x filter spectrum x inverse column FFT, inverse "
+                        "row FFTs; 8 B per cell each"}
+        elif st_f:
             K, L = run.terms, 2 * ca.filter_radius(200.0, 10.0) + 1
             flops = 2.0 * 2 * K * L * cells  # column and row passes, one FMA = 2 FLOP
             tf = flops / st_f["convolution"] / 1e9
@@ -588,12 +613,15 @@ def main():
                 "note": "k_lr_cols + k_lr_rows, 2 x K x (2r+1) FMA per cell"}
         line["exponential_pr200"]["fast_mode"] = fast
     if args.batch_tiles > 0:
+        log(f"batched form, {args.batch_tiles} tiles")
         line["batched"] = batched_leg(torch, ca, args)
     if args.surrogate_realizations > 0:
+        log("surrogate vs Monte Carlo")
         line["surrogate"] = surrogate_leg(torch, ca, dem, stream, tdem, tst, args)
     if args.large_size > 0:
         line["large_tile"] = large_leg(torch, ca, args)
     if args.cpu_sample > 0 and oracle_ok:
+        log(f"CPU baseline, {args.cpu_sample} realizations")
         cps, secs = cpu_baseline(size, args.cpu_sample, args.seed)
         line["cpu_baseline"] = {
             "value": round(cps, 5), "unit": "realizations/s", "cores": 1, "kind": "port",

@@ -204,8 +204,14 @@ constexpr int BAR_GEN = BAR_STRIDE;
 constexpr int BAR_ABORT = 2 * BAR_STRIDE;
 constexpr int BAR_GROUP = 3 * BAR_STRIDE;
 constexpr int BAR_WORDS = (3 + BAR_GROUPS) * BAR_STRIDE;
-// Polls between two checks of the abort words and the clock.
+// Polls between two checks of the device abort word and the clock, and
+// between two reads of the host's abort word.  The host word lives in host
+// memory: a read crosses PCIe, and one from every workgroup at every
+// barrier cost the pit fill 2 ms of its 7 ms at 4096^2 on an MI355X.  So a
+// launch reads it once at its start (workgroup 0, grid_aborted) and then
+// only in waits that have lasted this many polls (a stalled launch).
 constexpr uint32_t BAR_CHECK_EVERY = 64;
+constexpr uint32_t BAR_HOST_EVERY = 16 * BAR_CHECK_EVERY;
 // Default bound of one barrier wait, in seconds.
 constexpr double BAR_DEFAULT_TIMEOUT_S = 5.0;
 
@@ -254,12 +260,12 @@ __device__ inline bool grid_sync(const GridBar& gb, uint32_t n = 0) {
             }
         }
         const uint64_t t0 = wall_clock64();
-        for (uint32_t it = 0;
+        for (uint32_t it = 1;
              __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g; ++it) {
-            // the first check at once (an abandoned launch is left without a
-            // full poll period), the host word every fourth check
-            if (it % BAR_CHECK_EVERY == 0 &&
-                bar_should_stop(gb, t0, it % (4 * BAR_CHECK_EVERY) == 0)) {
+            // a zero bound gives up at the first poll that finds the
+            // barrier closed
+            if ((it % BAR_CHECK_EVERY == 0 || gb.limit == 0) &&
+                bar_should_stop(gb, t0, it % BAR_HOST_EVERY == 0)) {
                 __hip_atomic_store(&bar[BAR_ABORT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 atomicCAS(gb.status, 0u, (uint32_t)(-CAGEO_ERR_ABORTED));
                 fail = 1;
@@ -273,12 +279,22 @@ __device__ inline bool grid_sync(const GridBar& gb, uint32_t n = 0) {
 }
 
 // True (in every thread of the workgroup) when an earlier launch on this
-// barrier block was abandoned and the host has not cleared it yet: the
-// launch must end at once.
+// barrier block was abandoned and the host has not cleared it yet, or (read
+// by workgroup 0) the host's abort word is raised: the launch must end at
+// once.  Workgroups already past this point see the device word in their
+// next barrier wait.
 __device__ inline bool grid_aborted(const GridBar& gb) {
-    return __syncthreads_or(threadIdx.x == 0 &&
-                            __hip_atomic_load(&gb.bar[BAR_ABORT], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT) != 0u);
+    bool stop = false;
+    if (threadIdx.x == 0) {
+        stop = __hip_atomic_load(&gb.bar[BAR_ABORT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+        if (!stop && blockIdx.x == 0 && gb.host_abort &&
+            __hip_atomic_load(gb.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+            __hip_atomic_store(&gb.bar[BAR_ABORT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicCAS(gb.status, 0u, (uint32_t)(-CAGEO_ERR_ABORTED));
+            stop = true;
+        }
+    }
+    return __syncthreads_or(stop);
 }
 
 // Words of the control block (d_ctrl), shared by the persistent kernels.

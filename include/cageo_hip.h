@@ -200,9 +200,11 @@ int cageo_realization_from_noise(cageo_context* ctx, float* d_noise, float* d_mc
  * can take each other's places on the device; run them one after another. */
 int cageo_context_set_barrier_timeout(cageo_context* ctx, double seconds);
 /* Raises (set != 0) or clears the context's abort word.  While it is raised,
- * every persistent launch of the context ends at its next barrier wait and
- * the call returns CAGEO_ERR_ABORTED.  The only call that may be made from
- * another host thread while a call on the context runs. */
+ * every persistent launch of the context ends at its start, a launch already
+ * running ends in its first barrier wait that stalls (about a thousand polls
+ * of the barrier word), and the call returns CAGEO_ERR_ABORTED.  The only
+ * call that may be made from another host thread while a call on the
+ * context runs. */
 int cageo_context_abort(cageo_context* ctx, int set);
 
 /* Per-stage device time (ms, hipEvent) of the last cageo_run_realizations
