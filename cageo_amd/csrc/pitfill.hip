@@ -80,19 +80,23 @@ __device__ inline float min8x(const float2* s, int i) {
     return fminf(fminf(a, b), fminf(c, d));
 }
 // Cells of a row reached from the `seed` lanes by steps x-1 -> x allowed
-// by `gate_up` and x+1 -> x allowed by `gate_down` (64-bit lane masks; the
-// reach doubles each step, so 6 steps cover the row).
+// by `gate_up` and x+1 -> x allowed by `gate_down` (64-bit lane masks).
+// Upward, R(x) = S(x) | (G(x) & R(x-1)) is the carry recurrence of a binary
+// addition with generate bits S and propagate bits G & ~S, i.e. of
+// (S | G) + S: R(x) is the carry into bit x+1, so R = carries >> 1 with the
+// carry out of bit 63 as R(63).  Downward is the same on bit-reversed masks.
+// A few scalar instructions instead of a doubling loop.
+__device__ inline unsigned long long spread_up(unsigned long long s, unsigned long long g) {
+    const unsigned long long x = s | g;
+    unsigned long long sum;
+    const bool out = __builtin_add_overflow(x, s, &sum);
+    return ((sum ^ x ^ s) >> 1) | ((unsigned long long)out << 63);
+}
 __device__ inline unsigned long long spread_row(unsigned long long seed, unsigned long long gate_up,
                                                 unsigned long long gate_down) {
-    unsigned long long fr = seed, fl = seed;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        fr |= (fr << d) & gate_up;
-        gate_up &= gate_up << d;
-        fl |= (fl >> d) & gate_down;
-        gate_down &= gate_down >> d;
-    }
-    return fr | fl;
+    const unsigned long long down = __builtin_bitreverse64(
+        spread_up(__builtin_bitreverse64(seed), __builtin_bitreverse64(gate_down)));
+    return spread_up(seed, gate_up) | down;
 }
 
 // Relays of the six cells in the rows above and below (the row neighbours
