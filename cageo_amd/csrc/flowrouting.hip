@@ -268,17 +268,48 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
         }
         if (!__syncthreads_or(changed)) break;
     }
-    uint32_t bits = 0;
+    uint32_t bits = 0, seen = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
         if (changed_cells & (1u << k)) {
             const int ly = strip * STRIP + k;
             a.dist[(size_t)(y0 + ly) * a.W + x0 + lx] = S.dz[(ly + 1) * HALO_W + lx + 1].x;
-            bits |= border_bits(lx, ly);
+        }
+    }
+    // A changed border cell concerns a neighbour tile only if one of its
+    // cells in this tile's halo can take the new distance: a flat (d >= 1)
+    // not lower than the cell whose distance, as loaded (distances only
+    // decrease), exceeds the new one plus one.  Only such cells activate
+    // the tile they lie in.  A loop over the few border cells (not
+    // unrolled: the row is a variable).
+    {
+        const uint32_t edge = (lx == 0 || lx == TILE - 1)
+                                  ? 0xffffu
+                                  : ((strip == 0 ? 1u : 0u) |
+                                     (strip == TILE_THREADS / 64 - 1 ? 1u << (STRIP - 1) : 0u));
+        uint32_t cand = changed_cells & edge;
+        while (cand) {
+            const int k = __builtin_ctz(cand);
+            cand &= cand - 1;
+            const int ly = strip * STRIP + k;
+            const int li = (ly + 1) * HALO_W + lx + 1;
+            const int2 cv = S.dz[li];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                const int nx = lx + FDX[d], ny = ly + FDY[d];
+                if (nx >= 0 && ny >= 0 && nx < TILE && ny < TILE) continue;
+                const int2 nv = S.dz[li + FDY[d] * HALO_W + FDX[d]];
+                if (nv.x >= 1 && nv.x > cv.x + 1 &&
+                    __int_as_float(nv.y) >= __int_as_float(cv.y)) {
+                    bits |= 1u << tile_dir(nx < 0 ? -1 : (nx >= TILE ? 1 : 0),
+                                           ny < 0 ? -1 : (ny >= TILE ? 1 : 0));
+                    seen |= 1u << k;
+                }
+            }
         }
     }
     bits = wave_or8(bits);
-    edge_columns_put(S.wcol, S.ecol, changed_cells);
+    edge_columns_put(S.wcol, S.ecol, seen);
     if ((threadIdx.x & 63) == 0 && bits) atomicOr(&S.bits, bits);
     __syncthreads();
     if (threadIdx.x < 64 && S.bits) {  // uniform within the first wave

@@ -15,6 +15,12 @@ namespace cageo {
 // Neighbour-tile bits: N, NE, E, SE, S, SW, W, NW.
 __device__ constexpr int TDX[8] = {0, 1, 1, 1, 0, -1, -1, -1};
 __device__ constexpr int TDY[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+// Index d of the neighbour tile at offset (ox, oy), TDX[d] == ox, TDY[d] == oy.
+// (A table of nibbles in a register: an indexed array would live in
+// private memory.)
+__device__ inline int tile_dir(int ox, int oy) {
+    return (int)((0x345206107ull >> (4 * ((oy + 1) * 3 + ox + 1))) & 7u);
+}
 
 // ---- exact row solves of the LDS tile sweeps -------------------------------
 // A wave owns a strip of STRIP tile rows, one lane per column.  Each step
