@@ -135,6 +135,13 @@ def lib():
         "cageo_surrogate_check": ([vp, vp, vp, i, u64, ctypes.c_double, ctypes.c_double, vp,
                                    ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)], i),
+        "cageo_trainer_create": ([ctypes.POINTER(vp), i, u64, ctypes.c_char_p, vp], i),
+        "cageo_trainer_destroy": ([vp], i),
+        "cageo_trainer_gradients": ([vp, vp, vp, i, i, i, ctypes.POINTER(f)], i),
+        "cageo_trainer_step": ([vp, vp, vp, i, i, i, f, ctypes.POINTER(f)], i),
+        "cageo_surrogate_train": ([vp, vp, vp, i, i, i, i, f, f, i, i, i, u64, vp], i),
+        "cageo_trainer_read": ([vp, ctypes.c_char_p, i, vp, ctypes.c_longlong], i),
+        "cageo_trainer_save": ([vp, ctypes.c_char_p], i),
         "cageo_check_windows": ([vp, i, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "cageo_check_probability_map": ([vp, vp, i, u64, i, ctypes.c_double, ctypes.c_double,
                                          vp, vp, vp, vp], i),
@@ -334,6 +341,76 @@ class Surrogate:
         return bool(acc.value), b.value, r.value
 
 
+class Trainer:
+    """Training of the border-probability surrogate behind the C ABI
+    (cageo_trainer, cageo_surrogate_train): fp32 master weights, Adam, the
+    forward and backward passes on the library's BF16 MFMA kernels.
+    Inputs are fp32 device tensors [tiles][h][w][4], targets [tiles][h][w]."""
+
+    def __init__(self, base=32, seed=0, init_weights=None, stream=None):
+        h = ctypes.c_void_p()
+        sp = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream")
+                                          else stream)
+        path = None if init_weights is None else os.fsencode(init_weights)
+        _check(lib().cageo_trainer_create(ctypes.byref(h), base, seed, path, sp), "trainer_create")
+        self._h = h
+        self.base = base
+        _live.add(self)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cageo_trainer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def gradients(self, x4, y):
+        """Loss and gradients of the batch (x4 [n][h][w][4], y [n][h][w])."""
+        n, h, w = y.shape
+        loss = ctypes.c_float()
+        _check(lib().cageo_trainer_gradients(self._h, _ptr(x4), _ptr(y), n, h, w,
+                                             ctypes.byref(loss)), "trainer_gradients")
+        return loss.value
+
+    def step(self, x4, y, lr):
+        n, h, w = y.shape
+        loss = ctypes.c_float()
+        _check(lib().cageo_trainer_step(self._h, _ptr(x4), _ptr(y), n, h, w, lr,
+                                        ctypes.byref(loss)), "trainer_step")
+        return loss.value
+
+    def train(self, x4, y, batch, lr, lr_floor, first_step, nsteps, total_steps, seed):
+        """Steps first_step .. first_step + nsteps - 1 of a total_steps run
+        (cageo_surrogate_train); returns their losses (numpy float32)."""
+        import numpy as np
+        ntiles, h, w = y.shape
+        hist = np.zeros(max(nsteps, 1), np.float32)
+        _check(lib().cageo_surrogate_train(self._h, _ptr(x4), _ptr(y), ntiles, h, w, batch, lr,
+                                           lr_floor, first_step, nsteps, total_steps, seed,
+                                           hist.ctypes.data), "surrogate_train")
+        return hist[:nsteps]
+
+    def read(self, name, shape, grad=False):
+        import numpy as np
+        out = np.zeros(shape, np.float32)
+        _check(lib().cageo_trainer_read(self._h, name.encode(), 1 if grad else 0,
+                                        out.ctypes.data, out.size), "trainer_read")
+        return out
+
+    def save(self, path):
+        _check(lib().cageo_trainer_save(self._h, os.fsencode(path)), "trainer_save")
+
+
 def pit_fill_noalloc(filled, dem, nodata_dem, stream=None) -> None:
     """cageo_pit_fill_noalloc: the pit fill with no context and no device
     allocation (workspace in the library's own device array)."""
@@ -499,8 +576,9 @@ class Context:
                "refine_probability_map")
 
     def abort(self, set_=True):
-        """Raise (or clear) the abort word: persistent launches end at their
-        next barrier wait with CAGEO_ERR_ABORTED while it is raised."""
+        """Raise (or clear) the abort word: while it is raised, persistent
+        launches end at their start, or in a barrier wait that stalls, with
+        CAGEO_ERR_ABORTED."""
         _check(lib().cageo_context_abort(self._h, int(bool(set_))), "context_abort")
 
     def set_barrier_timeout(self, seconds):

@@ -291,6 +291,32 @@ __global__ void k_relu_back_bf16(const float* __restrict__ dy, const uint16_t* _
     }
 }
 
+// ReLU backward and bias gradient in one pass: g = bf16(dy * (act > 0)) and
+// db[c] += sum_p g[p][c] (of the rounded g).  C divides the workgroup size
+// and the grid stride is a multiple of it, so a thread only ever sees
+// channel threadIdx.x % C; a workgroup adds its threads' sums in LDS and
+// makes one atomic per channel.
+__global__ __launch_bounds__(256) void k_relu_back_bias(const float* __restrict__ dy,
+                                                        const uint16_t* __restrict__ act,
+                                                        uint16_t* __restrict__ g, long long n,
+                                                        int C, float* __restrict__ db) {
+    __shared__ float part[256];
+    float s = 0.0f;
+    GRID_STRIDE(i, n) {
+        const float v = (act[i] & 0x7fffu) && !(act[i] & 0x8000u) ? dy[i] : 0.0f;
+        const uint16_t b = f2bf(v);
+        g[i] = b;
+        s += bf2f(b);
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < C) {
+        float t = 0.0f;
+        for (int k = threadIdx.x; k < (int)blockDim.x; k += C) t += part[k];
+        atomicAdd(&db[threadIdx.x], t);
+    }
+}
+
 // Bias gradient: db[c] += sum_p g[p][c] (g bf16).
 __global__ void k_bias_grad(const uint16_t* __restrict__ g, float* __restrict__ db, long long P,
                             int C) {
@@ -330,11 +356,27 @@ __global__ void k_head(const uint16_t* __restrict__ act, const float* __restrict
 
 // Loss and head gradients: q = sigmoid(logit); binary cross-entropy against
 // the target frequency t; dlogit = (q - t) / P.  Writes q, the head weight and
-// bias gradients (atomics) and d(act) in fp32.  loss[0] += sum BCE / P.
-__global__ void k_head_back(const uint16_t* __restrict__ act, const float* __restrict__ w,
-                            const float* __restrict__ logit, const float* __restrict__ target,
-                            float* __restrict__ q, float* __restrict__ dact, float* __restrict__ dw,
-                            float* __restrict__ db, float* __restrict__ loss, long long P, int C) {
+// bias gradients and d(act) in fp32.  loss[0] += sum BCE / P.  C is a
+// multiple of 32 up to HEAD_MAX_C: each thread sums its pixels' terms of
+// the weight gradient in registers, a workgroup adds them up in LDS and
+// makes one atomic per channel.
+constexpr int HEAD_MAX_C = 128;
+template <int C>
+__global__ __launch_bounds__(256) void k_head_back(const uint16_t* __restrict__ act,
+                                                   const float* __restrict__ w,
+                                                   const float* __restrict__ logit,
+                                                   const float* __restrict__ target,
+                                                   float* __restrict__ q, float* __restrict__ dact,
+                                                   float* __restrict__ dw, float* __restrict__ db,
+                                                   float* __restrict__ loss, long long P) {
+    __shared__ float part[C];
+    __shared__ float red[2];
+    for (int c = threadIdx.x; c < C; c += blockDim.x) part[c] = 0.0f;
+    if (threadIdx.x < 2) red[threadIdx.x] = 0.0f;
+    __syncthreads();
+    float acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = 0.0f;
     float lsum = 0.0f, bsum = 0.0f;
     GRID_STRIDE(p, P) {
         const float z = logit[p];
@@ -345,18 +387,42 @@ __global__ void k_head_back(const uint16_t* __restrict__ act, const float* __res
         lsum += fmaxf(z, 0.0f) - z * t + log1pf(expf(-fabsf(z)));
         const float g = (qq - t) / (float)P;
         bsum += g;
-        for (int c = 0; c < C; ++c) {
-            dact[p * C + c] = g * w[c];
-            atomicAdd(&dw[c], g * bf2f(act[p * C + c]));
+        const uint4* a = reinterpret_cast<const uint4*>(act + p * C);
+        float4* d = reinterpret_cast<float4*>(dact + p * C);
+#pragma unroll
+        for (int c8 = 0; c8 < C / 8; ++c8) {
+            const uint4 v = a[c8];
+            const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = 8 * c8 + 2 * k;
+                acc[c] += g * bf2f((uint16_t)(u[k] & 0xffffu));
+                acc[c + 1] += g * bf2f((uint16_t)(u[k] >> 16));
+            }
+            d[2 * c8] = make_float4(g * w[8 * c8], g * w[8 * c8 + 1], g * w[8 * c8 + 2], g * w[8 * c8 + 3]);
+            d[2 * c8 + 1] = make_float4(g * w[8 * c8 + 4], g * w[8 * c8 + 5], g * w[8 * c8 + 6],
+                                        g * w[8 * c8 + 7]);
         }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        float v = acc[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&part[c], v);
     }
     for (int off = 32; off > 0; off >>= 1) {
         lsum += __shfl_xor(lsum, off);
         bsum += __shfl_xor(bsum, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(loss, lsum / (float)P);
-        atomicAdd(db, bsum);
+        atomicAdd(&red[0], lsum);
+        atomicAdd(&red[1], bsum);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dw[c], part[c]);
+    if (threadIdx.x == 0) {
+        atomicAdd(loss, red[0] / (float)P);
+        atomicAdd(db, red[1]);
     }
 }
 
@@ -522,9 +588,34 @@ int cageo_nn_head(void* stream, const uint16_t* act, const float* w, const float
 int cageo_nn_head_back(void* stream, const uint16_t* act, const float* w, const float* logit,
                        const float* target, float* q, float* dact, float* dw, float* db,
                        float* loss, long long p, int c) {
-    hipLaunchKernelGGL(k_head_back, dim3(grid_for(p)), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), act, w, logit, target, q, dact, dw,
-                       db, loss, p, c);
+    if (!act || !w || !logit || !q || !dact || !dw || !db || !loss || p <= 0) return CAGEO_ERR_ARG;
+    const int grid = (int)std::min<long long>(1024, (p + 255) / 256);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (c == 32)
+        hipLaunchKernelGGL(k_head_back<32>, dim3(grid), dim3(256), 0, s, act, w, logit, target, q,
+                           dact, dw, db, loss, p);
+    else if (c == 64)
+        hipLaunchKernelGGL(k_head_back<64>, dim3(grid), dim3(256), 0, s, act, w, logit, target, q,
+                           dact, dw, db, loss, p);
+    else if (c == 96)
+        hipLaunchKernelGGL(k_head_back<96>, dim3(grid), dim3(256), 0, s, act, w, logit, target, q,
+                           dact, dw, db, loss, p);
+    else if (c == HEAD_MAX_C)
+        hipLaunchKernelGGL(k_head_back<HEAD_MAX_C>, dim3(grid), dim3(256), 0, s, act, w, logit,
+                           target, q, dact, dw, db, loss, p);
+    else
+        return CAGEO_ERR_ARG;
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
+int cageo_nn_relu_back_bias(void* stream, const float* dy, const uint16_t* act, uint16_t* g,
+                            long long p, int c, float* db) {
+    if (!dy || !act || !g || !db || p <= 0 || c <= 0 || 256 % c) return CAGEO_ERR_ARG;
+    const long long n = p * c;
+    const long long blocks = std::min<long long>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_relu_back_bias, dim3((unsigned)blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), dy, act, g, n, c, db);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }

@@ -34,8 +34,9 @@ from . import lib, _check
 IN_CHANNELS = 4
 IN_PAD = 32  # the first convolution reads 32 channels (4 used)
 
-# Weights trained by `python -m cageo_amd.train_surrogate` (64 synthetic DEMs
-# of 2048^2, 200-realization Monte Carlo labels, 4000 Adam steps).
+# Weights trained by `python -m cageo_amd.train_surrogate` (320 synthetic DEMs
+# of 4096^2 cut into 20480 tiles of 512^2, 200-realization Monte Carlo
+# labels, Gaussian filter, 10000 Adam steps of 8 tiles).
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(__file__), "weights", "surrogate_b32.bin")
 
 
@@ -366,66 +367,81 @@ def mc_frequency(torch, dems, streams, rmses, ranges, realizations, seed, nodata
 
 def training_set(torch, n_dems=16, size=2048, crop=512, realizations=200, seed=1000,
                  acc_threshold=2000, nodata=-9999.0, model=None, rmse_range=(0.5, 3.0),
-                 pr_range=(30.0, 200.0), log=None, group=16):
-    """Inputs [N][crop][crop][4] and labels [N][crop][crop] (float32 numpy):
-    synthetic DEMs of the evaluation generator, each with its own RMSE ~ U(0.5,
-    3) m and practical range ~ U(30, 200) m, labelled by `realizations`
-    Monte Carlo realizations on the whole DEM, then cut into crops."""
-    from . import synth_dem
+                 pr_range=(30.0, 200.0), log=None, log_every=8):
+    """Inputs [N][crop][crop][4] and labels [N][crop][crop] as float32 device
+    tensors, N = n_dems * (size // crop)^2: synthetic DEMs of the evaluation
+    generator (seeds seed + k), each with its own RMSE ~ U(rmse_range) m and
+    practical range ~ U(pr_range) m, labelled by `realizations` Monte Carlo
+    realizations of the whole DEM on one context (Gaussian filter unless
+    `model`), input planes built on the device (the DEM z-scored over the
+    whole raster, as cageo_surrogate_predict does), then cut into crops.
+    The host generates the next DEM while the device labels the current one."""
+    from concurrent.futures import ThreadPoolExecutor
+    from . import Context, GAUSSIAN, create_filter, filter_radius, synth_dem
+    model = GAUSSIAN if model is None else model
     rng = np.random.default_rng(seed)
-    dems, streams, rmses, ranges = [], [], [], []
-    for k in range(n_dems):
-        d, s = synth_dem(size, size, seed + k, 10.0, acc_threshold, nodata)
-        dems.append(d)
-        streams.append(s)
-        rmses.append(float(rng.uniform(*rmse_range)))
-        ranges.append(float(rng.uniform(*pr_range)))
-    labels = []
-    for g in range(0, n_dems, group):  # groups of DEMs, one progress line each
-        labels += mc_frequency(torch, dems[g:g + group], streams[g:g + group], rmses[g:g + group],
-                               ranges[g:g + group], realizations, seed + g, nodata, model=model)
-        if log:
-            log(f"labels: {min(g + group, n_dems)}/{n_dems} DEMs")
-    xs, ys = [], []
-    for d, s, e, pr, lab in zip(dems, streams, rmses, ranges, labels):
-        x = make_inputs(d, s != nodata, e, pr, nodata, channels=IN_CHANNELS)
-        for i in range(0, size - crop + 1, crop):
-            for j in range(0, size - crop + 1, crop):
-                xs.append(x[i:i + crop, j:j + crop])
-                ys.append(lab[i:i + crop, j:j + crop])
-    return np.stack(xs), np.stack(ys), dict(rmse=rmses, practical_range=ranges)
-
-
-def train(torch, x4, y, steps=1500, batch=8, lr=2e-3, seed=0, base=32, log_every=100,
-          log=print):
-    """Adam on the BCE of the surrogate against Monte Carlo frequencies, with
-    random flips and quarter turns; inputs padded to IN_PAD channels on the
-    device.  Returns the network and its loss history."""
+    rmses = [float(rng.uniform(*rmse_range)) for _ in range(n_dems)]
+    ranges = [float(rng.uniform(*pr_range)) for _ in range(n_dems)]
+    g = size // crop
+    per = g * g
     T = torch
-    rng = np.random.default_rng(seed)
-    net = UNet(torch, base=base, seed=seed)
-    X = T.from_numpy(np.ascontiguousarray(x4)).cuda()
-    Y = T.from_numpy(np.ascontiguousarray(y)).cuda()
-    n = X.shape[0]
+    X = T.empty((n_dems * per, crop, crop, IN_CHANNELS), dtype=T.float32, device="cuda")
+    Y = T.empty((n_dems * per, crop, crop), dtype=T.float32, device="cuda")
+    mc = T.zeros((size, size), dtype=T.float32, device="cuda")
+    gen = lambda k: synth_dem(size, size, seed + k, 10.0, acc_threshold, nodata)
+    with ThreadPoolExecutor(1) as pool, Context(size, size, T.cuda.current_stream()) as ctx:
+        nxt = pool.submit(gen, 0)
+        for k in range(n_dems):
+            d, st = nxt.result()
+            if k + 1 < n_dems:
+                nxt = pool.submit(gen, k + 1)
+            dd, ss = T.from_numpy(d).cuda(), T.from_numpy(st).cuda()
+            r = filter_radius(ranges[k], 10.0)
+            ctx.set_filter(model, r, create_filter(model, ranges[k], 10.0, r))
+            ctx.mc_setup(dd, ss, nodata, nodata, 2, 0.0, rmses[k])
+            mc.zero_()
+            ctx.run_realizations(mc, realizations, seed + k, 0)
+            valid = dd != nodata
+            v = dd[valid].double()
+            mu = float(v.mean()) if v.numel() else 0.0
+            sd = max(float(v.std(unbiased=False)) if v.numel() else 1.0, 1e-6)
+            planes = T.empty((size, size, IN_CHANNELS), dtype=T.float32, device="cuda")
+            planes[..., 0] = T.where(valid, (dd - mu) / sd, T.zeros_like(dd))
+            planes[..., 1] = (ss != nodata).float()
+            planes[..., 2] = rmses[k] / 3.0
+            planes[..., 3] = ranges[k] / 200.0
+            X[k * per:(k + 1) * per] = (planes.view(g, crop, g, crop, IN_CHANNELS)
+                                        .permute(0, 2, 1, 3, 4).reshape(per, crop, crop, IN_CHANNELS))
+            Y[k * per:(k + 1) * per] = ((mc / float(realizations)).view(g, crop, g, crop)
+                                        .permute(0, 2, 1, 3).reshape(per, crop, crop))
+            if log and ((k + 1) % log_every == 0 or k + 1 == n_dems):
+                log(f"labels: {k + 1}/{n_dems} DEMs ({(k + 1) * per} tiles)")
+    return X, Y, dict(rmse=rmses, practical_range=ranges)
+
+
+def train(torch, x4, y, steps=1500, batch=8, lr=2e-3, seed=0, base=32, log_every=500,
+          log=print, lr_floor=0.02):
+    """Adam on the BCE of the surrogate against Monte Carlo frequencies
+    through the C ABI (cageo_surrogate_train): batches of `batch` tiles with
+    random dihedral transforms, linear warm-up then cosine decay from lr.
+    x4 [N][h][w][4] and y [N][h][w] (device tensors or numpy).  Runs in
+    pieces of `log_every` steps with a progress line each.  Returns the
+    Trainer and the loss history."""
+    from . import Trainer
+    T = torch
+    if isinstance(x4, np.ndarray):
+        x4 = T.from_numpy(np.ascontiguousarray(x4, np.float32)).cuda()
+    if isinstance(y, np.ndarray):
+        y = T.from_numpy(np.ascontiguousarray(y, np.float32)).cuda()
+    tr = Trainer(base, seed=seed, stream=T.cuda.current_stream())
     hist = []
-    for step in range(1, steps + 1):
-        idx = T.from_numpy(rng.choice(n, size=batch, replace=n < batch)).cuda()
-        xb, yb = X[idx], Y[idx]
-        k = int(rng.integers(4))
-        xb, yb = T.rot90(xb, k, dims=(1, 2)), T.rot90(yb, k, dims=(1, 2))
-        if rng.random() < 0.5:
-            xb, yb = T.flip(xb, dims=(2,)), T.flip(yb, dims=(2,))
-        xp = T.zeros(xb.shape[:3] + (IN_PAD,), dtype=T.float32, device="cuda")
-        xp[..., :IN_CHANNELS] = xb
-        cur = lr * 0.5 * (1.0 + math.cos(math.pi * (step - 1) / steps))  # cosine decay
-        net.zero_grad()
-        net.forward(xp.contiguous())
-        loss = net.backward(yb.contiguous())
-        net.adam_step(lr=max(cur, lr * 0.02))
-        hist.append(loss)
-        if log and (step % log_every == 0 or step == 1):
-            log(f"step {step} loss {np.mean(hist[-log_every:]):.5f} lr {cur:.2e}")
-    return net, hist
+    for first in range(0, steps, log_every):
+        n = min(log_every, steps - first)
+        h = tr.train(x4.contiguous(), y.contiguous(), batch, lr, lr_floor, first, n, steps, seed)
+        hist.extend(float(v) for v in h)
+        if log:
+            log(f"step {first + n} loss {np.mean(h[-min(n, 100):]):.5f}")
+    return tr, hist
 
 
 WEIGHTS_MAGIC = b"CGSURR01"

@@ -1,17 +1,18 @@
 """Train the border-probability surrogate on Monte Carlo maps and judge it.
 
-    python -m cageo_amd.train_surrogate --out weights.npz [--report report.json]
+    python -m cageo_amd.train_surrogate --out weights.bin [--report report.json]
     python -m cageo_amd.train_surrogate --load default [--report report.json]
 
 With --load only step 3 runs, on the given (or the packaged) weights.
 
 1. Training set: `--n-dems` synthetic DEMs of `--size`^2 (the evaluation
    generator, other seeds), each with RMSE ~ U(0.5, 3) m and practical range
-   ~ U(30, 200) m, labelled by `--realizations` Monte Carlo realizations of
-   the whole DEM (batched, one graph replay per realization), cut into
-   `--crop`^2 tiles.
-2. Training: `--steps` Adam steps of `--batch` tiles on the library's BF16
-   MFMA kernels.
+   ~ U(30, `--pr-max`) m, labelled by `--realizations` Monte Carlo
+   realizations of the whole DEM, cut into `--crop`^2 tiles; inputs and
+   labels stay on the device (the defaults: 320 DEMs of 4096^2, 20480 tiles
+   of 512^2).
+2. Training: `--steps` Adam steps of `--batch` tiles through the C ABI
+   (cageo_surrogate_train) on the library's BF16 MFMA kernels.
 3. Judgement: on the `--eval-size`^2 DEM (seed `--eval-seed`, RMSE
    `--eval-rmse`, practical range `--eval-range`) the prediction is compared
    with a `--eval-realizations` Monte Carlo: mean absolute and maximum
@@ -32,12 +33,12 @@ import numpy as np
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=None, help="weights file (.npz) to write")
+    ap.add_argument("--out", default=None, help="weights file (.bin) to write")
     ap.add_argument("--load", default=None,
                     help="judge these weights instead of training ('default': the packaged ones)")
     ap.add_argument("--report", default=None, help="also write the JSON report here")
-    ap.add_argument("--n-dems", type=int, default=16)
-    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--n-dems", type=int, default=320)
+    ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--crop", type=int, default=512)
     ap.add_argument("--realizations", type=int, default=200)
     ap.add_argument("--steps", type=int, default=1500)
@@ -67,7 +68,7 @@ def main(argv=None):
 
     if a.load:
         net = sg.load_net(torch, None if a.load == "default" else a.load)
-        hist, train_s = [], 0.0
+        hist, train_s, label_s = [], 0.0, 0.0
         log(f"weights <- {a.load}")
     else:
         if not a.out:
@@ -75,16 +76,21 @@ def main(argv=None):
         t0 = time.perf_counter()
         x4, y, meta = sg.training_set(torch, a.n_dems, a.size, a.crop, a.realizations, a.seed,
                                       model=model, pr_range=(30.0, a.pr_max), log=log)
-        log(f"training set: {x4.shape[0]} tiles of {a.crop}^2 from {a.n_dems} DEMs of "
-            f"{a.size}^2, {a.realizations} realizations each ({time.perf_counter() - t0:.1f} s)")
+        torch.cuda.synchronize()
+        label_s = time.perf_counter() - t0
+        ntiles = int(x4.shape[0])
+        log(f"training set: {ntiles} tiles of {a.crop}^2 from {a.n_dems} DEMs of "
+            f"{a.size}^2, {a.realizations} realizations each ({label_s:.1f} s)")
         t1 = time.perf_counter()
-        net, hist = sg.train(torch, x4, y, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
-                             log=log)
+        tr, hist = sg.train(torch, x4, y, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                            log=log)
         torch.cuda.synchronize()
         train_s = time.perf_counter() - t1
-        sg.save_params(net, a.out)
+        tr.save(a.out)
+        tr.close()
         log(f"trained {a.steps} steps in {train_s:.1f} s, weights -> {a.out}")
         del x4, y
+        net = sg.load_net(torch, a.out)
 
     dem, st = ca.synth_dem(a.eval_size, a.eval_size, a.eval_seed, 10.0, 2000, -9999.0)
     torch.cuda.synchronize()
@@ -105,6 +111,9 @@ def main(argv=None):
     report = {
         "weights": a.load or a.out,
         "train_tiles": int(len(hist) and a.n_dems * (a.size // a.crop) ** 2),
+        "train_dems": f"{a.n_dems} x {a.size}^2" if hist else None,
+        "label_s": round(label_s, 1) if hist else None,
+        "train_batch": a.batch if hist else None,
         "train_realizations": a.realizations, "train_steps": len(hist), "train_s": round(train_s, 2),
         "final_loss": float(np.mean(hist[-100:])) if hist else None,
         "eval_dem": f"{a.eval_size}x{a.eval_size} seed {a.eval_seed}",

@@ -319,6 +319,10 @@ int cageo_nn_relu_back(void* stream, const float* dy, const uint16_t* act, uint1
                        long long n);
 /* db[c] += sum_p g[p][c] */
 int cageo_nn_bias_grad(void* stream, const uint16_t* g, float* db, long long p, int c);
+/* Both in one pass: g = bf16(dy * (act > 0)) over p pixels of c channels
+ * (c divides 256), db[c] += sum_p g[p][c]. */
+int cageo_nn_relu_back_bias(void* stream, const float* dy, const uint16_t* act, uint16_t* g,
+                            long long p, int c, float* db);
 /* logit[p] = b + sum_c w[c] act[p][c]; with sigmoid != 0 the probability
  * 1 / (1 + exp(-logit)) is written instead. */
 int cageo_nn_head(void* stream, const uint16_t* act, const float* w, const float* b, float* logit,
@@ -390,6 +394,46 @@ int cageo_refine_probability_map(cageo_context* ctx, float* d_prob, int window,
 int cageo_surrogate_check(cageo_surrogate* s, cageo_context* ctx, const float* d_prob, int k,
                           uint64_t seed, double bias_tol, double rms_tol, float* d_scratch,
                           int* accepted, double* bias, double* rms);
+
+/* ---- border-probability surrogate: training --------------------------------
+ * No reference counterpart (the reference has no learned part).  A trainer
+ * holds the network's fp32 master weights, their bf16 copies, gradients
+ * and Adam moments (beta 0.9 / 0.999, eps 1e-8) for base `base` (a multiple
+ * of 32), and the workspace of one batch shape.  Initial weights: the
+ * weights file `init_weights` (surrogate format; NULL: He-normal from
+ * `seed`, zero biases).  Inputs are fp32 NHWC [tiles][h][w][4] (z-scored
+ * DEM, stream mask 0/1, RMSE / 3 m, practical range / 200 m; h, w multiples
+ * of 8), targets the Monte Carlo frequencies [tiles][h][w].  The loss is
+ * the mean binary cross-entropy of the sigmoid output.  Calls on one
+ * trainer must not overlap. */
+typedef struct cageo_trainer cageo_trainer;
+int cageo_trainer_create(cageo_trainer** out, int base, uint64_t seed, const char* init_weights,
+                         void* hip_stream);
+int cageo_trainer_destroy(cageo_trainer* t);
+/* Loss and parameter gradients (replaced, not added) of the batch of the n
+ * tiles of d_x / d_target, without an update. */
+int cageo_trainer_gradients(cageo_trainer* t, const float* d_x, const float* d_target, int n,
+                            int h, int w, float* loss);
+/* The same, then one Adam step with learning rate lr. */
+int cageo_trainer_step(cageo_trainer* t, const float* d_x, const float* d_target, int n, int h,
+                       int w, float lr, float* loss);
+/* Steps first_step .. first_step + nsteps - 1 of a run of total_steps Adam
+ * steps over the ntiles tiles of d_x / d_y: each step's batch is `batch`
+ * tiles drawn with replacement from (seed, step number), each under a
+ * random dihedral transform (quarter turns only for square tiles); the
+ * learning rate rises linearly to lr over the first 2 % of the steps (at
+ * most 200), then follows a half cosine from lr over total_steps, not
+ * below lr * lr_floor.  A run split into pieces computes what one call
+ * would.  loss_hist (host, nsteps floats, may be NULL) gets each step's
+ * loss. */
+int cageo_surrogate_train(cageo_trainer* t, const float* d_x, const float* d_y, int ntiles, int h,
+                          int w, int batch, float lr, float lr_floor, int first_step, int nsteps,
+                          int total_steps, uint64_t seed, float* loss_hist);
+/* Host copy of parameter `name` (what 0) or of its gradient (what 1);
+ * n must be its size. */
+int cageo_trainer_read(cageo_trainer* t, const char* name, int what, float* host, long long n);
+/* Weights file (the format cageo_surrogate_create reads). */
+int cageo_trainer_save(cageo_trainer* t, const char* path);
 
 #ifdef __cplusplus
 }
