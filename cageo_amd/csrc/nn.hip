@@ -131,6 +131,117 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn ci, const uint16_t* __re
     }
 }
 
+// The same convolution with its operands staged in LDS, for W % 64 == 0 and
+// H % 4 == 0.  A workgroup computes a tile of 4 rows x 64 pixels (wave w:
+// row w, 4 accumulator tiles of 16 pixels) x 16*NT output channels.  Per
+// 32-channel slice of the input it stages the 6 x 66 input pixels the 9
+// taps read (zero outside the image, upsampling and concatenation folded
+// into the loads) and the slice's weights of all 9 taps, then runs the 9
+// taps from LDS: each staged value is read from memory once per
+// workgroup instead of once per tap and wave.  Pixel rows in LDS are
+// padded to 40 bf16 (80 bytes) so 16 lanes reading 16 consecutive pixels
+// hit 16 distinct 4-bank groups.
+constexpr int CL_TX = 64, CL_TY = 4, CL_PAD = 40;
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn ci, const uint16_t* __restrict__ w,
+                                                     const float* __restrict__ bias,
+                                                     uint16_t* __restrict__ out,
+                                                     float* __restrict__ outf, int N, int H,
+                                                     int W, int Cout, int relu) {
+    constexpr int AX = CL_TX + 2, AY = CL_TY + 2, CO = 16 * NT;
+    __shared__ __align__(16) uint16_t sa[AY * AX * CL_PAD];
+    __shared__ __align__(16) uint16_t sb[9 * CO * CL_PAD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kb = lane >> 4;
+    const int tiles_x = W / CL_TX, tiles_y = H / CL_TY;
+    const int t = blockIdx.x;
+    const int n = t / (tiles_x * tiles_y);
+    const int ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
+    const int x0 = tx * CL_TX, y0 = ty * CL_TY;
+    const int co0 = blockIdx.y * CO;
+    const int Cin = ci.C1 + ci.C2;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int q = 0; q < NT; ++q) acc[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < Cin; c += 32) {
+        // every load of the slice in flight at once, then the LDS stores:
+        // input AY x AX pixels x 4 groups of 8 channels, weights 9 taps x CO
+        // output channels x 4 groups of 8 input channels
+        constexpr int AN = AY * AX * 4, AIT = (AN + 255) / 256;
+        constexpr int BN = 9 * CO * 4, BIT = (BN + 255) / 256;
+        uint4 av[AIT], bv[BIT];
+#pragma unroll
+        for (int j = 0; j < AIT; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            av[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < AN) {
+                const int g = i & 3, px = i >> 2;
+                const int yy = px / AX, xx = px - yy * AX;
+                av[j] = load_in8(ci, n, y0 - 1 + yy, x0 - 1 + xx, H, W, c + 8 * g);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < BIT; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            bv[j] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < BN) {
+                const int g = i & 3, r = i >> 2;  // r = tap * CO + co
+                const int tap = r / CO, co = r - tap * CO;
+                bv[j] = *reinterpret_cast<const uint4*>(
+                    w + ((size_t)tap * Cout + co0 + co) * Cin + c + 8 * g);
+            }
+        }
+        __syncthreads();  // the last slice's reads are done
+#pragma unroll
+        for (int j = 0; j < AIT; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (i < AN) *reinterpret_cast<uint4*>(&sa[(i >> 2) * CL_PAD + 8 * (i & 3)]) = av[j];
+        }
+#pragma unroll
+        for (int j = 0; j < BIT; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (i < BN) *reinterpret_cast<uint4*>(&sb[(i >> 2) * CL_PAD + 8 * (i & 3)]) = bv[j];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;  // halo-shifted offsets 0..2
+            bf16x8 a[4], b[NT];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                a[m] = as_bf16x8(*reinterpret_cast<const uint4*>(
+                    &sa[((wave + dy) * AX + 16 * m + row + dx) * CL_PAD + 8 * kb]));
+#pragma unroll
+            for (int q = 0; q < NT; ++q)
+                b[q] = as_bf16x8(*reinterpret_cast<const uint4*>(
+                    &sb[(tap * CO + 16 * q + row) * CL_PAD + 8 * kb]));
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int q = 0; q < NT; ++q)
+                    acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[q], acc[m][q], 0, 0, 0);
+        }
+    }
+    const long long prow = ((long long)n * H + y0 + wave) * W + x0;
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int co = co0 + 16 * q + row;  // accumulator column = lane % 16
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long pp = prow + 16 * m + 4 * kb + r;  // accumulator row
+                float v = acc[m][q][r] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                if (out) out[pp * Cout + co] = f2bf(v);
+                if (outf) outf[pp * Cout + co] = v;
+            }
+    }
+}
+
 // Weight gradient of the 3x3 convolution:
 //   dW[tap][co][ci] += sum_p g[p][co] * x[p shifted by tap][ci]
 // (GEMM rows co, columns ci, K = pixels).  A workgroup takes one tap, a
@@ -481,6 +592,21 @@ int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const u
     if (c1 <= 0 || c1 % 32 || c2 < 0 || c2 % 32 || (c2 && !in2) || cout % 16) return CAGEO_ERR_ARG;
     ConvIn ci{in1, in2, c1, c2, up1};
     const long long M = (long long)n * h * wdt;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (wdt % CL_TX == 0 && h % CL_TY == 0 && cout % 32 == 0) {
+        // operands staged in LDS: workgroups of 4 x 64 pixels x 64 (or 32)
+        // output channels
+        const dim3 g((unsigned)((long long)n * (h / CL_TY) * (wdt / CL_TX)),
+                     (unsigned)(cout % 64 == 0 ? cout / 64 : cout / 32));
+        if (cout % 64 == 0)
+            hipLaunchKernelGGL(k_conv3x3_lds<4>, g, dim3(256), 0, st, ci, w, bias, out, outf, n,
+                               h, wdt, cout, relu);
+        else
+            hipLaunchKernelGGL(k_conv3x3_lds<2>, g, dim3(256), 0, st, ci, w, bias, out, outf, n,
+                               h, wdt, cout, relu);
+        CAGEO_HIP_TRY(hipGetLastError());
+        return CAGEO_OK;
+    }
     // output channel tile: 64 (NT 4), 32 (NT 2) or 16 (NT 1); 4 pixel tiles
     // per wave (256 pixels per workgroup; 8 lowered occupancy and time)
     constexpr int MT = 4;

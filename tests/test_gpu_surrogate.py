@@ -50,6 +50,10 @@ CONV_CASES = [  # n, h, w, c1, up1, c2, cout, relu
     (1, 9, 7, 64, 1, 32, 32, 0),
     (1, 33, 17, 32, 0, 0, 16, 1),
     (3, 8, 8, 96, 0, 0, 128, 1),
+    # widths of 64 and heights of 4 take the LDS-staged kernel
+    (2, 8, 128, 64, 1, 32, 64, 1),
+    (1, 12, 64, 32, 0, 0, 32, 0),
+    (1, 4, 192, 128, 0, 0, 96, 1),
 ]
 
 
