@@ -132,67 +132,85 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn ci, const uint16_t* __re
 }
 
 // The same convolution with its operands staged in LDS, for W % 64 == 0 and
-// H % 4 == 0.  A workgroup computes a tile of 4 rows x 64 pixels (wave w:
-// row w, 4 accumulator tiles of 16 pixels) x 16*NT output channels.  Per
-// 32-channel slice of the input it stages the 6 x 66 input pixels the 9
-// taps read (zero outside the image, upsampling and concatenation folded
+// H % (4 RW) == 0.  A workgroup computes a tile of 4 RW rows x 64 pixels
+// (wave w: rows w RW .. w RW + RW - 1, 4 accumulator tiles of 16 pixels
+// each) x 16*NT output channels; with RW = 2 each weight fragment read
+// feeds twice the MFMAs (the 32-channel layers).  Per 32-channel slice of
+// the input it stages the (4 RW + 2) x 66 input pixels the 9 taps read (zero outside the image, upsampling and concatenation folded
 // into the loads) and the slice's weights of all 9 taps, then runs the 9
 // taps from LDS: each staged value is read from memory once per
 // workgroup instead of once per tap and wave.  Pixel rows in LDS are
 // padded to 40 bf16 (80 bytes) so 16 lanes reading 16 consecutive pixels
 // hit 16 distinct 4-bank groups.
-constexpr int CL_TX = 64, CL_TY = 4, CL_PAD = 40;
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn ci, const uint16_t* __restrict__ w,
+constexpr int CL_TX = 64, CL_PAD = 40;
+template <int NT, int RW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv3x3_lds(ConvIn ci, const uint16_t* __restrict__ w,
                                                      const float* __restrict__ bias,
                                                      uint16_t* __restrict__ out,
                                                      float* __restrict__ outf, int N, int H,
                                                      int W, int Cout, int relu) {
-    constexpr int AX = CL_TX + 2, AY = CL_TY + 2, CO = 16 * NT;
+    constexpr int TY = 4 * RW, AX = CL_TX + 2, AY = TY + 2, CO = 16 * NT;
+    constexpr int TAP_UNROLL = RW == 1 ? 9 : 1;  // two rows a wave: registers run short
     __shared__ __align__(16) uint16_t sa[AY * AX * CL_PAD];
     __shared__ __align__(16) uint16_t sb[9 * CO * CL_PAD];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, kb = lane >> 4;
-    const int tiles_x = W / CL_TX, tiles_y = H / CL_TY;
+    const int tiles_x = W / CL_TX, tiles_y = H / TY;
     const int t = blockIdx.x;
     const int n = t / (tiles_x * tiles_y);
     const int ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
-    const int x0 = tx * CL_TX, y0 = ty * CL_TY;
+    const int x0 = tx * CL_TX, y0 = ty * TY;
     const int co0 = blockIdx.y * CO;
     const int Cin = ci.C1 + ci.C2;
-    f32x4 acc[4][NT];
+    f32x4 acc[4 * RW][NT];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4 * RW; ++m)
 #pragma unroll
         for (int q = 0; q < NT; ++q) acc[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int c = 0; c < Cin; c += 32) {
-        // every load of the slice in flight at once, then the LDS stores:
-        // input AY x AX pixels x 4 groups of 8 channels, weights 9 taps x CO
-        // output channels x 4 groups of 8 input channels
-        constexpr int AN = AY * AX * 4, AIT = (AN + 255) / 256;
-        constexpr int BN = 9 * CO * 4, BIT = (BN + 255) / 256;
-        uint4 av[AIT], bv[BIT];
+    // Slice c's loads are issued before slice c - 32's MFMAs and stored to
+    // LDS after them, so their latency hides behind the arithmetic.  Input
+    // AY x AX pixels x 4 groups of 8 channels, weights 9 taps x CO output
+    // channels x 4 groups of 8 input channels.  What does not depend on the
+    // slice is worked out once: each staged input value's pixel in in1 (at
+    // half resolution with up1) and in in2, -1 outside the image, and each
+    // weight value's offset.  A slice lies in one of the two inputs (their
+    // channel counts are multiples of 32).
+    constexpr int AN = AY * AX * 4, AIT = (AN + 255) / 256;
+    constexpr int BN = 9 * CO * 4, BIT = (BN + 255) / 256;
+    const int H1 = ci.up1 ? (H + 1) / 2 : H, W1 = ci.up1 ? (W + 1) / 2 : W;
+    int pix1[AIT], pix2[AIT], boff[BIT];
+#pragma unroll
+    for (int j = 0; j < AIT; ++j) {
+        const int i = threadIdx.x + 256 * j;
+        const int px = i >> 2, yy = y0 - 1 + px / AX, xx = x0 - 1 + px % AX;
+        const bool in = i < AN && yy >= 0 && xx >= 0 && yy < H && xx < W;
+        pix1[j] = in ? (n * H1 + (ci.up1 ? yy / 2 : yy)) * W1 + (ci.up1 ? xx / 2 : xx) : -1;
+        pix2[j] = in ? (n * H + yy) * W + xx : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < BIT; ++j) {
+        const int i = threadIdx.x + 256 * j, r = i >> 2;  // r = tap * CO + co
+        const int tap = r / CO, co = r - tap * CO;
+        boff[j] = i < BN ? (tap * Cout + co0 + co) * Cin + 8 * (i & 3) : -1;
+    }
+    uint4 av[AIT], bv[BIT];
+    auto load_slice = [&](int c) {
+        const bool first = c < ci.C1;
+        const uint16_t* src = first ? ci.in1 : ci.in2;
+        const int cs = first ? ci.C1 : ci.C2, c0 = (first ? c : c - ci.C1) + 8 * (threadIdx.x & 3);
 #pragma unroll
         for (int j = 0; j < AIT; ++j) {
-            const int i = threadIdx.x + 256 * j;
-            av[j] = make_uint4(0u, 0u, 0u, 0u);
-            if (i < AN) {
-                const int g = i & 3, px = i >> 2;
-                const int yy = px / AX, xx = px - yy * AX;
-                av[j] = load_in8(ci, n, y0 - 1 + yy, x0 - 1 + xx, H, W, c + 8 * g);
-            }
+            const int px = first ? pix1[j] : pix2[j];
+            av[j] = px >= 0 ? *reinterpret_cast<const uint4*>(src + (size_t)px * cs + c0)
+                            : make_uint4(0u, 0u, 0u, 0u);
         }
 #pragma unroll
-        for (int j = 0; j < BIT; ++j) {
-            const int i = threadIdx.x + 256 * j;
-            bv[j] = make_uint4(0u, 0u, 0u, 0u);
-            if (i < BN) {
-                const int g = i & 3, r = i >> 2;  // r = tap * CO + co
-                const int tap = r / CO, co = r - tap * CO;
-                bv[j] = *reinterpret_cast<const uint4*>(
-                    w + ((size_t)tap * Cout + co0 + co) * Cin + c + 8 * g);
-            }
-        }
+        for (int j = 0; j < BIT; ++j)
+            bv[j] = boff[j] >= 0 ? *reinterpret_cast<const uint4*>(w + boff[j] + c)
+                                 : make_uint4(0u, 0u, 0u, 0u);
+    };
+    load_slice(0);
+    for (int c = 0; c < Cin; c += 32) {
         __syncthreads();  // the last slice's reads are done
 #pragma unroll
         for (int j = 0; j < AIT; ++j) {
@@ -205,36 +223,38 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn ci, const uint16_t* 
             if (i < BN) *reinterpret_cast<uint4*>(&sb[(i >> 2) * CL_PAD + 8 * (i & 3)]) = bv[j];
         }
         __syncthreads();
-#pragma unroll 1
+        if (c + 32 < Cin) load_slice(c + 32);
+#pragma unroll TAP_UNROLL
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap % 3;  // halo-shifted offsets 0..2
-            bf16x8 a[4], b[NT];
+            bf16x8 a[4 * RW], b[NT];
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
+            for (int m = 0; m < 4 * RW; ++m)
                 a[m] = as_bf16x8(*reinterpret_cast<const uint4*>(
-                    &sa[((wave + dy) * AX + 16 * m + row + dx) * CL_PAD + 8 * kb]));
+                    &sa[((wave * RW + m / 4 + dy) * AX + 16 * (m % 4) + row + dx) * CL_PAD +
+                        8 * kb]));
 #pragma unroll
             for (int q = 0; q < NT; ++q)
                 b[q] = as_bf16x8(*reinterpret_cast<const uint4*>(
                     &sb[(tap * CO + 16 * q + row) * CL_PAD + 8 * kb]));
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
+            for (int m = 0; m < 4 * RW; ++m)
 #pragma unroll
                 for (int q = 0; q < NT; ++q)
                     acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[q], acc[m][q], 0, 0, 0);
         }
     }
-    const long long prow = ((long long)n * H + y0 + wave) * W + x0;
+    const long long prow = ((long long)n * H + y0 + wave * RW) * W + x0;
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
         const int co = co0 + 16 * q + row;  // accumulator column = lane % 16
-        const float bv = bias ? bias[co] : 0.0f;
+        const float bias_co = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+        for (int m = 0; m < 4 * RW; ++m)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long long pp = prow + 16 * m + 4 * kb + r;  // accumulator row
-                float v = acc[m][q][r] + bv;
+                const long long pp = prow + (long long)(m / 4) * W + 16 * (m % 4) + 4 * kb + r;
+                float v = acc[m][q][r] + bias_co;
                 if (relu) v = fmaxf(v, 0.0f);
                 if (out) out[pp * Cout + co] = f2bf(v);
                 if (outf) outf[pp * Cout + co] = v;
@@ -593,17 +613,20 @@ int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const u
     ConvIn ci{in1, in2, c1, c2, up1};
     const long long M = (long long)n * h * wdt;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (wdt % CL_TX == 0 && h % CL_TY == 0 && cout % 32 == 0) {
-        // operands staged in LDS: workgroups of 4 x 64 pixels x 64 (or 32)
-        // output channels
-        const dim3 g((unsigned)((long long)n * (h / CL_TY) * (wdt / CL_TX)),
-                     (unsigned)(cout % 64 == 0 ? cout / 64 : cout / 32));
+    if (wdt % CL_TX == 0 && h % 4 == 0 && cout % 32 == 0) {
+        // operands staged in LDS: workgroups of 4 x 64 pixels x 64 output
+        // channels, or 8 x 64 pixels x 32 output channels (4 x 64 when the
+        // height is not a multiple of 8)
+        const unsigned tx = (unsigned)(wdt / CL_TX);
         if (cout % 64 == 0)
-            hipLaunchKernelGGL(k_conv3x3_lds<4>, g, dim3(256), 0, st, ci, w, bias, out, outf, n,
-                               h, wdt, cout, relu);
+            hipLaunchKernelGGL((k_conv3x3_lds<4, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 64),
+                               dim3(256), 0, st, ci, w, bias, out, outf, n, h, wdt, cout, relu);
+        else if (h % 8 == 0)
+            hipLaunchKernelGGL((k_conv3x3_lds<2, 2>), dim3((unsigned)n * (h / 8) * tx, cout / 32),
+                               dim3(256), 0, st, ci, w, bias, out, outf, n, h, wdt, cout, relu);
         else
-            hipLaunchKernelGGL(k_conv3x3_lds<2>, g, dim3(256), 0, st, ci, w, bias, out, outf, n,
-                               h, wdt, cout, relu);
+            hipLaunchKernelGGL((k_conv3x3_lds<2, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 32),
+                               dim3(256), 0, st, ci, w, bias, out, outf, n, h, wdt, cout, relu);
         CAGEO_HIP_TRY(hipGetLastError());
         return CAGEO_OK;
     }

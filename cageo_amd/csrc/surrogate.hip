@@ -70,14 +70,21 @@ __global__ void k_dem_partials(const float* __restrict__ dem, long long n, float
 }
 
 // Mean and (population) standard deviation from the partials -> stats[0..1].
+// One wave of 64 threads: strided partial sums, then a butterfly.
 __global__ void k_dem_stats(const double* __restrict__ part, int nparts, float* __restrict__ stats) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
     double c = 0.0, a = 0.0, q = 0.0;
-    for (int i = 0; i < nparts; ++i) {
+    for (int i = threadIdx.x; i < nparts; i += 64) {
         c += part[(size_t)i * 3];
         a += part[(size_t)i * 3 + 1];
         q += part[(size_t)i * 3 + 2];
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off);
+        a += __shfl_xor(a, off);
+        q += __shfl_xor(q, off);
+    }
+    if (threadIdx.x != 0) return;
     double mu = c > 0 ? a / c : 0.0;
     double var = c > 0 ? q / c - mu * mu : 1.0;
     double sd = var > 0 ? sqrt(var) : 0.0;

@@ -72,7 +72,7 @@ struct Options {
     std::string surrogate;
     int surrogateCheck = 8;
     double surrogateL1 = 0.02;
-    double surrogateRmsRatio = 2.36;
+    double surrogateRmsRatio = 2.46;
     int surrogateWindow = 0;
     int surrogateRefine = 200;
     std::string batchFile;
@@ -116,7 +116,7 @@ const char* kHelp =
     "  --surrogate-l1 arg (=0.02)          Mean absolute error the map may have: the\n"
     "                                      check accepts a mean signed error up to half\n"
     "                                      of it and an RMS error up to 1.2 x ratio x it\n"
-    "  --surrogate-rms-ratio arg (=2.36)   RMS / mean absolute error of the network's\n"
+    "  --surrogate-rms-ratio arg (=2.46)   RMS / mean absolute error of the network's\n"
     "                                      errors (measured for the packaged weights)\n"
     "  --surrogate-window arg (=0)         Check windows of N x N cells; failing windows\n"
     "                                      take a --surrogate-refine Monte Carlo (0: one\n"

@@ -549,9 +549,11 @@ def device_inputs(torch, dem, stream, rmse, practical_range, nodata=-9999.0):
 
 
 # RMS / L1 ratio of the packaged network's per-cell errors against
-# 1000-realization Monte Carlo maps, on synthetic 4096^2 DEMs outside its
-# training set (seeds 20240601, 20240611, 20240621: 2.33 to 2.37).
-ERROR_RMS_L1_RATIO = 2.36
+# 1000-realization Monte Carlo maps (RMS net of the maps' own sampling
+# variance), on synthetic 4096^2 DEMs outside its training set (seeds
+# 20240601, 20240611, 20240621, Gaussian PR 60 m, RMSE 1.8 m: 2.44 to 2.46;
+# L1 0.0158 to 0.0161).
+ERROR_RMS_L1_RATIO = 2.46
 
 
 def check_tolerances(l1_target=0.02, rms_l1_ratio=ERROR_RMS_L1_RATIO, margin=1.2):
