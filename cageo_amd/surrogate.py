@@ -367,7 +367,7 @@ def mc_frequency(torch, dems, streams, rmses, ranges, realizations, seed, nodata
 
 def training_set(torch, n_dems=16, size=2048, crop=512, realizations=200, seed=1000,
                  acc_threshold=2000, nodata=-9999.0, model=None, rmse_range=(0.5, 3.0),
-                 pr_range=(30.0, 200.0), log=None, log_every=8):
+                 pr_range=(30.0, 200.0), log=None, log_every=8, fast_convolution=False):
     """Inputs [N][crop][crop][4] and labels [N][crop][crop] as float32 device
     tensors, N = n_dems * (size // crop)^2: synthetic DEMs of the evaluation
     generator (seeds seed + k), each with its own RMSE ~ U(rmse_range) m and
@@ -375,9 +375,11 @@ def training_set(torch, n_dems=16, size=2048, crop=512, realizations=200, seed=1
     realizations of the whole DEM on one context (Gaussian filter unless
     `model`), input planes built on the device (the DEM z-scored over the
     whole raster, as cageo_surrogate_predict does), then cut into crops.
-    The host generates the next DEM while the device labels the current one."""
+    With fast_convolution the labels' error fields come from the fast
+    convolution mode (within 1e-5 of the exact field).  The host generates
+    the next DEM while the device labels the current one."""
     from concurrent.futures import ThreadPoolExecutor
-    from . import Context, GAUSSIAN, create_filter, filter_radius, synth_dem
+    from . import CONV_FAST, Context, GAUSSIAN, create_filter, filter_radius, synth_dem
     model = GAUSSIAN if model is None else model
     rng = np.random.default_rng(seed)
     rmses = [float(rng.uniform(*rmse_range)) for _ in range(n_dems)]
@@ -398,6 +400,8 @@ def training_set(torch, n_dems=16, size=2048, crop=512, realizations=200, seed=1
             dd, ss = T.from_numpy(d).cuda(), T.from_numpy(st).cuda()
             r = filter_radius(ranges[k], 10.0)
             ctx.set_filter(model, r, create_filter(model, ranges[k], 10.0, r))
+            if fast_convolution:
+                ctx.set_conv_mode(CONV_FAST)
             ctx.mc_setup(dd, ss, nodata, nodata, 2, 0.0, rmses[k])
             mc.zero_()
             ctx.run_realizations(mc, realizations, seed + k, 0)
