@@ -106,8 +106,10 @@ const char* kHelp =
     "  --checkpoint-every arg (=0)         Write the resume point every K iterations\n"
     "  --resume arg                        Continue from a resume-point file\n"
     "  --synth W,H,SEED                    Generate synthetic inputs under --dem-path\n"
-    "  --fast-convolution                  Low-rank separable exponential convolution\n"
-    "                                      (within 1e-5 of the exact order, not bit-exact)\n"
+    "  --fast-convolution                  Exponential convolution through FFTs of the\n"
+    "                                      grid's torus (power-of-two sides; else the\n"
+    "                                      filter's low-rank separable form): within 1e-5\n"
+    "                                      of the exact order, not bit-exact\n"
     "  --surrogate arg                     Border-probability network weights (.bin):\n"
     "                                      predict, check with a short Monte Carlo, and\n"
     "                                      write probability x iterations; the full Monte\n"
