@@ -171,7 +171,7 @@ int cageo_refine_probability_map(cageo_context* ctx, float* d_prob, int window,
     CAGEO_TRY(cageo_run_realizations(ctx, d_scratch, n, seed, 0));
     // the window flags ride in the tracing workspace, free after the run
     int* flags = reinterpret_cast<int*>(c->d_ptr);
-    if ((size_t)nwin * sizeof(int) > c->n * sizeof(uint64_t)) return CAGEO_ERR_ARG;
+    if ((size_t)nwin * sizeof(int) > c->n * sizeof(*c->d_ptr)) return CAGEO_ERR_ARG;
     CAGEO_HIP_TRY(hipMemcpyAsync(flags, accepted, (size_t)nwin * sizeof(int),
                                  hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(cageo::k_refine_windows, dim3(2048), dim3(256), 0, c->stream, d_prob,

@@ -285,3 +285,39 @@ def test_native_surrogate_matches_python(env):
                 rw = np.sqrt(max(u[sl].mean(), 0.0))
                 assert abs(wbias[wy, wx] - bw) <= 1e-9 and abs(wrms[wy, wx] - rw) <= 1e-9
                 assert wacc[wy, wx] == (abs(bw) <= 0.01 and rw <= 0.05)
+
+
+def test_refinement_replaces_exactly_the_failing_windows(env):
+    """cageo_refine_probability_map: windows flagged as failing take the
+    frequency count / n of an n-realization Monte Carlo of the whole DEM
+    (the same counts cageo_run_realizations gives for that seed), ragged
+    edge windows included; accepted windows keep the prediction bit for
+    bit."""
+    import cageo_amd as ca
+    torch, sg, L = env
+    announce("refinement of flagged windows 200x150, window 64")
+    dem, st = ca.synth_dem(200, 150, 17, 10.0, 300, -9999.0)
+    ddem, dst = torch.from_numpy(dem).cuda(), torch.from_numpy(st).cuda()
+    r = ca.filter_radius(60.0, 10.0)
+    with ca.Context(200, 150, torch.cuda.current_stream()) as ctx:
+        ctx.set_filter(ca.GAUSSIAN, r, ca.create_filter(ca.GAUSSIAN, 60.0, 10.0, r))
+        ctx.mc_setup(ddem, dst, -9999.0, -9999.0, 2, 0.0, 1.8)
+        nx, ny = ctx.check_windows(64)
+        assert (nx, ny) == (4, 3)  # ceil(200 / 64), ceil(150 / 64)
+        q0 = torch.rand((150, 200), device="cuda", generator=None)
+        q = q0.clone()
+        acc = np.ones((ny, nx), bool)
+        acc[0, 1] = acc[2, 3] = acc[1, 0] = False  # an edge window, the corner, an inner one
+        scratch = torch.empty((150, 200), dtype=torch.float32, device="cuda")
+        ctx.refine_probability_map(q, acc, 12, 33, scratch, window=64)
+        counts = torch.zeros((150, 200), dtype=torch.float32, device="cuda")
+        ctx.run_realizations(counts, 12, 33, 0)
+        torch.cuda.synchronize()
+        # the library scales by the float reciprocal of n
+        freq = (counts.cpu().numpy() * (np.float32(1) / np.float32(12))).astype(np.float32)
+        got, before = q.cpu().numpy(), q0.cpu().numpy()
+    for wy in range(ny):
+        for wx in range(nx):
+            sl = (slice(64 * wy, 64 * wy + 64), slice(64 * wx, 64 * wx + 64))
+            want = before[sl] if acc[wy, wx] else freq[sl]
+            assert np.array_equal(got[sl], want), (wy, wx)
