@@ -243,6 +243,91 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0,
     }
 }
 
+// load_tile and the tile's own states in fewer dependent rounds: each
+// thread loads its strip's 16 cells (state, value and, with zq, the
+// quantized elevation into S.zt) in two batches of 8 rows, the first with
+// the halo ring's cells, so a tile costs two rounds of global loads.
+// Returns the thread's OPEN cells (bit k = row k of its strip).
+__device__ inline uint32_t load_tile_own(PfShared& S, const PfRaster& a, int x0, int y0,
+                                        bool restart, const float* zq) {
+    constexpr int RING = 4 * HALO_W - 4;  // 260 cells
+    constexpr int RIT = (RING + TILE_THREADS - 1) / TILE_THREADS;
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int gx = x0 + lx;
+    auto relay = [&](uint8_t st, float w) {
+        const float r = (st == PF_FIXED_LIVE || (st == PF_OPEN && !restart)) ? w : PF_INF;
+        return make_float2(r, st == PF_FIXED_LIVE ? r : PF_INF);
+    };
+    uint8_t rs[RIT];
+    float rw[RIT];
+    int rl[RIT];
+#pragma unroll
+    for (int j = 0; j < RIT; ++j) {
+        const int k = threadIdx.x + j * TILE_THREADS;
+        rs[j] = PF_NODATA;
+        rw[j] = PF_INF;
+        rl[j] = -1;
+        if (k < RING) {
+            int hx, hy;
+            if (k < HALO_W) {
+                hy = 0;
+                hx = k;
+            } else if (k < 2 * HALO_W) {
+                hy = HALO_W - 1;
+                hx = k - HALO_W;
+            } else {
+                const int m = k - 2 * HALO_W;
+                hy = 1 + (m >> 1);
+                hx = (m & 1) ? HALO_W - 1 : 0;
+            }
+            rl[j] = hy * HALO_W + hx;
+            const int rx = x0 - 1 + hx, ry = y0 - 1 + hy;
+            if (rx >= 0 && ry >= 0 && rx < a.W && ry < a.H) {
+                const size_t gi = (size_t)ry * a.W + rx;
+                rs[j] = a.st[gi];
+                rw[j] = a.w[gi];
+            }
+        }
+    }
+    uint32_t open = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        constexpr int HB = STRIP / 2;
+        uint8_t sv[HB];
+        float wv[HB], zv[HB];
+#pragma unroll
+        for (int k = 0; k < HB; ++k) {
+            const int gy = y0 + strip * STRIP + h * HB + k;
+            sv[k] = PF_NODATA;
+            wv[k] = PF_INF;
+            zv[k] = 0.0f;
+            if (gx < a.W && gy < a.H) {
+                const size_t gi = (size_t)gy * a.W + gx;
+                sv[k] = a.st[gi];
+                wv[k] = a.w[gi];
+                if (zq) zv[k] = zq[gi];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < HB; ++k) {
+            const int ly = strip * STRIP + h * HB + k;
+            S.rel[(ly + 1) * HALO_W + lx + 1] = relay(sv[k], wv[k]);
+            if (zq) S.zt[ly * TILE + lx] = zv[k];
+            if (sv[k] == PF_OPEN) open |= 1u << (h * HB + k);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RIT; ++j)
+        if (rl[j] >= 0) S.rel[rl[j]] = relay(rs[j], rw[j]);
+    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+        S.bits = 0;
+        S.any = 0;
+        S.steps = 0;
+    }
+    return open;
+}
+
 // Ends a tile: OR of border bits, the next phase's list (a tile still open
 // after the first pass of this phase joins phase list `phase_out`),
 // neighbour activation with the rows each neighbour must solve again in
@@ -424,35 +509,15 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         return;
     }
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    load_tile(S, a, x0, y0, restart);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
-    uint32_t open = 0, zlm = 0;  // OPEN cells; OPEN cells whose Z is live
-#pragma unroll
-    for (int k0 = 0; k0 < STRIP; k0 += STRIP / 2) {  // two batches of independent loads
-        uint8_t sv[STRIP / 2];
-        float zv[STRIP / 2];
-#pragma unroll
-        for (int j = 0; j < STRIP / 2; ++j) {
-            const int gy = y0 + strip * STRIP + k0 + j;
-            sv[j] = PF_NODATA;
-            zv[j] = 0.0f;
-            if (gx < a.W && gy < a.H) {
-                const size_t gi = (size_t)gy * a.W + gx;
-                sv[j] = a.st[gi];
-                zv[j] = a.zq[gi];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < STRIP / 2; ++j) {
-            if (sv[j] == PF_OPEN) {
-                open |= 1u << (k0 + j);
-                if (pf_live(zv[j], mn)) zlm |= 1u << (k0 + j);
-            }
-            S.zt[(strip * STRIP + k0 + j) * TILE + lx] = zv[j];
-        }
-    }
+    uint32_t open = load_tile_own(S, a, x0, y0, restart, a.zq);  // OPEN cells
     __syncthreads();
+    uint32_t zlm = 0;  // OPEN cells whose Z is live
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k)
+        if (((open >> k) & 1u) && pf_live(S.zt[(strip * STRIP + k) * TILE + lx], mn))
+            zlm |= 1u << k;
     clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
     const uint32_t rewrite = restart && round > 0 ? open : 0u;  // stale values of the last round
 
@@ -545,23 +610,16 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     if (pass == 0 && __builtin_amdgcn_readfirstlane(ld(&args.topen[b.gbase + b.lb])) == 0u)
         return;  // closed since it joined the phase list
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    load_tile(S, a, x0, y0, false);
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
+    const uint32_t sopen = load_tile_own(S, a, x0, y0, false, nullptr);
     uint32_t open = 0, open_inf = 0;
-    uint8_t sv[STRIP];
-#pragma unroll
-    for (int k = 0; k < STRIP; ++k) {
-        const int gy = y0 + strip * STRIP + k;
-        sv[k] = PF_NODATA;
-        if (gx < a.W && gy < a.H) sv[k] = a.st[(size_t)gy * a.W + gx];
-    }
-    __syncthreads();  // the relays of load_tile are in LDS
+    __syncthreads();  // the relays are in LDS
     clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
     uint32_t lm = 0;  // OPEN cells whose (converged, fixed here) value is live
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        if (sv[k] == PF_OPEN) {
+        if ((sopen >> k) & 1u) {
             const float wk = S.rel[(strip * STRIP + k + 1) * HALO_W + lx + 1].x;
             if (wk < PF_INF) {
                 open |= 1u << k;
