@@ -166,12 +166,12 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     if (dirty == 0) return;  // uniform
     const int tx = b.lb % a.ntx, ty = b.lb / a.ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
-    // halo tile in two batches of independent loads (each thread's loads of
-    // a batch are in flight together)
+    // halo tile in one batch of independent loads (each thread's loads are
+    // in flight together)
     {
         constexpr int N = HALO_W * HALO_W;
         constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
-        constexpr int CHUNK = (ITER + 1) / 2;
+        constexpr int CHUNK = ITER;
 #pragma unroll
         for (int c0 = 0; c0 < ITER; c0 += CHUNK) {
             int2 v[CHUNK];
@@ -321,66 +321,76 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     __syncthreads();
 }
 
+// The three worklist counts start at zero (the D8 kernel fills list 0).
+__global__ void k_fr_reset(uint32_t* __restrict__ ctrl) {
+    if (threadIdx.x < 3) ctrl[CTL_COUNT + threadIdx.x] = 0u;
+}
+
+// D8, one workgroup per block, from the block's halo tile of elevations
+// staged in LDS (one batch of loads); blocks holding flats open list 0.  A
+// launch of its own with every block a workgroup: in the persistent kernel
+// each workgroup walked its blocks one after another, a latency chain.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_fr_d8(FrArgs a) {
+    __shared__ float zt[HALO_W * HALO_W];
+    const int g = blockIdx.x;
+    const FrBlock b = fr_block<MULTI>(a, g);
+    const FrRaster& R = b.R;
+    const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
+    {
+        constexpr int N = HALO_W * HALO_W;
+        constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
+        float v[ITER];
+#pragma unroll
+        for (int j = 0; j < ITER; ++j) {
+            const int k = threadIdx.x + j * TILE_THREADS;
+            const int hy = k / HALO_W, hx = k - hy * HALO_W;
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+            v[j] = 0.0f;
+            if (k < N && gx >= 0 && gy >= 0 && gx < R.W && gy < R.H)
+                v[j] = R.z[(size_t)gy * R.W + gx];
+        }
+#pragma unroll
+        for (int j = 0; j < ITER; ++j) {
+            const int k = threadIdx.x + j * TILE_THREADS;
+            if (k < N) zt[k] = v[j];
+        }
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 63;
+    const int x = x0 + lx;
+    int has_flat = 0;
+    if (x < R.W) {
+#pragma unroll
+        for (int k = 0; k < STRIP; ++k) {
+            const int ly = (threadIdx.x >> 6) * STRIP + k;
+            const int y = y0 + ly;
+            if (y < R.H) {
+                const int32_t d = d8_cell(zt, R.fd, x, y, lx, ly, R.W, R.H);
+                R.dist[(size_t)y * R.W + x] = d;
+                has_flat |= d == D_INF;
+            }
+        }
+    }
+    // the block's dirty rows and epoch stamp start clear: a launch that was
+    // abandoned may have left its first pass's (the persistent kernel has no
+    // barrier before that pass, and the epoch counter only advances when a
+    // launch completes)
+    if (threadIdx.x == 0) {
+        reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
+        a.mark[g] = 0u;
+    }
+    if (__syncthreads_or(has_flat) && threadIdx.x == 0)
+        a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
+}
+
+// The flat passes and the direction pick, after k_fr_d8.
 // MULTI: a table of rasters (else the one raster of the arguments).
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
     __shared__ FrShared S;
     if (grid_aborted(a.gb)) return;
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH_FR]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctrl[CTL_COUNT + 0] = 0;
-        a.ctrl[CTL_COUNT + 1] = 0;
-        a.ctrl[CTL_COUNT + 2] = 0;
-    }
-    if (!grid_sync(a.gb)) return;
-
-    // D8 block by block, from the block's halo tile of elevations staged in
-    // LDS (one batch of loads); blocks holding flats open list 0.
-    float* zt = reinterpret_cast<float*>(S.dz);  // HALO_W x HALO_W floats
-    for (int g = blockIdx.x; g < a.total; g += gridDim.x) {
-        const FrBlock b = fr_block<MULTI>(a, g);
-        const FrRaster& R = b.R;
-        const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
-        {
-            constexpr int N = HALO_W * HALO_W;
-            constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
-            float v[ITER];
-#pragma unroll
-            for (int j = 0; j < ITER; ++j) {
-                const int k = threadIdx.x + j * TILE_THREADS;
-                const int hy = k / HALO_W, hx = k - hy * HALO_W;
-                const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-                v[j] = 0.0f;
-                if (k < N && gx >= 0 && gy >= 0 && gx < R.W && gy < R.H)
-                    v[j] = R.z[(size_t)gy * R.W + gx];
-            }
-#pragma unroll
-            for (int j = 0; j < ITER; ++j) {
-                const int k = threadIdx.x + j * TILE_THREADS;
-                if (k < N) zt[k] = v[j];
-            }
-        }
-        __syncthreads();
-        const int lx = threadIdx.x & 63;
-        const int x = x0 + lx;
-        int has_flat = 0;
-        if (x < R.W) {
-#pragma unroll
-            for (int k = 0; k < STRIP; ++k) {
-                const int ly = (threadIdx.x >> 6) * STRIP + k;
-                const int y = y0 + ly;
-                if (y < R.H) {
-                    const int32_t d = d8_cell(zt, R.fd, x, y, lx, ly, R.W, R.H);
-                    R.dist[(size_t)y * R.W + x] = d;
-                    has_flat |= d == D_INF;
-                }
-            }
-        }
-        if (threadIdx.x == 0) reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
-        if (__syncthreads_or(has_flat) && threadIdx.x == 0)
-            a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
-    }
-    if (!grid_sync(a.gb)) return;
 
     int li = 0;
     bool converged = false;
@@ -494,6 +504,8 @@ int flow_routing(Context* c, const float* z, int8_t* fd) {
     a.ctrl = c->d_ctrl;
     a.gb = context_gridbar(c);
     a.max_passes = 1 << 20;
+    hipLaunchKernelGGL(k_fr_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
+    hipLaunchKernelGGL(k_fr_d8<false>, dim3(a.total), dim3(TILE_THREADS), 0, c->stream, a);
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
     hipLaunchKernelGGL(k_fr_persistent<false>, dim3(c->fr_grid), dim3(TILE_THREADS), 0, c->stream,
                        a);
@@ -533,6 +545,8 @@ int flow_routing_multi(MultiState* m, hipStream_t s) {
     a.ctrl = m->d_ctrl;
     a.gb = multi_gridbar(m);
     a.max_passes = 1 << 20;
+    hipLaunchKernelGGL(k_fr_reset, dim3(1), dim3(64), 0, s, a.ctrl);
+    hipLaunchKernelGGL(k_fr_d8<true>, dim3(a.total), dim3(TILE_THREADS), 0, s, a);
     hipLaunchKernelGGL(k_fr_persistent<true>, dim3(m->fr_grid), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;

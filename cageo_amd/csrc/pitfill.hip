@@ -1061,6 +1061,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         a.topen[g] = 1u;  // until its first solve
         a.tcost[g] = 0u;
         reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
+        // an abandoned launch may have stamped this block with an epoch the
+        // next launch reuses (the counter advances when a launch completes)
+        a.mark[g] = 0u;
     }
     for (int k = threadIdx.x; k < 3 * a.nr; k += blockDim.x)
         if (blockIdx.x == 0) a.cert[k] = 0;
