@@ -66,8 +66,9 @@ STAGE_NAMES = ["random_field", "convolution", "realistic_burn", "pit_fill", "flo
 # kernel ends at once when the first kernel converged alone).
 STAGE_KERNEL = {
     "random_field": ("k_random_field",), "realistic_burn": ("k_realistic_burn_quant",),
-    "pit_fill": ("k_pf_persistent", "k_pf_resident"), "flow_routing": ("k_fr_persistent",),
-    "flow_tracing": ("k_tr_persistent",), "border": ("k_border",),
+    "pit_fill": ("k_pf_persistent", "k_pf_resident"),
+    "flow_routing": ("k_fr_d8", "k_fr_persistent", "k_fr_pick"),
+    "flow_tracing": ("k_tr_tiles", "k_tr_jump", "k_tr_exits"), "border": ("k_border",),
 }
 
 

@@ -317,7 +317,8 @@ enum : int {
     CTL_COMPACTB = 29,  // back count of its compacted fill list
     CTL_RESIDENT = 30,  // pit fill: open tiles handed to the resident rounds (0: none)
     CTL_RROUND = 31,    // pit fill: the round the resident rounds go on from
-    CTL_WORDS = 32
+    CTL_FR_DONE = 32,   // flow routing: 1 once the flat passes of the launch converged
+    CTL_WORDS = 33
 };
 
 // Several rasters in one persistent launch: raster r owns the global 64x64

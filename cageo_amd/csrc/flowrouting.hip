@@ -133,8 +133,11 @@ __device__ inline int32_t d8_cell(const float* zt, int8_t* __restrict__ fd, int 
         if (zc == 0.0f) {
             d = D_STREAM;
         } else {
+            // the direction is carried as its offsets, each a constant of
+            // the unrolled loop (indexing the tables by a variable would load
+            // them from memory)
             float steepest = 0.0f;
-            int sk = -1;
+            d = D_INF;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 float s = zc - zt[c + FDY[k] * HALO_W + FDX[k]];
@@ -142,16 +145,11 @@ __device__ inline int32_t d8_cell(const float* zt, int8_t* __restrict__ fd, int 
                     if (k & 1) s = s / 1.41421356237309505f;
                     if (s > steepest) {
                         steepest = s;
-                        sk = k;
+                        ox = (int8_t)FDX[k];
+                        oy = (int8_t)FDY[k];
+                        d = 0;
                     }
                 }
-            }
-            if (sk >= 0) {
-                ox = (int8_t)FDX[sk];
-                oy = (int8_t)FDY[sk];
-                d = 0;
-            } else {
-                d = D_INF;
             }
         }
     }
@@ -324,6 +322,7 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
 // The three worklist counts start at zero (the D8 kernel fills list 0).
 __global__ void k_fr_reset(uint32_t* __restrict__ ctrl) {
     if (threadIdx.x < 3) ctrl[CTL_COUNT + threadIdx.x] = 0u;
+    if (threadIdx.x == 3) ctrl[CTL_FR_DONE] = 0u;
 }
 
 // D8, one workgroup per block, from the block's halo tile of elevations
@@ -411,56 +410,91 @@ __global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
         if (!grid_sync(a.gb)) return;
         li = cout;
     }
-    // Nothing below writes a control word that the pass loop reads, so no
-    // barrier is needed between the last count read and the direction pick.
+    // The words written below are not read by the pass loop, so no barrier
+    // is needed between the last count read and them.
     if (!converged) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             a.ctrl[CTL_STATUS] = (uint32_t)(-CAGEO_ERR_NOT_CONVERGED);
         return;
     }
 
-    // Directions of the resolved flats, in units of UNIT_ROWS x 64 cells.
-    const long long units = (long long)a.total * UNITS_PER_BLOCK;
-    for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
-        const FrBlock b = fr_block<MULTI>(a, (int)(u / UNITS_PER_BLOCK));
-        const FrRaster& R = b.R;
-        const int x = (b.lb % R.ntx) * TILE + (threadIdx.x & 63);
-        const int y0 = (b.lb / R.ntx) * TILE + (int)(u % UNITS_PER_BLOCK) * UNIT_ROWS;
-        if (x >= R.W) continue;
-        int32_t dv[UNIT_ROWS];
-#pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k)
-            dv[k] = y0 + k < R.H ? R.dist[(size_t)(y0 + k) * R.W + x] : 0;
-#pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k) {
-            const int y = y0 + k;
-            const int32_t d = dv[k];
-            if (y >= R.H || d < 1 || d >= D_INF) continue;
-            const size_t i = (size_t)y * R.W + x;
-            // the cell's loads at once: no load waits on a test of another
-            // (a neighbour outside the grid reads the cell itself, distance d)
-            const float zc = R.z[i];
-            int32_t nd[8];
-            float nz[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int nx = x + FDX[q], ny = y + FDY[q];
-                const bool in = nx >= 0 && ny >= 0 && nx < R.W && ny < R.H;
-                const size_t ni = in ? (size_t)ny * R.W + nx : i;
-                nd[q] = R.dist[ni];
-                nz[q] = R.z[ni];
-            }
-            int sk = -1;
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (nd[q] == d - 1 && nz[q] <= zc) sk = q;
-            if (sk >= 0)
-                reinterpret_cast<char2*>(R.fd)[i] = make_char2((int8_t)FDX[sk], (int8_t)FDY[sk]);
-        }
-    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctrl[CTL_EPOCH_FR] = epoch;
         a.ctrl[CTL_FR_PASSES] = (uint32_t)pass;  // read by the host only
+        a.ctrl[CTL_FR_DONE] = 1u;               // k_fr_pick may go on
+    }
+}
+
+// Directions of the resolved flats (FlowRoutingAlgorithmCPUOpenMP.h:
+// 153-249): the last neighbour in order with d(n) = d(c) - 1 and
+// z_n <= z_c.  One workgroup per block: its own distances first (a block
+// without a resolved flat is done), then distances and elevations with a
+// one-cell halo staged in LDS in one batch of loads; a halo cell outside the
+// grid reads D_OUT, which no d(c) - 1 >= 0 equals.  Runs only after flat
+// passes that converged (k_fr_persistent sets CTL_FR_DONE).
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_fr_pick(FrArgs a) {
+    __shared__ int2 dz[HALO_W * HALO_W];  // x: distance, y: bits of z
+    if (ld(&a.ctrl[CTL_FR_DONE]) == 0u) return;
+    const FrBlock b = fr_block<MULTI>(a, blockIdx.x);
+    const FrRaster& R = b.R;
+    const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
+    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int x = x0 + lx;
+    uint32_t flat = 0;
+    {
+        int32_t dv[STRIP];
+#pragma unroll
+        for (int k = 0; k < STRIP; ++k) {
+            const int y = y0 + strip * STRIP + k;
+            const bool in = x < R.W && y < R.H;
+            dv[k] = R.dist[in ? (size_t)y * R.W + x : 0];
+            if (!in) dv[k] = 0;
+        }
+#pragma unroll
+        for (int k = 0; k < STRIP; ++k)
+            if (dv[k] >= 1 && dv[k] < D_INF) flat |= 1u << k;
+    }
+    if (!__syncthreads_or(flat != 0u)) return;
+    {
+        constexpr int N = HALO_W * HALO_W;
+        constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
+        int2 v[ITER];
+#pragma unroll
+        for (int j = 0; j < ITER; ++j) {
+            const int k = threadIdx.x + j * TILE_THREADS;
+            const int hy = k / HALO_W, hx = k - hy * HALO_W;
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+            const bool in = k < N && gx >= 0 && gy >= 0 && gx < R.W && gy < R.H;
+            const size_t gi = in ? (size_t)gy * R.W + gx : 0;
+            v[j] = make_int2(R.dist[gi], __float_as_int(R.z[gi]));
+            if (!in) v[j].x = D_OUT;
+        }
+#pragma unroll
+        for (int j = 0; j < ITER; ++j) {
+            const int k = threadIdx.x + j * TILE_THREADS;
+            if (k < N) dz[k] = v[j];
+        }
+    }
+    __syncthreads();
+    while (flat) {
+        const int k = __builtin_ctz(flat);
+        flat &= flat - 1;
+        const int ly = strip * STRIP + k;
+        const int li = (ly + 1) * HALO_W + lx + 1;
+        const int2 c = dz[li];
+        const float zc = __int_as_float(c.y);
+        bool found = false;
+        char2 dir = make_char2(0, 0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int2 n = dz[li + FDY[q] * HALO_W + FDX[q]];
+            if (n.x == c.x - 1 && __int_as_float(n.y) <= zc) {  // the last match
+                found = true;
+                dir = make_char2((int8_t)FDX[q], (int8_t)FDY[q]);
+            }
+        }
+        if (found) reinterpret_cast<char2*>(R.fd)[(size_t)(y0 + ly) * R.W + x] = dir;
     }
 }
 
@@ -509,6 +543,7 @@ int flow_routing(Context* c, const float* z, int8_t* fd) {
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
     hipLaunchKernelGGL(k_fr_persistent<false>, dim3(c->fr_grid), dim3(TILE_THREADS), 0, c->stream,
                        a);
+    hipLaunchKernelGGL(k_fr_pick<false>, dim3(a.total), dim3(TILE_THREADS), 0, c->stream, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
@@ -548,6 +583,7 @@ int flow_routing_multi(MultiState* m, hipStream_t s) {
     hipLaunchKernelGGL(k_fr_reset, dim3(1), dim3(64), 0, s, a.ctrl);
     hipLaunchKernelGGL(k_fr_d8<true>, dim3(a.total), dim3(TILE_THREADS), 0, s, a);
     hipLaunchKernelGGL(k_fr_persistent<true>, dim3(m->fr_grid), dim3(TILE_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_fr_pick<true>, dim3(a.total), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }

@@ -73,10 +73,14 @@ __device__ inline uint64_t hop(uint64_t e, uint64_t f) {
     return pack(k, s, value_of(f));
 }
 
-__device__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
-                           const uint8_t* __restrict__ sf, const int32_t* __restrict__ slabel,
-                           uint64_t* __restrict__ ent, int32_t* __restrict__ lab,
-                           uint64_t* __restrict__ bent, int W, int H, int ntx, int t) {
+// Inlined: as a call its frame spilled to scratch and its registers set the
+// kernel's occupancy.
+__device__ __forceinline__ void trace_tile(uint64_t* L, const int8_t* __restrict__ fd,
+                                           const uint8_t* __restrict__ sf,
+                                           const int32_t* __restrict__ slabel,
+                                           uint64_t* __restrict__ ent, int32_t* __restrict__ lab,
+                                           uint64_t* __restrict__ bent, int W, int H, int ntx,
+                                           int t) {
     const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
@@ -290,32 +294,33 @@ __device__ inline int32_t label_of_cell0(const TrRaster& R) {
     return kind_of(f) == K_RESOLVED ? value_of(f) : 0;
 }
 
-// The tile pass, the border-table jumping, the exit cells' labels and the
-// connect step in one persistent launch over every raster.  Round r of the
-// jumping raises flag r % 3 while exits remain and clears flag (r+1) % 3,
-// whose last read (end of round r-2) is separated from the clear by
-// barriers.  The exit cells' pass goes by units of UNIT_ROWS x 64 cells, one
-// wave each, with each unit's loads issued before its stores.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
-    __shared__ uint64_t L[TILE * TILE];
-    if (grid_aborted(a.gb)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctrl[CTL_FLAG + 0] = 0;
-        a.ctrl[CTL_FLAG + 1] = 0;
-        a.ctrl[CTL_FLAG + 2] = 0;
-    }
-    for (int g = blockIdx.x; g < a.total; g += gridDim.x) {
-        const TrBlock b = tr_block<MULTI>(a, g);
-        const TrRaster& R = b.R;
-        if (b.lb == 0 && threadIdx.x == 0) R.scal[SC_LONGPATH] = 0;
-        trace_tile(L, R.fd, R.sf, R.slabel, R.ent, R.lab, R.bent, R.W, R.H, R.ntx, b.lb);
-    }
-    if (!grid_sync(a.gb)) return;
+// Three launches over every raster: the tile pass (one workgroup per
+// block), the border-table jumping (persistent, grid barriers between
+// rounds) and the exit cells' labels with the connect step (one workgroup
+// per block).  The two per-block launches are short-lived and light on
+// registers, so many blocks' load chains are in flight at once; only the
+// jumping needs co-resident workgroups.
 
-    // Jumping over the border tables: thread s takes slot s % BORDER_SLOTS
-    // of global block s / BORDER_SLOTS (the blocks of one raster are
-    // consecutive, so a wave's 64 slots lie in one raster).
+// Tile pass.  Also clears the jumping's three round flags and the long-path
+// count: the launch boundary orders both before their first use.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_tr_tiles(TrArgs a) {
+    __shared__ uint64_t L[TILE * TILE];
+    if (blockIdx.x == 0 && threadIdx.x < 3) a.ctrl[CTL_FLAG + threadIdx.x] = 0;
+    const TrBlock b = tr_block<MULTI>(a, blockIdx.x);
+    const TrRaster& R = b.R;
+    if (b.lb == 0 && threadIdx.x == 0) R.scal[SC_LONGPATH] = 0;
+    trace_tile(L, R.fd, R.sf, R.slabel, R.ent, R.lab, R.bent, R.W, R.H, R.ntx, b.lb);
+}
+
+// Jumping over the border tables: thread s takes slot s % BORDER_SLOTS of
+// global block s / BORDER_SLOTS (the blocks of one raster are consecutive,
+// so a wave's 64 slots lie in one raster).  Round r raises flag r % 3 while
+// exits remain and clears flag (r+1) % 3, whose last read (end of round
+// r-2) is separated from the clear by barriers.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_tr_jump(TrArgs a) {
+    if (grid_aborted(a.gb)) return;
     const long long slots = (long long)a.total * BORDER_SLOTS;
     int rounds = 0;
     for (int r = 0; r < a.max_rounds; ++r) {
@@ -339,13 +344,17 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
         if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
+}
 
-    // Exit cells take their slot's label; performConnectFlowPath maps a
-    // label 0 to label[0] when that is negative.  An exit whose chain is
-    // still moving (a cycle through tiles) or ends stuck stays 0.
-    const long long units = (long long)a.total * UNITS_PER_BLOCK;
-    for (long long u = wave_unit_first(); u < units; u += wave_unit_stride()) {
-        const TrUnit t = tr_unit<MULTI>(a, u);
+// Exit cells take their slot's label; performConnectFlowPath maps a label 0
+// to label[0] when that is negative.  An exit whose chain is still moving
+// (a cycle through tiles) or ends stuck stays 0.  Each wave takes units of
+// UNIT_ROWS x 64 cells of the workgroup's block, each unit's loads issued
+// before its stores.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_tr_exits(TrArgs a) {
+    for (int k0 = (int)(threadIdx.x >> 6); k0 < UNITS_PER_BLOCK; k0 += TILE_THREADS / 64) {
+        const TrUnit t = tr_unit<MULTI>(a, (long long)blockIdx.x * UNITS_PER_BLOCK + k0);
         const int32_t l0 = label_of_cell0(t.R);
         int32_t v[UNIT_ROWS];
 #pragma unroll
@@ -384,6 +393,19 @@ __global__ __launch_bounds__(256) void k_tr_persistent(TrArgs a) {
     }
 }
 
+// The jumping's co-resident workgroups: at most two per CU; with a few
+// thousand slots per round more workgroups would only lengthen its
+// barriers.
+int jump_cap(const void* fn, int device, int* cap) {
+    int per_cu = 0;
+    CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE_THREADS, 0));
+    hipDeviceProp_t prop;
+    CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (per_cu < 1) return CAGEO_ERR_HIP;
+    *cap = std::min(per_cu, 2) * prop.multiProcessorCount;
+    return CAGEO_OK;
+}
+
 int max_rounds_for(size_t cells) {
     int m = 2;
     for (size_t k = 1; k < cells; k <<= 1) ++m;
@@ -412,14 +434,7 @@ TrRaster tr_raster(Context* c, const int8_t* fd, const uint8_t* sflags, const in
 // left in the scalar block.
 // Co-resident workgroups of the persistent kernel on this device.
 int flow_tracing_grid_cap(Context* c, int* cap) {
-    int per_cu = 0;
-    CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k_tr_persistent<false>), TILE_THREADS, 0));
-    hipDeviceProp_t prop;
-    CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, c->device));
-    if (per_cu < 1) return CAGEO_ERR_HIP;
-    *cap = per_cu * prop.multiProcessorCount;
-    return CAGEO_OK;
+    return jump_cap(reinterpret_cast<const void*>(k_tr_jump<false>), c->device, cap);
 }
 
 int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int32_t* slabel,
@@ -434,9 +449,10 @@ int flow_tracing(Context* c, const int8_t* fd, const uint8_t* sflags, const int3
     a.max_rounds = max_rounds_for(c->n);
     a.ctrl = c->d_ctrl;
     a.gb = context_gridbar(c);
+    hipLaunchKernelGGL(k_tr_tiles<false>, dim3(a.total), dim3(TILE_THREADS), 0, c->stream, a);
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
-    hipLaunchKernelGGL(k_tr_persistent<false>, dim3(c->tr_grid), dim3(TILE_THREADS), 0, c->stream,
-                       a);
+    hipLaunchKernelGGL(k_tr_jump<false>, dim3(c->tr_grid), dim3(TILE_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(k_tr_exits<false>, dim3(a.total), dim3(TILE_THREADS), 0, c->stream, a);
     CAGEO_HIP_TRY(hipGetLastError());
     if (long_paths) {
         CAGEO_TRY(read_scalars(c));
@@ -455,15 +471,7 @@ int flow_tracing_tables(MultiState* m, Context* const* ctxs) {
     }
     CAGEO_HIP_TRY(hipMalloc(&m->d_tr, t.size() * sizeof(TrRaster)));
     CAGEO_HIP_TRY(hipMemcpy(m->d_tr, t.data(), t.size() * sizeof(TrRaster), hipMemcpyHostToDevice));
-    {
-        int per_cu = 0;
-        CAGEO_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(k_tr_persistent<true>), TILE_THREADS, 0));
-        hipDeviceProp_t prop;
-        CAGEO_HIP_TRY(hipGetDeviceProperties(&prop, ctxs[0]->device));
-        if (per_cu < 1) return CAGEO_ERR_HIP;
-        m->tr_cap = per_cu * prop.multiProcessorCount;
-    }
+    CAGEO_TRY(jump_cap(reinterpret_cast<const void*>(k_tr_jump<true>), ctxs[0]->device, &m->tr_cap));
     if (m->tr_grid <= 0) m->tr_grid = m->tr_cap;
     return CAGEO_OK;
 }
@@ -478,7 +486,9 @@ int flow_tracing_multi(MultiState* m, hipStream_t s) {
     a.max_rounds = max_rounds_for((size_t)m->max_cells);
     a.ctrl = m->d_ctrl;
     a.gb = multi_gridbar(m);
-    hipLaunchKernelGGL(k_tr_persistent<true>, dim3(m->tr_grid), dim3(TILE_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_tr_tiles<true>, dim3(a.total), dim3(TILE_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_tr_jump<true>, dim3(m->tr_grid), dim3(TILE_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_tr_exits<true>, dim3(a.total), dim3(TILE_THREADS), 0, s, a);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }

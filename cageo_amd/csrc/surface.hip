@@ -60,20 +60,35 @@ __global__ __launch_bounds__(256) void k_realistic(const float* __restrict__ dem
 }
 
 // Burning works on LDS tiles of 64 x 16 cells with an R-cell halo holding
-// the unburned values and the stream flags.  For an in-grid neighbour the
-// reference's "already processed" test (raster index n < i) reduces to the
-// offset alone: dy < 0, or dy == 0 and dx < 0.
+// the unburned values and, per cell, the two facts the search asks about a
+// neighbour n, given that n is in the grid and a stream cell:
+//   BF_BURN  n is a stream cell;
+//   BF_LATE  n is a stream cell whose unburned value is not nodata_dem.
+// A neighbour that precedes the cell in raster order (offset dy < 0, or
+// dy == 0 and dx < 0: for in-grid cells the reference's n < i reduces to
+// the offset alone) has been burned to 0 already and qualifies when
+// 0 != nodata_dem, i.e. by BF_BURN under that launch constant; a later one
+// qualifies by BF_LATE.  Halo cells outside the grid hold no flag, which
+// is the reference's in-grid test.  Each offset's LDS step and the flag it
+// needs are packed into one word of a table staged in LDS.
 constexpr int BT_X = 64, BT_Y = 16;
 constexpr int BH_MAX = CAGEO_MAX_BEVEL_RADIUS;
 constexpr int BL_W = BT_X + 2 * BH_MAX, BL_H = BT_Y + 2 * BH_MAX;
+constexpr int BOFF_MAX = (2 * BH_MAX + 1) * (2 * BH_MAX + 1);
+constexpr uint8_t BF_BURN = 1, BF_LATE = 2;
 
-template <class ZR>
+// REAL: the tile's values are realistic_value(z0, z1) (the fused Monte
+// Carlo pass), else z0 itself.
+template <bool REAL>
 __device__ inline void burn_tile(int W, int H, const uint8_t* __restrict__ sf,
                                  const int* __restrict__ off, const float* __restrict__ fac,
-                                 int n_off, int R, float nodata_dem, ZR zr, float out[4],
-                                 int gx_out[4], int gy_out[4]) {
+                                 int n_off, int R, float nodata_dem,
+                                 const float* __restrict__ z0, const float* __restrict__ z1,
+                                 float out[4], int gx_out[4], int gy_out[4]) {
     __shared__ float zt[BL_W * BL_H];
     __shared__ uint8_t ft[BL_W * BL_H];
+    __shared__ int s_step[BOFF_MAX];  // LDS step << 2 | flag the offset needs
+    __shared__ float s_fac[BOFF_MAX];
     constexpr int ITER = (BL_W * BL_H + 255) / 256;  // loads per thread at the largest radius
     // Blocks go round robin over the 8 XCDs; renumbering them so that each
     // XCD takes a contiguous band of tiles keeps the neighbours whose cache
@@ -84,60 +99,78 @@ __device__ inline void burn_tile(int W, int H, const uint8_t* __restrict__ sf,
     if ((nb & 7) == 0) t = (t & 7) * (nb >> 3) + (t >> 3);
     const int x0 = (int)(t % gridDim.x) * BT_X, y0 = (int)(t / gridDim.x) * BT_Y;
     const int lw = BT_X + 2 * R, lh = BT_Y + 2 * R;
-    // all of a thread's loads are issued before its LDS stores
-    float zv[ITER];
+    // All of a thread's loads are issued before any of them is used: a cell
+    // outside the grid or the halo of radius R loads cell 0 (cached) and is
+    // replaced afterwards, so no load sits in a branch that would have to
+    // wait for it.
+    float av[ITER], bv[ITER];
     uint8_t fv[ITER];
+    bool in[ITER];
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
         const int k = threadIdx.x + it * 256;
-        zv[it] = 0.0f;
-        fv[it] = 0;
-        if (k < lw * lh) {
-            const int ly = k / lw, lx = k - ly * lw;
-            const int gx = x0 - R + lx, gy = y0 - R + ly;
-            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-                const size_t gi = (size_t)gy * W + gx;
-                zv[it] = zr(gi);
-                fv[it] = sf[gi];
-            }
-        }
+        const int ly = k / lw, lx = k - ly * lw;
+        const int gx = x0 - R + lx, gy = y0 - R + ly;
+        in[it] = k < lw * lh && gx >= 0 && gy >= 0 && gx < W && gy < H;
+        const size_t gi = in[it] ? (size_t)gy * W + gx : 0;
+        av[it] = z0[gi];
+        bv[it] = REAL ? z1[gi] : 0.0f;
+        fv[it] = sf[gi];
+    }
+    if ((int)threadIdx.x < n_off) {
+        const int o = threadIdx.x, ox = off[2 * o], oy = off[2 * o + 1];
+        const bool early = oy < 0 || (oy == 0 && ox < 0);
+        const int need = early ? (nodata_dem != 0.0f ? BF_BURN : 0) : BF_LATE;
+        s_step[o] = (oy * BL_W + ox) * 4 + need;
+        s_fac[o] = fac[o];
     }
 #pragma unroll
     for (int it = 0; it < ITER; ++it) {
         const int k = threadIdx.x + it * 256;
         if (k < lw * lh) {
             const int ly = k / lw, lx = k - ly * lw;
-            zt[ly * BL_W + lx] = zv[it];
-            ft[ly * BL_W + lx] = fv[it];
+            float v = 0.0f;
+            uint8_t f = 0;
+            if (in[it]) {
+                v = REAL ? realistic_value(av[it], bv[it], nodata_dem) : av[it];
+                if (fv[it] & SF_BURN) f = BF_BURN | (v != nodata_dem ? BF_LATE : 0);
+            }
+            zt[ly * BL_W + lx] = v;
+            ft[ly * BL_W + lx] = f;
         }
     }
     __syncthreads();
+    // Each thread's four cells (one column, four rows) search side by side:
+    // per offset one table read and four independent flag reads.  A cell
+    // takes the first offset, in the search array's order, whose cell
+    // qualifies; stream cells become 0.
     const int lx = threadIdx.x & 63;
+    int c[4];
+    bool done[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int ly = (threadIdx.x >> 6) * 4 + j;
-        const int x = x0 + lx, y = y0 + ly;
-        gx_out[j] = x;
-        gy_out[j] = y;
-        const int c = (ly + R) * BL_W + lx + R;
-        float v = zt[c];
-        if (ft[c] & SF_BURN) {
-            v = 0.0f;
-        } else {
-            for (int o = 0; o < n_off; ++o) {
-                const int ox = off[2 * o], oy = off[2 * o + 1];
-                const int nx = x + ox, ny = y + oy;
-                if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
-                const int ni = c + oy * BL_W + ox;
-                if (!(ft[ni] & SF_BURN)) continue;
-                const float seen = (oy < 0 || (oy == 0 && ox < 0)) ? 0.0f : zt[ni];
-                if (seen != nodata_dem) {
-                    v = v * fac[o];
-                    break;
-                }
+        gx_out[j] = x0 + lx;
+        gy_out[j] = y0 + ly;
+        c[j] = (ly + R) * BL_W + lx + R;
+        out[j] = zt[c[j]];
+        done[j] = (ft[c[j]] & BF_BURN) != 0;
+        if (done[j]) out[j] = 0.0f;
+    }
+    for (int o = 0; o < n_off; ++o) {
+        if (!__any(!(done[0] && done[1] && done[2] && done[3]))) break;
+        const int s = s_step[o];
+        const float f = s_fac[o];
+        uint8_t q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = ft[c[j] + (s >> 2)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!done[j] && (q[j] & s & 3)) {
+                out[j] = out[j] * f;
+                done[j] = true;
             }
         }
-        out[j] = v;
     }
 }
 
@@ -149,8 +182,7 @@ __global__ __launch_bounds__(256) void k_burn(const float* __restrict__ zin,
                                               int W, int H, float nodata_dem) {
     float v[4];
     int gx[4], gy[4];
-    burn_tile(W, H, sf, off, fac, n_off, R, nodata_dem, [&](size_t i) { return zin[i]; }, v, gx,
-              gy);
+    burn_tile<false>(W, H, sf, off, fac, n_off, R, nodata_dem, zin, nullptr, v, gx, gy);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
         if (gx[j] < W && gy[j] < H) zout[(size_t)gy[j] * W + gx[j]] = v[j];
@@ -228,8 +260,7 @@ __global__ __launch_bounds__(256) void k_realistic_burn_quant(
     float nodata_dem, float* __restrict__ zq, uint32_t* __restrict__ scal) {
     float v[4];
     int gx[4], gy[4];
-    burn_tile(W, H, sf, off, fac, n_off, R, nodata_dem,
-              [&](size_t i) { return realistic_value(dem[i], e[i], nodata_dem); }, v, gx, gy);
+    burn_tile<true>(W, H, sf, off, fac, n_off, R, nodata_dem, dem, e, v, gx, gy);
     uint32_t mn = 0xffffffffu, mx = 0u, bad = 0u;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
