@@ -142,17 +142,40 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn ci, const uint16_t* __re
 // workgroup instead of once per tap and wave.  Pixel rows in LDS are
 // padded to 40 bf16 (80 bytes) so 16 lanes reading 16 consecutive pixels
 // hit 16 distinct 4-bank groups.
+//
+// Without an fp32 output the results go out through LDS: the workgroup's
+// bias + ReLU outputs, rounded to bf16, are staged there, then written
+// with 16-byte stores (a pixel's channels are contiguous), and the fused
+// consumers of the output read them there (ConvFuse): the 2x2 average pool
+// and the 1x1 head, each in the same arithmetic as its own kernel
+// (k_avgpool2, k_head), so a fused layer gives the same bits as the layer
+// followed by that kernel.
 constexpr int CL_TX = 64, CL_PAD = 40;
+
+struct ConvFuse {
+    uint16_t* pool;       // [N][H/2][W/2][Cout]: k_avgpool2 of the output, or null
+    const float* head_w;  // [Cout] and [1]: the 1x1 head of k_head
+    const float* head_b;
+    float* prob;          // [N][H][W] head output, or null (then head_w/head_b unused)
+    int sigmoid;
+};
+
 template <int NT, int RW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv3x3_lds(ConvIn ci, const uint16_t* __restrict__ w,
                                                      const float* __restrict__ bias,
                                                      uint16_t* __restrict__ out,
-                                                     float* __restrict__ outf, int N, int H,
-                                                     int W, int Cout, int relu) {
+                                                     float* __restrict__ outf, ConvFuse fz,
+                                                     int N, int H, int W, int Cout, int relu) {
     constexpr int TY = 4 * RW, AX = CL_TX + 2, AY = TY + 2, CO = 16 * NT;
     constexpr int TAP_UNROLL = RW == 1 ? 9 : 1;  // two rows a wave: registers run short
-    __shared__ __align__(16) uint16_t sa[AY * AX * CL_PAD];
-    __shared__ __align__(16) uint16_t sb[9 * CO * CL_PAD];
+    constexpr int SA_N = AY * AX * CL_PAD, SB_N = 9 * CO * CL_PAD;
+    constexpr int NPX = TY * CL_TX;      // pixels of the output tile
+    constexpr int PS = CO + 8;           // staged pixel stride (bf16), padded by 16 bytes
+    constexpr int HP = NPX + 2;          // staged channel stride of the head's layout
+    static_assert(NPX * PS <= SA_N + SB_N && CO * HP <= SA_N + SB_N, "epilogue staging");
+    __shared__ __align__(16) uint16_t smem[SA_N + SB_N];
+    uint16_t* const sa = smem;
+    uint16_t* const sb = smem + SA_N;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, kb = lane >> 4;
     const int tiles_x = W / CL_TX, tiles_y = H / TY;
@@ -244,21 +267,93 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[q], acc[m][q], 0, 0, 0);
         }
     }
-    const long long prow = ((long long)n * H + y0 + wave * RW) * W + x0;
+    if (outf) {  // fp32 output too: straight from the accumulators
+        const long long prow = ((long long)n * H + y0 + wave * RW) * W + x0;
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const int co = co0 + 16 * q + row;  // accumulator column = lane % 16
+            const float bias_co = bias ? bias[co] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < 4 * RW; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long long pp = prow + (long long)(m / 4) * W + 16 * (m % 4) + 4 * kb + r;
+                    float v = acc[m][q][r] + bias_co;
+                    if (relu) v = fmaxf(v, 0.0f);
+                    if (out) out[pp * Cout + co] = f2bf(v);
+                    outf[pp * Cout + co] = v;
+                }
+        }
+        return;
+    }
+    __syncthreads();  // every wave's operand reads of the last slice are done
+    // the output tile's pixel (0, 0) in the NHWC raster; the head layout is
+    // channel-major (a thread then reads its pixel's channels in order, lanes
+    // on consecutive pixels), the other pixel-major
+    const size_t pix0 = ((size_t)n * H + y0) * W + x0;
+    const bool head = fz.prob != nullptr;
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
-        const int co = co0 + 16 * q + row;  // accumulator column = lane % 16
-        const float bias_co = bias ? bias[co] : 0.0f;
+        const int c = 16 * q + row;  // accumulator column = lane % 16
+        const float bias_co = bias ? bias[co0 + c] : 0.0f;
 #pragma unroll
         for (int m = 0; m < 4 * RW; ++m)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long long pp = prow + (long long)(m / 4) * W + 16 * (m % 4) + 4 * kb + r;
+                const int px = (wave * RW + m / 4) * CL_TX + 16 * (m % 4) + 4 * kb + r;
                 float v = acc[m][q][r] + bias_co;
                 if (relu) v = fmaxf(v, 0.0f);
-                if (out) out[pp * Cout + co] = f2bf(v);
-                if (outf) outf[pp * Cout + co] = v;
+                smem[head ? c * HP + px : px * PS + c] = f2bf(v);
             }
+    }
+    __syncthreads();
+    if (head) {  // CO == Cout (the host checks): the workgroup holds every channel
+        for (int px = threadIdx.x; px < NPX; px += 256) {
+            float s = fz.head_b[0];
+            for (int c = 0; c < CO; ++c) s += fz.head_w[c] * bf2f(smem[c * HP + px]);
+            fz.prob[pix0 + (size_t)(px / CL_TX) * W + px % CL_TX] =
+                fz.sigmoid ? 1.0f / (1.0f + expf(-s)) : s;
+        }
+        return;
+    }
+    constexpr int CH = CO / 8;  // 16-byte chunks of a pixel's channels
+    if (out) {
+        for (int i = threadIdx.x; i < NPX * CH; i += 256) {
+            const int px = i / CH, part = i - px * CH;
+            const size_t g = (pix0 + (size_t)(px / CL_TX) * W + px % CL_TX) * Cout + co0 + 8 * part;
+            *reinterpret_cast<uint4*>(out + g) =
+                *reinterpret_cast<const uint4*>(smem + px * PS + 8 * part);
+        }
+    }
+    if (fz.pool) {
+        const int Wo = W / 2;
+        const size_t q0 = ((size_t)n * (H / 2) + y0 / 2) * Wo + x0 / 2;
+        for (int i = threadIdx.x; i < NPX / 4 * CH; i += 256) {
+            const int pp = i / CH, part = i - pp * CH;
+            const int py = pp / (CL_TX / 2), px = pp - py * (CL_TX / 2);
+            const uint16_t* s00 = smem + (2 * py * CL_TX + 2 * px) * PS + 8 * part;
+            const uint4 a = *reinterpret_cast<const uint4*>(s00);                 // (2y, 2x)
+            const uint4 b = *reinterpret_cast<const uint4*>(s00 + PS);            // (2y, 2x+1)
+            const uint4 c = *reinterpret_cast<const uint4*>(s00 + CL_TX * PS);    // (2y+1, 2x)
+            const uint4 d = *reinterpret_cast<const uint4*>(s00 + (CL_TX + 1) * PS);
+            const uint32_t ua[4] = {a.x, a.y, a.z, a.w}, ub[4] = {b.x, b.y, b.z, b.w};
+            const uint32_t uc[4] = {c.x, c.y, c.z, c.w}, ud[4] = {d.x, d.y, d.z, d.w};
+            uint32_t res[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t h2 = 0;
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    const int sh = 16 * hh;
+                    const float sum = bf2f((uint16_t)(ua[k] >> sh)) + bf2f((uint16_t)(ub[k] >> sh)) +
+                                      bf2f((uint16_t)(uc[k] >> sh)) + bf2f((uint16_t)(ud[k] >> sh));
+                    h2 |= (uint32_t)f2bf(0.25f * sum) << sh;
+                }
+                res[k] = h2;
+            }
+            *reinterpret_cast<uint4*>(fz.pool + (q0 + (size_t)py * Wo + px) * Cout + co0 + 8 * part) =
+                make_uint4(res[0], res[1], res[2], res[3]);
+        }
     }
 }
 
@@ -598,6 +693,34 @@ int grid_for(long long n) {
 
 }  // namespace
 
+// The LDS-staged convolution applies: whole 64-pixel row segments, whole
+// 4-row blocks, output channels in blocks of 32.
+inline bool lds_conv_applies(int h, int wdt, int cout) {
+    return wdt % CL_TX == 0 && h % 4 == 0 && cout % 32 == 0;
+}
+// Output channels of one workgroup of the LDS-staged convolution.
+inline int lds_conv_block(int cout) { return cout % 64 == 0 ? 64 : 32; }
+
+// Workgroups of 4 x 64 pixels x 64 output channels, or 8 x 64 pixels x 32
+// output channels (4 x 64 when the height is not a multiple of 8).
+int launch_conv_lds(void* stream, const ConvIn& ci, const uint16_t* w, const float* bias,
+                    uint16_t* out, float* outf, const ConvFuse& fz, int n, int h, int wdt, int cout,
+                    int relu) {
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const unsigned tx = (unsigned)(wdt / CL_TX);
+    if (cout % 64 == 0)
+        hipLaunchKernelGGL((k_conv3x3_lds<4, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 64),
+                           dim3(256), 0, st, ci, w, bias, out, outf, fz, n, h, wdt, cout, relu);
+    else if (h % 8 == 0)
+        hipLaunchKernelGGL((k_conv3x3_lds<2, 2>), dim3((unsigned)n * (h / 8) * tx, cout / 32),
+                           dim3(256), 0, st, ci, w, bias, out, outf, fz, n, h, wdt, cout, relu);
+    else
+        hipLaunchKernelGGL((k_conv3x3_lds<2, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 32),
+                           dim3(256), 0, st, ci, w, bias, out, outf, fz, n, h, wdt, cout, relu);
+    CAGEO_HIP_TRY(hipGetLastError());
+    return CAGEO_OK;
+}
+
 }  // namespace cageo
 
 // ---- C ABI ------------------------------------------------------------------
@@ -612,24 +735,8 @@ int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const u
     if (c1 <= 0 || c1 % 32 || c2 < 0 || c2 % 32 || (c2 && !in2) || cout % 16) return CAGEO_ERR_ARG;
     ConvIn ci{in1, in2, c1, c2, up1};
     const long long M = (long long)n * h * wdt;
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (wdt % CL_TX == 0 && h % 4 == 0 && cout % 32 == 0) {
-        // operands staged in LDS: workgroups of 4 x 64 pixels x 64 output
-        // channels, or 8 x 64 pixels x 32 output channels (4 x 64 when the
-        // height is not a multiple of 8)
-        const unsigned tx = (unsigned)(wdt / CL_TX);
-        if (cout % 64 == 0)
-            hipLaunchKernelGGL((k_conv3x3_lds<4, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 64),
-                               dim3(256), 0, st, ci, w, bias, out, outf, n, h, wdt, cout, relu);
-        else if (h % 8 == 0)
-            hipLaunchKernelGGL((k_conv3x3_lds<2, 2>), dim3((unsigned)n * (h / 8) * tx, cout / 32),
-                               dim3(256), 0, st, ci, w, bias, out, outf, n, h, wdt, cout, relu);
-        else
-            hipLaunchKernelGGL((k_conv3x3_lds<2, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 32),
-                               dim3(256), 0, st, ci, w, bias, out, outf, n, h, wdt, cout, relu);
-        CAGEO_HIP_TRY(hipGetLastError());
-        return CAGEO_OK;
-    }
+    if (lds_conv_applies(h, wdt, cout))
+        return launch_conv_lds(stream, ci, w, bias, out, outf, ConvFuse{}, n, h, wdt, cout, relu);
     // output channel tile: 64 (NT 4), 32 (NT 2) or 16 (NT 1); 4 pixel tiles
     // per wave (256 pixels per workgroup; 8 lowered occupancy and time)
     constexpr int MT = 4;
@@ -647,6 +754,26 @@ int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const u
                            wdt, cout, relu);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
+}
+
+int cageo_nn_conv3x3_fusable(int h, int wdt, int cout, int head) {
+    if (h <= 0 || wdt <= 0 || cout <= 0 || !lds_conv_applies(h, wdt, cout)) return 0;
+    return head ? (lds_conv_block(cout) == cout ? 1 : 0) : 1;
+}
+
+int cageo_nn_conv3x3_fused(void* stream, const uint16_t* in1, int c1, int up1,
+                           const uint16_t* in2, int c2, const uint16_t* w, const float* bias,
+                           uint16_t* out, uint16_t* pool, const float* head_w,
+                           const float* head_b, float* prob, int head_sigmoid, int n, int h,
+                           int wdt, int cout, int relu) {
+    if (!in1 || !w || n <= 0 || h <= 0 || wdt <= 0) return CAGEO_ERR_ARG;
+    if (c1 <= 0 || c1 % 32 || c2 < 0 || c2 % 32 || (c2 && !in2)) return CAGEO_ERR_ARG;
+    if (!out && !pool && !prob) return CAGEO_ERR_ARG;
+    if (prob && (out || pool || !head_w || !head_b)) return CAGEO_ERR_ARG;
+    if (!cageo_nn_conv3x3_fusable(h, wdt, cout, prob ? 1 : 0)) return CAGEO_ERR_ARG;
+    ConvIn ci{in1, in2, c1, c2, up1};
+    ConvFuse fz{pool, head_w, head_b, prob, head_sigmoid};
+    return launch_conv_lds(stream, ci, w, bias, out, nullptr, fz, n, h, wdt, cout, relu);
 }
 
 int cageo_nn_conv3x3_wgrad(void* stream, const uint16_t* in1, int c1, int up1,

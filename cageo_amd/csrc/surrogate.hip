@@ -27,6 +27,11 @@ int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const u
 int cageo_nn_avgpool2(void* stream, const uint16_t* in, uint16_t* out, int n, int h, int w, int c);
 int cageo_nn_head(void* stream, const uint16_t* act, const float* w, const float* b, float* logit,
                   long long p, int c, int sigmoid);
+int cageo_nn_conv3x3_fusable(int h, int w, int cout, int head);
+int cageo_nn_conv3x3_fused(void* stream, const uint16_t* in1, int c1, int up1, const uint16_t* in2,
+                           int c2, const uint16_t* w, const float* bias, uint16_t* out,
+                           uint16_t* pool, const float* head_w, const float* head_b, float* prob,
+                           int head_sigmoid, int n, int h, int wdt, int cout, int relu);
 }
 
 namespace cageo {
@@ -117,6 +122,112 @@ __global__ void k_make_inputs(const float* __restrict__ dem, const float* __rest
         o[1] = make_uint4(0, 0, 0, 0);
         o[2] = make_uint4(0, 0, 0, 0);
         o[3] = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// The first convolution (enc1a, base 32) straight from the DEM and the
+// stream raster, for cageo_surrogate_predict: the input planes of
+// k_make_inputs are formed in LDS for a 4-row x 64-pixel tile and its halo
+// and never stored, and the convolution reads only the 4 planes it uses:
+// K = 9 taps x 4 channels in two 32-wide MFMA steps (taps 0-7, then tap 8
+// and zeros) instead of 9 taps x 32 padded channels.  Same products, fp32
+// accumulation in another order than the padded layer's.  Output
+// [hp][wp][32] bf16 after bias and ReLU, staged in LDS and written with
+// 16-byte stores.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int FL_TX = 64, FL_TY = 4, FL_AX = FL_TX + 2, FL_AN = (FL_TY + 2) * FL_AX;
+constexpr int FL_PS = 40;  // staged pixel stride (bf16): 32 channels + 16 bytes
+
+__global__ __launch_bounds__(256) void k_first_layer(
+    const float* __restrict__ dem, const float* __restrict__ stream, int W, int H, int wp, int hp,
+    float nodata_dem, float nodata_stream, float c2, float c3, const float* __restrict__ stats,
+    const uint16_t* __restrict__ w, const float* __restrict__ bias, uint16_t* __restrict__ out) {
+    __shared__ uint2 pl[FL_AN];  // the 4 bf16 planes of each halo-tile pixel
+    __shared__ __align__(16) uint16_t so[FL_TY * FL_TX * FL_PS];
+    const int x0 = blockIdx.x * FL_TX, y0 = blockIdx.y * FL_TY;
+    const float mu = stats[0], sd = stats[1];
+    const uint32_t cc = (uint32_t)to_bf16(c2) | ((uint32_t)to_bf16(c3) << 16);
+    {
+        // all loads first (a pixel outside the raster loads cell 0), then
+        // the planes; zero outside the raster, as k_make_inputs writes them
+        constexpr int IT = (FL_AN + 255) / 256;
+        float dv[IT], sv[IT];
+        bool in[IT];
+#pragma unroll
+        for (int j = 0; j < IT; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            const int yy = y0 - 1 + i / FL_AX, xx = x0 - 1 + i % FL_AX;
+            in[j] = i < FL_AN && yy >= 0 && xx >= 0 && yy < H && xx < W;
+            const size_t k = in[j] ? (size_t)yy * W + xx : 0;
+            dv[j] = dem[k];
+            sv[j] = stream[k];
+        }
+#pragma unroll
+        for (int j = 0; j < IT; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (i >= FL_AN) continue;
+            uint2 v = make_uint2(0u, 0u);
+            if (in[j]) {
+                const float z = dv[j] != nodata_dem ? (dv[j] - mu) / sd : 0.0f;
+                const float sm = sv[j] != nodata_stream ? 1.0f : 0.0f;
+                v = make_uint2((uint32_t)to_bf16(z) | ((uint32_t)to_bf16(sm) << 16), cc);
+            }
+            pl[i] = v;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kb = lane >> 4;
+    // B: lane (column row, K block kb) holds K 8kb..8kb+7 = taps 2kb, 2kb+1 x
+    // channels 0..3 (first step), tap 8 x channels 0..3 for kb 0 (second)
+    bf16x8 b0[2], b1[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int co = 16 * q + row;
+        const uint2 t0 = *reinterpret_cast<const uint2*>(w + ((2 * kb) * 32 + co) * 32);
+        const uint2 t1 = *reinterpret_cast<const uint2*>(w + ((2 * kb + 1) * 32 + co) * 32);
+        const uint2 t8 = kb == 0 ? *reinterpret_cast<const uint2*>(w + (8 * 32 + co) * 32)
+                                 : make_uint2(0u, 0u);
+        b0[q] = __builtin_bit_cast(bf16x8, make_uint4(t0.x, t0.y, t1.x, t1.y));
+        b1[q] = __builtin_bit_cast(bf16x8, make_uint4(t8.x, t8.y, 0u, 0u));
+    }
+    __syncthreads();
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int px = 16 * m + row;  // pixel of the wave's row; halo-shifted taps below
+        const int ta = 2 * kb, tb = 2 * kb + 1;
+        const uint2 a0 = pl[(wave + ta / 3) * FL_AX + px + ta % 3];
+        const uint2 a1 = pl[(wave + tb / 3) * FL_AX + px + tb % 3];
+        const uint2 a8 = kb == 0 ? pl[(wave + 2) * FL_AX + px + 2] : make_uint2(0u, 0u);
+        const bf16x8 A0 = __builtin_bit_cast(bf16x8, make_uint4(a0.x, a0.y, a1.x, a1.y));
+        const bf16x8 A1 = __builtin_bit_cast(bf16x8, make_uint4(a8.x, a8.y, 0u, 0u));
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, b0[q], f32x4{0.f, 0.f, 0.f, 0.f},
+                                                                0, 0, 0);
+            acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, b1[q], acc[m][q], 0, 0, 0);
+        }
+    }
+    // accumulator row 4 kb + r = pixel, column = channel 16 q + row
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int c = 16 * q + row;
+        const float bc = bias[c];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                so[(wave * FL_TX + 16 * m + 4 * kb + r) * FL_PS + c] =
+                    to_bf16(fmaxf(acc[m][q][r] + bc, 0.0f));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < FL_TY * FL_TX * 4; i += 256) {
+        const int px = i >> 2, part = i & 3;
+        const int y = y0 + px / FL_TX, x = x0 + px % FL_TX;
+        if (x < wp && y < hp)
+            *reinterpret_cast<uint4*>(out + ((size_t)y * wp + x) * 32 + 8 * part) =
+                *reinterpret_cast<const uint4*>(so + px * FL_PS + 8 * part);
     }
 }
 
@@ -214,8 +325,48 @@ int conv(cageo_surrogate* s, const char* name, const uint16_t* in1, int c1, int 
                             static_cast<const float*>(bi->second), out, nullptr, 1, h, w, cout, 1);
 }
 
-// Forward pass of x (bf16 [hp][wp][IN_PAD] = act[0]) into prob_pad.
-int forward(cageo_surrogate* s) {
+// A level's second convolution, its output also 2x2-average pooled: in one
+// launch where the staged convolution applies (the same bits as the pool
+// kernel after it), else the two kernels.
+int conv_pool(cageo_surrogate* s, const char* name, const uint16_t* in1, int c1, uint16_t* out,
+              uint16_t* pooled, int h, int w) {
+    const std::string n(name);
+    auto wi = s->dev.find(n + ".w"), bi = s->dev.find(n + ".b");
+    if (wi == s->dev.end() || bi == s->dev.end()) return CAGEO_ERR_STATE;
+    const int cout = s->shape[n + ".b"][0];
+    if (cageo_nn_conv3x3_fusable(h, w, cout, 0))
+        return cageo_nn_conv3x3_fused(s->stream, in1, c1, 0, nullptr, 0,
+                                      static_cast<const uint16_t*>(wi->second),
+                                      static_cast<const float*>(bi->second), out, pooled, nullptr,
+                                      nullptr, nullptr, 0, 1, h, w, cout, 1);
+    CAGEO_TRY(conv(s, name, in1, c1, 0, nullptr, 0, out, h, w));
+    return cageo_nn_avgpool2(s->stream, out, pooled, 1, h, w, cout);
+}
+
+// The last convolution and the sigmoid head into prob_pad: one launch where
+// the staged convolution holds all of its channels in a workgroup (its
+// output then never reaches memory), else the convolution into `tmp` and
+// the head kernel.
+int conv_head(cageo_surrogate* s, const char* name, const uint16_t* in1, int c1, uint16_t* tmp,
+              int h, int w) {
+    const std::string n(name);
+    auto wi = s->dev.find(n + ".w"), bi = s->dev.find(n + ".b");
+    if (wi == s->dev.end() || bi == s->dev.end()) return CAGEO_ERR_STATE;
+    const int cout = s->shape[n + ".b"][0];
+    const float* hw = static_cast<const float*>(s->dev["head.w"]);
+    const float* hb = static_cast<const float*>(s->dev["head.b"]);
+    if (cageo_nn_conv3x3_fusable(h, w, cout, 1))
+        return cageo_nn_conv3x3_fused(s->stream, in1, c1, 0, nullptr, 0,
+                                      static_cast<const uint16_t*>(wi->second),
+                                      static_cast<const float*>(bi->second), nullptr, nullptr, hw,
+                                      hb, s->prob_pad, 1, 1, h, w, cout, 1);
+    CAGEO_TRY(conv(s, name, in1, c1, 0, nullptr, 0, tmp, h, w));
+    return cageo_nn_head(s->stream, tmp, hw, hb, s->prob_pad, (long long)h * w, cout, 1);
+}
+
+// Forward pass of x (bf16 [hp][wp][IN_PAD] = act[0]) into prob_pad; with
+// `first_done`, enc1a's output is already in its buffer (k_first_layer).
+int forward(cageo_surrogate* s, bool first_done = false) {
     const int b = s->base, hp = s->hp, wp = s->wp;
     auto T = [&](int lv) { return s->act[1 + 4 * lv + 0]; };  // temporary
     auto E = [&](int lv) { return s->act[1 + 4 * lv + 1]; };  // encoder output (skip)
@@ -225,15 +376,12 @@ int forward(cageo_surrogate* s) {
     auto Wd = [&](int lv) { return wp >> lv; };
     const uint16_t* x = s->act[0];
     // encoder
-    CAGEO_TRY(conv(s, "enc1a", x, IN_PAD, 0, nullptr, 0, T(0), H(0), Wd(0)));
-    CAGEO_TRY(conv(s, "enc1b", T(0), b, 0, nullptr, 0, E(0), H(0), Wd(0)));
-    CAGEO_TRY(cageo_nn_avgpool2(s->stream, E(0), P(0), 1, H(0), Wd(0), b));
+    if (!first_done) CAGEO_TRY(conv(s, "enc1a", x, IN_PAD, 0, nullptr, 0, T(0), H(0), Wd(0)));
+    CAGEO_TRY(conv_pool(s, "enc1b", T(0), b, E(0), P(0), H(0), Wd(0)));
     CAGEO_TRY(conv(s, "enc2a", P(0), b, 0, nullptr, 0, T(1), H(1), Wd(1)));
-    CAGEO_TRY(conv(s, "enc2b", T(1), 2 * b, 0, nullptr, 0, E(1), H(1), Wd(1)));
-    CAGEO_TRY(cageo_nn_avgpool2(s->stream, E(1), P(1), 1, H(1), Wd(1), 2 * b));
+    CAGEO_TRY(conv_pool(s, "enc2b", T(1), 2 * b, E(1), P(1), H(1), Wd(1)));
     CAGEO_TRY(conv(s, "enc3a", P(1), 2 * b, 0, nullptr, 0, T(2), H(2), Wd(2)));
-    CAGEO_TRY(conv(s, "enc3b", T(2), 4 * b, 0, nullptr, 0, E(2), H(2), Wd(2)));
-    CAGEO_TRY(cageo_nn_avgpool2(s->stream, E(2), P(2), 1, H(2), Wd(2), 4 * b));
+    CAGEO_TRY(conv_pool(s, "enc3b", T(2), 4 * b, E(2), P(2), H(2), Wd(2)));
     CAGEO_TRY(conv(s, "mida", P(2), 4 * b, 0, nullptr, 0, T(3), H(3), Wd(3)));
     CAGEO_TRY(conv(s, "midb", T(3), 8 * b, 0, nullptr, 0, E(3), H(3), Wd(3)));
     // decoder: the coarser tensor is upsampled inside the convolution's loads
@@ -242,10 +390,7 @@ int forward(cageo_surrogate* s) {
     CAGEO_TRY(conv(s, "dec2a", D(2), 4 * b, 1, E(1), 2 * b, T(1), H(1), Wd(1)));
     CAGEO_TRY(conv(s, "dec2b", T(1), 2 * b, 0, nullptr, 0, D(1), H(1), Wd(1)));
     CAGEO_TRY(conv(s, "dec1a", D(1), 2 * b, 1, E(0), b, T(0), H(0), Wd(0)));
-    CAGEO_TRY(conv(s, "dec1b", T(0), b, 0, nullptr, 0, D(0), H(0), Wd(0)));
-    return cageo_nn_head(s->stream, D(0), static_cast<const float*>(s->dev["head.w"]),
-                         static_cast<const float*>(s->dev["head.b"]), s->prob_pad,
-                         (long long)hp * wp, b, 1);
+    return conv_head(s, "dec1b", T(0), b, D(0), H(0), Wd(0));
 }
 
 bool read_exact(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
@@ -370,11 +515,25 @@ int cageo_surrogate_predict(cageo_surrogate* s, const float* d_dem, const float*
     hipLaunchKernelGGL(cageo::k_dem_stats, dim3(1), dim3(64), 0, s->stream, s->part,
                        cageo::RED_BLOCKS, s->stats);
     CAGEO_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(cageo::k_make_inputs, dim3(2048), dim3(256), 0, s->stream, d_dem, d_stream,
-                       W, H, wp, hp, nodata_dem, nodata_stream, rmse / 3.0f,
-                       practical_range / 200.0f, s->stats, s->act[0]);
-    CAGEO_HIP_TRY(hipGetLastError());
-    CAGEO_TRY(cageo::forward(s));
+    const std::vector<int>& e1 = s->shape["enc1a.w"];
+    if (s->base == 32 && e1[1] == 32 && e1[2] == cageo::IN_PAD) {
+        // enc1a from the rasters (its output: the first level's temporary)
+        hipLaunchKernelGGL(cageo::k_first_layer,
+                           dim3((unsigned)((wp + cageo::FL_TX - 1) / cageo::FL_TX),
+                                (unsigned)(hp / cageo::FL_TY)),
+                           dim3(256), 0, s->stream, d_dem, d_stream, W, H, wp, hp, nodata_dem,
+                           nodata_stream, rmse / 3.0f, practical_range / 200.0f, s->stats,
+                           static_cast<const uint16_t*>(s->dev["enc1a.w"]),
+                           static_cast<const float*>(s->dev["enc1a.b"]), s->act[1]);
+        CAGEO_HIP_TRY(hipGetLastError());
+        CAGEO_TRY(cageo::forward(s, true));
+    } else {
+        hipLaunchKernelGGL(cageo::k_make_inputs, dim3(2048), dim3(256), 0, s->stream, d_dem,
+                           d_stream, W, H, wp, hp, nodata_dem, nodata_stream, rmse / 3.0f,
+                           practical_range / 200.0f, s->stats, s->act[0]);
+        CAGEO_HIP_TRY(hipGetLastError());
+        CAGEO_TRY(cageo::forward(s));
+    }
     hipLaunchKernelGGL(cageo::k_crop, dim3(2048), dim3(256), 0, s->stream, s->prob_pad, wp,
                        d_prob, W, H);
     CAGEO_HIP_TRY(hipGetLastError());

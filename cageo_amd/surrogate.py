@@ -54,6 +54,9 @@ def _nn():
         vp, i, f, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
         sigs = {
             "cageo_nn_conv3x3": [vp, vp, i, i, vp, i, vp, vp, vp, vp, i, i, i, i, i],
+            "cageo_nn_conv3x3_fused": [vp, vp, i, i, vp, i, vp, vp, vp, vp, vp, vp, vp, i,
+                                       i, i, i, i, i],
+            "cageo_nn_conv3x3_fusable": [i, i, i, i],
             "cageo_nn_conv3x3_wgrad": [vp, vp, i, i, vp, i, vp, vp, i, i, i, i],
             "cageo_nn_flip_transpose": [vp, vp, vp, i, i],
             "cageo_nn_avgpool2": [vp, vp, vp, i, i, i, i],
@@ -97,7 +100,8 @@ LAYER_LEVEL = {"enc1a": 0, "enc1b": 0, "enc2a": 1, "enc2b": 1, "enc3a": 2, "enc3
 
 def forward_flops(h, w, base=32, in_channels=IN_PAD):
     """Multiply-add FLOPs (2 per MAC) of one forward pass over an h x w input,
-    as the kernels execute it (the first layer reads all `in_channels`)."""
+    the first layer over `in_channels` input channels (IN_PAD as the padded
+    layer executes it, IN_CHANNELS for the products that count)."""
     total = 0
     for name, c1, _, c2, co in layer_spec(base):
         if name == "enc1a":

@@ -299,6 +299,23 @@ int cageo_context_configure(cageo_context* ctx, int persistent_workgroups, int c
 int cageo_nn_conv3x3(void* stream, const uint16_t* in1, int c1, int up1, const uint16_t* in2,
                      int c2, const uint16_t* w, const float* bias, uint16_t* out, float* outf,
                      int n, int h, int w_, int cout, int relu);
+/* The forward convolution (bf16 output, no fp32 output) with consumers of
+ * its output fused into the launch, each giving the same bits as its own
+ * kernel run after the convolution:
+ *   pool (or null): cageo_nn_avgpool2 of the output, [n][h/2][w/2][cout];
+ *   head_w, head_b, prob (or null): cageo_nn_head of the output into prob
+ *     ([n][h][w] fp32, the sigmoid when head_sigmoid); out and pool must
+ *     then be null (the output never reaches memory).
+ * out may be null when the output itself is not needed.  Only where
+ * cageo_nn_conv3x3_fusable(h, w_, cout, head) returns 1 (w_ % 64 == 0,
+ * h % 4 == 0, cout % 32 == 0; with the head, cout 32 or 64: one workgroup
+ * holds all of a pixel's channels); CAGEO_ERR_ARG otherwise.  No reference
+ * counterpart (surrogate inference). */
+int cageo_nn_conv3x3_fused(void* stream, const uint16_t* in1, int c1, int up1, const uint16_t* in2,
+                           int c2, const uint16_t* w, const float* bias, uint16_t* out,
+                           uint16_t* pool, const float* head_w, const float* head_b, float* prob,
+                           int head_sigmoid, int n, int h, int w_, int cout, int relu);
+int cageo_nn_conv3x3_fusable(int h, int w_, int cout, int head);
 /* dw[tap][co][ci] += sum_p g[p][co] * in[p + offset(tap)][ci] (fp32 dw). */
 int cageo_nn_conv3x3_wgrad(void* stream, const uint16_t* in1, int c1, int up1,
                            const uint16_t* in2, int c2, const uint16_t* g, float* dw, int n, int h,

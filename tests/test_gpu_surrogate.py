@@ -113,6 +113,62 @@ def test_conv_forward_and_gradients(env, case):
     assert np.abs(dx.cpu().numpy() - dx_ref).max() <= 1e-5 * np.abs(dx_ref).max() * cout ** 0.5
 
 
+FUSED_CASES = [  # n, h, w, c1, up1, c2, cout, pool, head
+    (1, 8, 128, 32, 0, 0, 32, True, False),    # 8-row workgroups, 32 channels
+    (1, 8, 128, 32, 0, 0, 32, False, True),
+    (2, 4, 64, 64, 1, 32, 64, True, False),    # 4-row workgroups, 64 channels
+    (1, 4, 64, 32, 0, 0, 64, False, True),
+    (1, 12, 64, 32, 0, 0, 32, True, False),    # 4-row workgroups, 32 channels
+    (1, 12, 64, 32, 0, 0, 32, False, True),
+    (1, 8, 64, 32, 0, 0, 96, True, False),     # two channel blocks: pool only
+]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES)
+def test_fused_consumers_equal_their_own_kernels(env, case):
+    """cageo_nn_conv3x3_fused: the output, its 2x2 average pool and the 1x1
+    head computed in the convolution's launch are the bits of the
+    convolution followed by cageo_nn_avgpool2 / cageo_nn_head."""
+    torch, sg, L = env
+    n, h, w, c1, up1, c2, cout, pool, head = case
+    announce("nn fused conv", case)
+    r = np.random.default_rng(sum(case))
+    h1, w1 = ((h + 1) // 2, (w + 1) // 2) if up1 else (h, w)
+    d1 = dev_bits(torch, r.normal(size=(n, h1, w1, c1)))
+    d2 = dev_bits(torch, r.normal(size=(n, h, w, c2))) if c2 else None
+    dw_ = dev_bits(torch, r.normal(size=(9, cout, c1 + c2)) * 0.1)
+    db = torch.from_numpy(r.normal(size=cout).astype(np.float32)).cuda()
+    hw = torch.from_numpy(r.normal(size=cout).astype(np.float32)).cuda()
+    hb = torch.from_numpy(r.normal(size=1).astype(np.float32)).cuda()
+    p = sg._p
+    assert L.cageo_nn_conv3x3_fusable(h, w, cout, int(head)) == 1
+    assert L.cageo_nn_conv3x3_fusable(h, w + 8, cout, 0) == 0
+    ref = torch.empty((n, h, w, cout), dtype=torch.int16, device="cuda")
+    assert L.cageo_nn_conv3x3(None, p(d1), c1, up1, p(d2), c2, p(dw_), p(db), p(ref), None, n, h,
+                              w, cout, 1) == 0
+    if pool:
+        ref_pool = torch.empty((n, h // 2, w // 2, cout), dtype=torch.int16, device="cuda")
+        assert L.cageo_nn_avgpool2(None, p(ref), p(ref_pool), n, h, w, cout) == 0
+        out = torch.full((n, h, w, cout), -1, dtype=torch.int16, device="cuda")
+        got = torch.full((n, h // 2, w // 2, cout), -1, dtype=torch.int16, device="cuda")
+        assert L.cageo_nn_conv3x3_fused(None, p(d1), c1, up1, p(d2), c2, p(dw_), p(db), p(out),
+                                        p(got), None, None, None, 0, n, h, w, cout, 1) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref) and torch.equal(got, ref_pool)
+    if head:
+        ref_prob = torch.empty((n, h, w), dtype=torch.float32, device="cuda")
+        assert L.cageo_nn_head(None, p(ref), p(hw), p(hb), p(ref_prob), n * h * w, cout, 1) == 0
+        got = torch.full((n, h, w), -1.0, dtype=torch.float32, device="cuda")
+        assert L.cageo_nn_conv3x3_fused(None, p(d1), c1, up1, p(d2), c2, p(dw_), p(db), None,
+                                        None, p(hw), p(hb), p(got), 1, n, h, w, cout, 1) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref_prob)
+    else:
+        # a head needs the whole channel count in one workgroup (32 or 64)
+        expect = 1 if cout in (32, 64) else 0
+        assert L.cageo_nn_conv3x3_fusable(h, w, cout, 1) == expect
+
+
 def test_network_gradients_match_oracle(env):
     """Whole U-Net (base 32) at 32x32 against the float64 oracle with the same
     (bf16-rounded) weights and inputs, twice:
