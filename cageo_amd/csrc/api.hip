@@ -43,7 +43,7 @@ int alloc(T** p, size_t count) {
 }
 
 void free_all(Context* c) {
-    void* ptrs[] = {c->d_mark, c->d_mark_pf, c->d_topen, c->d_tcost, c->d_rows, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_filter,
+    void* ptrs[] = {c->d_mark, c->d_mark_pf, c->d_topen, c->d_tcost, c->d_rows, c->d_work3, c->d_ctrl, c->d_bar, c->d_scal, c->d_bmm, c->d_filter,
                     c->d_sflags, c->d_slabel, c->d_off, c->d_factor, c->d_a, c->d_b, c->d_z,
                     c->d_w, c->d_st, c->d_fd, c->d_dist, c->d_lab, c->d_ptr, c->d_bent,
                     c->d_lr_col, c->d_lr_row, c->d_lr_planes};
@@ -66,8 +66,7 @@ int realization_pre(Context* c, hipEvent_t* evs) {
     float* e = nullptr;
     CAGEO_TRY(launch_convolve_to(c, c->d_a, c->d_b, &e));
     if (evs) CAGEO_HIP_TRY(hipEventRecord(evs[2], c->stream));
-    CAGEO_TRY(reset_minmax(c));
-    CAGEO_TRY(launch_realistic_burn_quant(c, c->dem, e, c->d_z));
+    CAGEO_TRY(launch_realistic_burn_quant(c, c->dem, e, c->d_z));  // min/max words included
     if (evs) CAGEO_HIP_TRY(hipEventRecord(evs[3], c->stream));
     return CAGEO_OK;
 }
@@ -244,6 +243,7 @@ int cageo_context_create(cageo_context** out, int W, int H, void* stream) {
     A(cageo::alloc(&c->d_ctrl, cageo::CTL_WORDS));
     A(cageo::alloc(&c->d_bar, cageo::BAR_WORDS));
     A(cageo::alloc(&c->d_scal, cageo::SC_NSLOTS));
+    A(cageo::alloc(&c->d_bmm, 3 * cageo::burn_blocks(W, H)));
     A(cageo::alloc(&c->d_sflags, n));
     A(cageo::alloc(&c->d_slabel, n));
     const int maxoff = (2 * CAGEO_MAX_BEVEL_RADIUS + 1) * (2 * CAGEO_MAX_BEVEL_RADIUS + 1);

@@ -81,6 +81,7 @@ struct Context {
 
     // scalars
     uint32_t* d_scal = nullptr;
+    uint32_t* d_bmm = nullptr;   // per-block (min, max, bad) of k_realistic_burn_quant
     uint32_t* h_scal = nullptr;  // pinned mirror
 
     // filter
@@ -402,6 +403,7 @@ int prepare_offsets(Context* c, int radius);
 int launch_stream_flags(Context* c, const float* stream, float nodata_stream, uint8_t* sflags,
                         int32_t* slabel);
 int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq);
+size_t burn_blocks(int W, int H);
 int pit_fill_quantized(Context* c, float* zq, float* w, float nodata, bool stats_ready);
 int check_status(Context* c);
 int reset_minmax(Context* c);

@@ -164,12 +164,13 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     if (dirty == 0) return;  // uniform
     const int tx = b.lb % a.ntx, ty = b.lb / a.ntx;
     const int x0 = tx * TILE, y0 = ty * TILE;
-    // halo tile in one batch of independent loads (each thread's loads are
-    // in flight together)
+    // halo tile in two batches of independent loads (each batch's loads are
+    // in flight together; one batch would take the registers of the fourth
+    // workgroup a CU can hold)
     {
         constexpr int N = HALO_W * HALO_W;
         constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
-        constexpr int CHUNK = ITER;
+        constexpr int CHUNK = (ITER + 1) / 2;
 #pragma unroll
         for (int c0 = 0; c0 < ITER; c0 += CHUNK) {
             int2 v[CHUNK];
@@ -383,10 +384,11 @@ __global__ __launch_bounds__(256) void k_fr_d8(FrArgs a) {
         a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
 }
 
-// The flat passes and the direction pick, after k_fr_d8.
+// The flat passes, after k_fr_d8 (k_fr_pick follows).
 // MULTI: a table of rasters (else the one raster of the arguments).
+// Four waves a SIMD: the LDS of a flat tile allows four workgroups per CU.
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_fr_persistent(FrArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_fr_persistent(FrArgs a) {
     __shared__ FrShared S;
     if (grid_aborted(a.gb)) return;
     uint32_t epoch = ld(&a.ctrl[CTL_EPOCH_FR]);

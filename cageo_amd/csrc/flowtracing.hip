@@ -85,11 +85,12 @@ __device__ __forceinline__ void trace_tile(uint64_t* L, const int8_t* __restrict
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     const int gx = x0 + lx;
 
-    // Two batches of rows; in each, the cells' flags and directions are
-    // loaded together, then their targets' flags and labels (no load waits
-    // on a test; a target outside the grid reads the cell itself, unused).
+    // The strip's cells' flags and directions are loaded together, then
+    // their targets' flags and labels: two dependent rounds of loads for the
+    // tile (no load waits on a test; a target outside the grid reads the
+    // cell itself, unused).
     uint32_t local = 0;
-    constexpr int HALF = STRIP / 2;
+    constexpr int HALF = STRIP;  // rows per round of loads: all of the strip
 #pragma unroll
     for (int k0 = 0; k0 < STRIP; k0 += HALF) {
         uint8_t sfc[HALF], sfn[HALF];
@@ -268,24 +269,6 @@ __device__ inline TrBlock tr_block(const TrArgs& a, int g) {
     return b;
 }
 
-// A unit of UNIT_ROWS x 64 cells: the raster, the wave's column x, the first
-// row y0 and how many of the unit's rows lie in the raster (0 when the
-// wave's column is outside it).
-struct TrUnit {
-    TrRaster R;
-    int x, y0, rows;
-};
-template <bool MULTI>
-__device__ inline TrUnit tr_unit(const TrArgs& a, long long u) {
-    const TrBlock b = tr_block<MULTI>(a, (int)(u / UNITS_PER_BLOCK));
-    TrUnit t;
-    t.R = b.R;
-    t.x = (b.lb % b.R.ntx) * TILE + (threadIdx.x & 63);
-    t.y0 = (b.lb / b.R.ntx) * TILE + (int)(u % UNITS_PER_BLOCK) * UNIT_ROWS;
-    t.rows = t.x < b.R.W ? min(UNIT_ROWS, b.R.H - t.y0) : 0;
-    return t;
-}
-
 // Final label of cell 0 of a raster (between steps 2 and 3 of the launch).
 __device__ inline int32_t label_of_cell0(const TrRaster& R) {
     const int32_t v = R.lab[0];
@@ -348,47 +331,48 @@ __global__ __launch_bounds__(256) void k_tr_jump(TrArgs a) {
 
 // Exit cells take their slot's label; performConnectFlowPath maps a label 0
 // to label[0] when that is negative.  An exit whose chain is still moving
-// (a cycle through tiles) or ends stuck stays 0.  Each wave takes units of
-// UNIT_ROWS x 64 cells of the workgroup's block, each unit's loads issued
-// before its stores.
+// (a cycle through tiles) or ends stuck stays 0.  One workgroup per block,
+// a wave per strip of STRIP rows (one lane per column): the strip's labels,
+// then its exit cells' entries, then their slots' entries, each round of
+// loads issued whole before the next.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_tr_exits(TrArgs a) {
-    for (int k0 = (int)(threadIdx.x >> 6); k0 < UNITS_PER_BLOCK; k0 += TILE_THREADS / 64) {
-        const TrUnit t = tr_unit<MULTI>(a, (long long)blockIdx.x * UNITS_PER_BLOCK + k0);
-        const int32_t l0 = label_of_cell0(t.R);
-        int32_t v[UNIT_ROWS];
+    const TrBlock b = tr_block<MULTI>(a, blockIdx.x);
+    const TrRaster& R = b.R;
+    const int x = (b.lb % R.ntx) * TILE + (threadIdx.x & 63);
+    const int y0 = (b.lb / R.ntx) * TILE + (int)(threadIdx.x >> 6) * STRIP;
+    const int rows = x < R.W ? min(STRIP, R.H - y0) : 0;
+    const int32_t l0 = label_of_cell0(R);
+    int32_t v[STRIP];
 #pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k)
-            v[k] = k < t.rows ? t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] : 1;
-        uint64_t e[UNIT_ROWS];
+    for (int k = 0; k < STRIP; ++k) v[k] = k < rows ? R.lab[(size_t)(y0 + k) * R.W + x] : 1;
+    uint64_t e[STRIP];
 #pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k)
-            e[k] = v[k] == EXIT_MARK ? t.R.ent[(size_t)(t.y0 + k) * t.R.W + t.x] : 0;
-        uint64_t f[UNIT_ROWS];
+    for (int k = 0; k < STRIP; ++k)
+        e[k] = v[k] == EXIT_MARK ? R.ent[(size_t)(y0 + k) * R.W + x] : 0;
+    uint64_t f[STRIP];
 #pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k)
-            f[k] = v[k] == EXIT_MARK ? ld64(&t.R.bent[value_of(e[k])]) : 0;
-        uint32_t longp = 0;
+    for (int k = 0; k < STRIP; ++k) f[k] = v[k] == EXIT_MARK ? ld64(&R.bent[value_of(e[k])]) : 0;
+    uint32_t longp = 0;
 #pragma unroll
-        for (int k = 0; k < UNIT_ROWS; ++k) {
-            if (k >= t.rows) continue;
-            int32_t w = v[k];
-            if (w == EXIT_MARK) {
-                w = 0;
-                if (kind_of(f[k]) == K_RESOLVED) {
-                    w = value_of(f[k]);
-                    if (steps_of(e[k]) + steps_of(f[k]) > 100001u) ++longp;  // past the walk guard
-                }
+    for (int k = 0; k < STRIP; ++k) {
+        if (k >= rows) continue;
+        int32_t w = v[k];
+        if (w == EXIT_MARK) {
+            w = 0;
+            if (kind_of(f[k]) == K_RESOLVED) {
+                w = value_of(f[k]);
+                if (steps_of(e[k]) + steps_of(f[k]) > 100001u) ++longp;  // past the walk guard
             }
-            if (w == 0 && l0 < 0) w = l0;
-            if (w != v[k]) t.R.lab[(size_t)(t.y0 + k) * t.R.W + t.x] = w;
         }
-        if (__any(longp)) {
-            for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
-            if ((threadIdx.x & 63) == 0) {
-                atomicAdd(&t.R.scal[SC_LONGPATH], longp);
-                atomicAdd(&t.R.scal[SC_LONGRUN], longp);
-            }
+        if (w == 0 && l0 < 0) w = l0;
+        if (w != v[k]) R.lab[(size_t)(y0 + k) * R.W + x] = w;
+    }
+    if (__any(longp)) {
+        for (int off = 32; off > 0; off >>= 1) longp += (uint32_t)__shfl_xor((int)longp, off);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&R.scal[SC_LONGPATH], longp);
+            atomicAdd(&R.scal[SC_LONGRUN], longp);
         }
     }
 }

@@ -237,6 +237,70 @@ __device__ inline void block_minmax(uint32_t mn, uint32_t mx, uint32_t bad, uint
     }
 }
 
+// The block's (min, max, bad) into part[0..2] (thread 0; plain stores, no
+// wait on the global words): reduced by k_minmax_finish.
+__device__ inline void block_minmax_store(uint32_t mn, uint32_t mx, uint32_t bad, uint32_t* part) {
+    __shared__ uint32_t s_mn[4], s_mx[4], s_bad[4];
+    int a = (int)mn, b = (int)mx;
+    CAGEO_SCAN_UP(dpp_i, a, b, (int)0xffffffffu, 0, minmax_after);  // lane 63 holds the wave's
+    const int w = threadIdx.x >> 6;
+    const uint32_t wb = __any(bad != 0u) ? 1u : 0u;
+    if ((threadIdx.x & 63) == 63) {
+        s_mn[w] = (uint32_t)a;
+        s_mx[w] = (uint32_t)b;
+        s_bad[w] = wb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m0 = s_mn[0], m1 = s_mx[0], bb = s_bad[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            m0 = min(m0, s_mn[k]);
+            m1 = max(m1, s_mx[k]);
+            bb |= s_bad[k];
+        }
+        part[0] = m0;
+        part[1] = m1;
+        part[2] = bb;
+    }
+}
+
+// The raster's quantized min / max (ordered) and domain flag from the
+// blocks' partials: one workgroup, its own loads of the partials all in
+// flight per round, the three scalar words written once.
+constexpr int MMF_T = 1024;
+__global__ __launch_bounds__(MMF_T) void k_minmax_finish(const uint32_t* __restrict__ part, int nb,
+                                                         uint32_t* __restrict__ scal) {
+    uint32_t mn = 0xffffffffu, mx = 0u, bad = 0u;
+#pragma unroll 8
+    for (int i = threadIdx.x; i < nb; i += MMF_T) {
+        mn = min(mn, part[3 * (size_t)i]);
+        mx = max(mx, part[3 * (size_t)i + 1]);
+        bad |= part[3 * (size_t)i + 2];
+    }
+    __shared__ uint32_t s_mn[MMF_T / 64], s_mx[MMF_T / 64], s_bad[MMF_T / 64];
+    int a = (int)mn, b = (int)mx;
+    CAGEO_SCAN_UP(dpp_i, a, b, (int)0xffffffffu, 0, minmax_after);
+    const int w = threadIdx.x >> 6;
+    const uint32_t wb = __any(bad != 0u) ? 1u : 0u;
+    if ((threadIdx.x & 63) == 63) {
+        s_mn[w] = (uint32_t)a;
+        s_mx[w] = (uint32_t)b;
+        s_bad[w] = wb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m0 = s_mn[0], m1 = s_mx[0], bb = s_bad[0];
+        for (int k = 1; k < MMF_T / 64; ++k) {
+            m0 = min(m0, s_mn[k]);
+            m1 = max(m1, s_mx[k]);
+            bb |= s_bad[k];
+        }
+        scal[SC_MIN] = m0;
+        scal[SC_MAX] = m1;
+        scal[SC_BAD] = bb;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_quantize_minmax(float* __restrict__ z, int64_t n,
                                                          float nodata,
                                                          uint32_t* __restrict__ scal) {
@@ -253,11 +317,13 @@ __global__ __launch_bounds__(256) void k_quantize_minmax(float* __restrict__ z, 
 }
 
 // Fused realistic DEM + burn + quantisation + min/max for the Monte Carlo
-// driver: one pass over dem, error field and stream flags.
+// driver: one pass over dem, error field and stream flags.  Each block
+// leaves its min/max partial in `part` (3 words per block) for
+// k_minmax_finish, so no block waits on the global words.
 __global__ __launch_bounds__(256) void k_realistic_burn_quant(
     const float* __restrict__ dem, const float* __restrict__ e, const uint8_t* __restrict__ sf,
     const int* __restrict__ off, const float* __restrict__ fac, int n_off, int R, int W, int H,
-    float nodata_dem, float* __restrict__ zq, uint32_t* __restrict__ scal) {
+    float nodata_dem, float* __restrict__ zq, uint32_t* __restrict__ part) {
     float v[4];
     int gx[4], gy[4];
     burn_tile<true>(W, H, sf, off, fac, n_off, R, nodata_dem, dem, e, v, gx, gy);
@@ -272,7 +338,7 @@ __global__ __launch_bounds__(256) void k_realistic_burn_quant(
             mx = max(mx, o);
         }
     }
-    block_minmax(mn, mx, bad, scal);
+    block_minmax_store(mn, mx, bad, part + 3 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x));
 }
 
 inline unsigned grid_1d(size_t n) {
@@ -352,11 +418,20 @@ int launch_quantize(Context* c, float* dem, float nodata) {
     return CAGEO_OK;
 }
 
+// Blocks of k_realistic_burn_quant (the size of the context's d_bmm, 3 words each).
+size_t burn_blocks(int W, int H) {
+    return (size_t)((W + BT_X - 1) / BT_X) * (size_t)((H + BT_Y - 1) / BT_Y);
+}
+
+// Writes the raster's min / max / domain words (SC_MIN, SC_MAX, SC_BAD) as
+// well: they need no reset before it.
 int launch_realistic_burn_quant(Context* c, const float* dem, const float* e, float* zq) {
     dim3 grid((c->W + BT_X - 1) / BT_X, (c->H + BT_Y - 1) / BT_Y);
     hipLaunchKernelGGL(k_realistic_burn_quant, grid, dim3(256), 0, c->stream, dem, e,
                        c->d_sflags, c->d_off, c->d_factor, c->n_offsets, c->off_radius, c->W, c->H,
-                       c->nodata_dem, zq, c->d_scal);
+                       c->nodata_dem, zq, c->d_bmm);
+    hipLaunchKernelGGL(k_minmax_finish, dim3(1), dim3(MMF_T), 0, c->stream, c->d_bmm,
+                       (int)burn_blocks(c->W, c->H), c->d_scal);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
 }
