@@ -102,9 +102,9 @@ int read_scalars(Context* c) {
 
 __global__ void k_reset_launch_state(uint32_t* __restrict__ status, uint32_t* __restrict__ bar,
                                      uint32_t* __restrict__ scal) {
-    // the generation word keeps counting; every other barrier word restarts
+    // the generation word (64-bit) keeps counting; every other barrier word restarts
     for (int i = threadIdx.x; i < BAR_WORDS; i += blockDim.x)
-        if (i != BAR_GEN) bar[i] = 0u;
+        if (i != BAR_GEN && i != BAR_GEN + 1) bar[i] = 0u;
     if (threadIdx.x == 0) {
         *status = 0u;
         if (scal) scal[SC_LONGRUN] = 0u;

@@ -187,6 +187,15 @@ __host__ __device__ inline bool pf_live(float v, float min_elev) {
 // so the reset is seen before any arrival at the next barrier (those follow
 // the generation bump, which follows the top count).
 //
+// The barrier also sums two counts over the workgroups (grid_sync_sum): the
+// arrival words are 64-bit, arrivals in bits 0-15 and the two partial sums
+// in bits 16-39 and 40-63, and the last arrival publishes the totals with
+// the new generation in one 64-bit word (generation in bits 40-63, the sums
+// in 20 bits each), so a waiting workgroup has them with the poll that ends
+// its wait instead of one more dependent read after it.  The sums must stay
+// below 2^20 (callers with more blocks than that read their counts after
+// the barrier instead).
+//
 // The wait is bounded.  If co-residency is lost (another launch holds part
 // of the device for long, or workgroups of a persistent launch on another
 // stream of the same device take the places of some of this launch's), or
@@ -235,34 +244,52 @@ __device__ inline bool bar_should_stop(const GridBar& gb, uint64_t t0, bool host
 // Returns false when the launch is being abandoned (then every workgroup
 // must leave its loops and end without another barrier).  The barrier is
 // over workgroups 0 .. n-1 (all of the grid by default; a launch may go on
-// with its first n workgroups once the others have ended).
-__device__ inline bool grid_sync(const GridBar& gb, uint32_t n = 0) {
+// with its first n workgroups once the others have ended).  add0 / add1
+// (thread 0's) are summed over the workgroups into sum0 / sum1 (every
+// thread's).
+constexpr uint64_t BAR_SUM_LIMIT = 1u << 20;
+__device__ inline bool grid_sync_sum(const GridBar& gb, uint32_t n, uint32_t add0, uint32_t add1,
+                                     uint32_t& sum0, uint32_t& sum1) {
+    __shared__ uint32_t s_sum[2];
     if (n == 0) n = gridDim.x;
     __syncthreads();
     int fail = 0;
     if (threadIdx.x == 0) {
         uint32_t* bar = gb.bar;
-        uint32_t* gen = &bar[BAR_GEN];
-        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t* gen = reinterpret_cast<uint64_t*>(&bar[BAR_GEN]);
+        const uint64_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 40;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t groups = n < (uint32_t)BAR_GROUPS ? n : (uint32_t)BAR_GROUPS;
         const uint32_t grp = blockIdx.x % groups;
         const uint32_t members = (n - grp + groups - 1) / groups;
-        uint32_t* gc = &bar[BAR_GROUP + grp * BAR_STRIDE];
-        if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
-            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t* gc = reinterpret_cast<uint64_t*>(&bar[BAR_GROUP + grp * BAR_STRIDE]);
+        const uint64_t mine = 1u | ((uint64_t)add0 << 16) | ((uint64_t)add1 << 40);
+        const uint64_t old = __hip_atomic_fetch_add(gc, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((old & 0xffffu) == members - 1) {
+            const uint64_t group = old + mine;  // the group's arrivals and sums
+            __hip_atomic_store(gc, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            uint32_t* top = &bar[BAR_COUNT];
-            if (__hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == groups - 1) {
-                // last arrival: reset the top counter, then open the next generation
-                __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t* top = reinterpret_cast<uint64_t*>(&bar[BAR_COUNT]);
+            const uint64_t gm = 1u | (group & ~(uint64_t)0xffffu);
+            const uint64_t told = __hip_atomic_fetch_add(top, gm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((told & 0xffffu) == groups - 1) {
+                // last arrival: reset the top counter, then open the next
+                // generation with the totals
+                const uint64_t all = told + gm;
+                const uint64_t s0 = (all >> 16) & 0xffffffu, s1 = all >> 40;
+                __hip_atomic_store(top, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(gen,
+                                   (((g + 1) & 0xffffffu) << 40) | ((s0 & 0xfffffu) << 20) |
+                                       (s1 & 0xfffffu),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         const uint64_t t0 = wall_clock64();
+        uint64_t now = 0;
         for (uint32_t it = 1;
-             __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g; ++it) {
+             ((now = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 40) == g;
+             ++it) {
             // a zero bound gives up at the first poll that finds the
             // barrier closed
             if ((it % BAR_CHECK_EVERY == 0 || gb.limit == 0) &&
@@ -275,8 +302,17 @@ __device__ inline bool grid_sync(const GridBar& gb, uint32_t n = 0) {
             __builtin_amdgcn_s_sleep(2);
         }
         if (!fail) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_sum[0] = (uint32_t)((now >> 20) & 0xfffffu);
+        s_sum[1] = (uint32_t)(now & 0xfffffu);
     }
-    return !__syncthreads_or(fail);
+    const bool ok = !__syncthreads_or(fail);
+    sum0 = s_sum[0];
+    sum1 = s_sum[1];
+    return ok;
+}
+__device__ inline bool grid_sync(const GridBar& gb, uint32_t n = 0) {
+    uint32_t s0, s1;
+    return grid_sync_sum(gb, n, 0u, 0u, s0, s1);
 }
 
 // True (in every thread of the workgroup) when an earlier launch on this
@@ -306,7 +342,7 @@ enum : int {
     CTL_EPOCH = 7,      // epoch counter of the pit-fill worklists
     CTL_ROUNDS = 8,
     CTL_PASSES = 9,
-    CTL_FLAG = 10,      // 3 rotating "pending" flags (tracing)
+    CTL_FLAG = 10,      // 3 words, free (pass-end flags now travel with the barrier)
     CTL_EPOCH_FR = 13,  // epoch counter of the flat-resolution worklists
     CTL_FR_PASSES = 14, // flat-resolution passes of the last launch
     CTL_TR_ROUNDS = 15, // exit-jump rounds of the last tracing launch

@@ -77,13 +77,15 @@ __device__ inline float2 root(int t, int k) {
     return make_float2(ROOT16_RE[m], INV ? -ROOT16_IM[m] : ROOT16_IM[m]);
 }
 
+// The steps are inlined: as calls their frames set the kernels' registers
+// (215 VGPRs against about 70 inlined).
 // r consecutive radix-2 stages (R = 2^r values per unit in registers) over C
 // sequences of length N = 2^L stored at sh + c * sp (padded).  DIT (forward,
 // INV false): stages s .. s+r-1 with half sizes h .. h 2^(r-1), h = 2^s.
 // DIF (inverse, INV true): the same stages in reverse order.  A unit holds
 // positions base + m h, m < R, base = (u / h) R h + (u mod h).
 template <int R, bool INV>
-__device__ void fft_step(float2* sh, int sp, int C, int L, int lgh, const float2* __restrict__ tw,
+__device__ __forceinline__ void fft_step(float2* sh, int sp, int C, int L, int lgh, const float2* __restrict__ tw,
                          int lgM) {
     constexpr int r = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1;
     const int h = 1 << lgh;
@@ -131,7 +133,7 @@ __device__ void fft_step(float2* sh, int sp, int C, int L, int lgh, const float2
 }
 
 template <bool INV>
-__device__ void fft_dispatch(float2* sh, int sp, int C, int L, int lgh, int r,
+__device__ __forceinline__ void fft_dispatch(float2* sh, int sp, int C, int L, int lgh, int r,
                              const float2* __restrict__ tw, int lgM) {
     switch (r) {
         case 4: fft_step<16, INV>(sh, sp, C, L, lgh, tw, lgM); break;
@@ -143,7 +145,7 @@ __device__ void fft_dispatch(float2* sh, int sp, int C, int L, int lgh, int r,
 
 // Forward transform of C sequences loaded in bit-reversed order; natural
 // order out.
-__device__ void fft_forward(float2* sh, int sp, int C, int L, const float2* __restrict__ tw,
+__device__ __forceinline__ void fft_forward(float2* sh, int sp, int C, int L, const float2* __restrict__ tw,
                             int lgM) {
     for (int s = 0; s < L;) {
         const int r = min(4, L - s);
@@ -154,7 +156,7 @@ __device__ void fft_forward(float2* sh, int sp, int C, int L, const float2* __re
 }
 // Inverse (unnormalised) transform of C sequences in natural order;
 // bit-reversed order out.
-__device__ void fft_inverse(float2* sh, int sp, int C, int L, const float2* __restrict__ tw,
+__device__ __forceinline__ void fft_inverse(float2* sh, int sp, int C, int L, const float2* __restrict__ tw,
                             int lgM) {
     for (int s = L; s > 0;) {
         const int r = min(4, s);

@@ -38,6 +38,7 @@ struct FrShared {
     uint32_t wcol[TILE_THREADS / 64], ecol[TILE_THREADS / 64];  // changed rows of columns 0, TILE-1
     uint32_t bits;
     uint32_t any;
+    uint32_t pushed;  // tiles this workgroup listed in the pass (thread 0's)
 };
 
 // Min-plus map v -> min(a, v + k) on distances saturated at D_INF (k = D_INF
@@ -315,7 +316,8 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
         unsigned long long w64, e64;
         edge_columns_get(S.wcol, S.ecol, w64, e64);
         activate_tiles(S.bits, tx, ty, a.ntx, a.nty, b.gbase, work_out, count_out, args.mark,
-                       epoch, args.rows, (pass + 1) & 1, w64, e64);
+                       epoch, args.rows, (pass + 1) & 1, w64, e64, nullptr, nullptr,
+                       &S.pushed);
     }
     __syncthreads();
 }
@@ -396,9 +398,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     int li = 0;
     bool converged = false;
     int pass = 0;
+    // after a pass the next list's count comes with the barrier (the sum of
+    // what each workgroup listed) where the blocks are fewer than
+    // BAR_SUM_LIMIT
+    const bool sums = (uint64_t)a.total < BAR_SUM_LIMIT;
+    uint32_t next_count = 0;
     for (; pass < a.max_passes; ++pass) {
         const int cin = li, cout = (li + 1) % 3, cclr = (li + 2) % 3;
-        const uint32_t count = ld(&a.ctrl[CTL_COUNT + cin]);
+        const uint32_t count = pass > 0 && sums ? next_count : ld(&a.ctrl[CTL_COUNT + cin]);
         if (count == 0) {
             converged = true;
             break;
@@ -407,9 +414,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_COUNT + cclr] = 0;
         const int* win = a.work + (size_t)cin * a.total;
         int* wout = a.work + (size_t)cout * a.total;
+        if (threadIdx.x == 0) S.pushed = 0u;
         for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
             bfs_tile(S, a, fr_block<MULTI>(a, win[i]), wout, &a.ctrl[CTL_COUNT + cout], epoch, pass);
-        if (!grid_sync(a.gb)) return;
+        uint32_t unused = 0;
+        if (!grid_sync_sum(a.gb, 0, sums ? S.pushed : 0u, 0u, next_count, unused)) return;
         li = cout;
     }
     // The words written below are not read by the pass loop, so no barrier

@@ -284,12 +284,11 @@ __device__ inline int32_t label_of_cell0(const TrRaster& R) {
 // registers, so many blocks' load chains are in flight at once; only the
 // jumping needs co-resident workgroups.
 
-// Tile pass.  Also clears the jumping's three round flags and the long-path
-// count: the launch boundary orders both before their first use.
+// Tile pass.  Also clears the long-path count: the launch boundary orders
+// that before its first use.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_tr_tiles(TrArgs a) {
     __shared__ uint64_t L[TILE * TILE];
-    if (blockIdx.x == 0 && threadIdx.x < 3) a.ctrl[CTL_FLAG + threadIdx.x] = 0;
     const TrBlock b = tr_block<MULTI>(a, blockIdx.x);
     const TrRaster& R = b.R;
     if (b.lb == 0 && threadIdx.x == 0) R.scal[SC_LONGPATH] = 0;
@@ -298,9 +297,8 @@ __global__ __launch_bounds__(256) void k_tr_tiles(TrArgs a) {
 
 // Jumping over the border tables: thread s takes slot s % BORDER_SLOTS of
 // global block s / BORDER_SLOTS (the blocks of one raster are consecutive,
-// so a wave's 64 slots lie in one raster).  Round r raises flag r % 3 while
-// exits remain and clears flag (r+1) % 3, whose last read (end of round
-// r-2) is separated from the clear by barriers.
+// so a wave's 64 slots lie in one raster).  The barrier ending a round sums
+// the workgroups' "an exit remains" bits.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_tr_jump(TrArgs a) {
     if (grid_aborted(a.gb)) return;
@@ -321,10 +319,10 @@ __global__ __launch_bounds__(256) void k_tr_jump(TrArgs a) {
                 st64(&tb[own], ne);
             }
         }
-        if (__any(pending) && (threadIdx.x & 63) == 0) atomicOr(&a.ctrl[CTL_FLAG + r % 3], 1u);
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_FLAG + (r + 1) % 3] = 0;
-        if (!grid_sync(a.gb)) return;
-        if (!ld(&a.ctrl[CTL_FLAG + r % 3])) break;
+        const uint32_t mine = __syncthreads_or(pending != 0u) ? 1u : 0u;
+        uint32_t any = 0, unused = 0;
+        if (!grid_sync_sum(a.gb, 0, mine, 0u, any, unused)) return;
+        if (!any) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl[CTL_TR_ROUNDS] = (uint32_t)rounds;  // host only
 }

@@ -63,6 +63,7 @@ struct PfShared {
     uint32_t next;  // the workgroup's next tile index in the pass
     uint32_t steps; // row steps of the tile's solve, all waves
     uint32_t ringrows[2];  // resident rounds: tile rows next to changed halo cells
+    uint32_t pfront, pback;  // entries this workgroup listed in the pass (thread 0's)
 };
 
 // run_passes result of a launch being abandoned at a barrier.
@@ -367,7 +368,7 @@ __device__ inline void finish_tile(PfShared& S, const PfArgs& a, const PfBlock& 
         // cells never reappear) cannot change and is not listed
         activate_tiles(S.bits, b.lb % b.R.ntx, b.lb / b.R.ntx, b.R.ntx, b.R.nty, b.gbase,
                        work_out, count_out, a.mark, epoch, a.rows, (pass + 1) & 1, w64, e64,
-                       &rank, a.topen);
+                       &rank, a.topen, &S.pfront, &S.pback);
     }
     __syncthreads();
 }
@@ -660,17 +661,23 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
 // ended the phase before, is cleared for the phase after.  Fixed cells
 // count into counter `slot` of their raster.
 // With `compacted`, the first pass reads the compacted fill list instead.
+// After a pass, the next list's front and back counts come with the
+// barrier (the sums of what each workgroup listed) where the blocks are
+// fewer than BAR_SUM_LIMIT; else they are read from the control words.
 template <int KIND, bool MULTI>
 __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int slot, int round,
                           bool compacted, uint32_t& epoch, float mn1, bool ok1) {
     const int pin = phase % 3, pout = (phase + 1) % 3;
+    const bool sums = (uint64_t)a.total < BAR_SUM_LIMIT;
+    uint32_t next_front = 0, next_back = 0;
     for (int pass = 0; pass < a.max_passes; ++pass) {
         const int cout = (li + 1) % 3, cclr = (li + 2) % 3;
         const int lin = pass > 0 ? li : compacted ? 6 : 3 + pin;
         const int cf = pass > 0 ? CTL_COUNT + li : compacted ? CTL_COMPACT : CTL_PHASE + pin;
         const int cb = pass > 0 ? CTL_COUNTB + li : compacted ? CTL_COMPACTB : CTL_PHASEB + pin;
-        const uint32_t nfront = ld(&a.ctrl[cf]);
-        const uint32_t count = nfront + ld(&a.ctrl[cb]);
+        const bool from_bar = pass > 0 && sums;
+        const uint32_t nfront = from_bar ? next_front : ld(&a.ctrl[cf]);
+        const uint32_t count = nfront + (from_bar ? next_back : ld(&a.ctrl[cb]));
         if (count == 0) return li;
         ++epoch;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -690,6 +697,10 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
         }
         const int* win = a.work + (size_t)lin * a.total;
         int* wout = a.work + (size_t)cout * a.total;
+        if (threadIdx.x == 0) {
+            S.pfront = 0u;
+            S.pback = 0u;
+        }
         for (uint32_t i = blockIdx.x; i < count;) {
             uint32_t next = count;
             if (threadIdx.x == 0) next = claim_next(&a.ctrl[CTL_CLAIM + li], count);
@@ -708,7 +719,9 @@ __device__ int run_passes(PfShared& S, const PfArgs& a, int li, int phase, int s
             i = S.next;
             __syncthreads();
         }
-        if (!grid_sync(a.gb)) return PF_ABANDONED;
+        if (!grid_sync_sum(a.gb, 0, sums ? S.pfront : 0u, sums ? S.pback : 0u, next_front,
+                           next_back))
+            return PF_ABANDONED;
         li = cout;
     }
     return -1;
@@ -746,11 +759,10 @@ __device__ inline void compact_open(const PfArgs& a, int from) {
 // rounds and per pass reloads only the 260 halo cells around it, solving the
 // rows next to halo cells that changed.  A pass ends at a barrier over these
 // workgroups, and a phase ends after a pass in which no tile changed a
-// border cell (then no halo changes and every tile is at its fixed point).
-// Passes raise rotating flags CTL_FLAG + q % 3 (q counts the resident
-// passes); workgroup 0 clears flag (q + 1) % 3 during pass q, whose last read
-// ended pass q - 2.  The rules of each phase are those of the classic
-// passes: the same sweeps, write-back and counters.
+// border cell (then no halo changes and every tile is at its fixed point):
+// the barrier sums the workgroups' "changed a border cell" bits.  The rules
+// of each phase are those of the classic passes: the same sweeps,
+// write-back and counters.
 
 // Reloads the halo ring (the restart rule as in load_tile); returns the
 // 64-bit mask of tile rows next to ring cells whose relay or live value
@@ -904,10 +916,7 @@ __device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round
             unsigned long long dirty = ~0ull;
             const unsigned long long ring = load_ring(S, R, x0, y0, pass == 0);
             if (pass > 0) dirty = ring;
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                a.ctrl[CTL_FLAG + (q + 1) % 3] = 0;
-                atomicAdd(&a.ctrl[CTL_PASSES], 1u);
-            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.ctrl[CTL_PASSES], 1u);
             uint32_t any_border = 0;
             if (dirty) {
                 if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
@@ -931,9 +940,10 @@ __device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round
                 rewrite = 0;
                 resident_reduce(S, nfixed, bits, &a.cert[slot], any_border);
             }
-            if (any_border && threadIdx.x == 0) atomicOr(&a.ctrl[CTL_FLAG + q % 3], 1u);
-            if (!grid_sync(a.gb, n)) return PF_ABANDONED;
-            if (!ld(&a.ctrl[CTL_FLAG + q % 3])) {
+            uint32_t changed = 0, unused = 0;
+            if (!grid_sync_sum(a.gb, n, any_border ? 1u : 0u, 0u, changed, unused))
+                return PF_ABANDONED;
+            if (!changed) {
                 ++q;
                 break;
             }
@@ -957,10 +967,7 @@ __device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round
             if (pass >= a.max_passes) return -1;
             const unsigned long long ring = load_ring(S, R, x0, y0, false);
             const unsigned long long dirty = pass == 0 ? ~0ull : ring;
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                a.ctrl[CTL_FLAG + (q + 1) % 3] = 0;
-                atomicAdd(&a.ctrl[CTL_PASSES], 1u);
-            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.ctrl[CTL_PASSES], 1u);
             uint32_t any_border = 0;
             if (dirty) {
                 if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
@@ -983,9 +990,10 @@ __device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round
                 open &= ~fixed;
                 resident_reduce(S, nfixed, bits, &a.cert[slot], any_border);
             }
-            if (any_border && threadIdx.x == 0) atomicOr(&a.ctrl[CTL_FLAG + q % 3], 1u);
-            if (!grid_sync(a.gb, n)) return PF_ABANDONED;
-            if (!ld(&a.ctrl[CTL_FLAG + q % 3])) {
+            uint32_t changed = 0, unused = 0;
+            if (!grid_sync_sum(a.gb, n, any_border ? 1u : 0u, 0u, changed, unused))
+                return PF_ABANDONED;
+            if (!changed) {
                 ++q;
                 break;
             }
@@ -1085,9 +1093,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         a.ctrl[CTL_PHASEB + 1] = 0;
         a.ctrl[CTL_PHASEB + 2] = 0;
         a.ctrl[CTL_COMPACTB] = 0;
-        a.ctrl[CTL_FLAG + 0] = 0;  // pass flags of the resident rounds
-        a.ctrl[CTL_FLAG + 1] = 0;
-        a.ctrl[CTL_FLAG + 2] = 0;
         a.ctrl[CTL_RESIDENT] = 0;
         a.ctrl[CTL_PASSES] = 0;  // written by workgroup 0 only, read by the host
     }

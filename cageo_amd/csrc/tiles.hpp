@@ -298,14 +298,17 @@ __device__ inline int ranked_entry(const int* lst, int cap, uint32_t nfront, uin
 // last solve took at least `rank->heavy` row steps goes to the front.
 // With `open` set, a neighbour whose word there is 0 (it can no longer
 // change) is not listed; the word is read with the mark exchange, so the
-// check adds no round trip.
+// check adds no round trip.  pushed_front / pushed_back (lane 0's, usually
+// LDS words of the caller's thread 0) count the entries this call listed.
 __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, int nty, int gbase,
                                       int* __restrict__ work_out, uint32_t* count_out,
                                       uint32_t* __restrict__ mark, uint32_t epoch,
                                       uint32_t* rows = nullptr, int rpar = 0,
                                       unsigned long long w64 = 0, unsigned long long e64 = 0,
                                       const Ranked* rank = nullptr,
-                                      const uint32_t* open = nullptr) {
+                                      const uint32_t* open = nullptr,
+                                      uint32_t* pushed_front = nullptr,
+                                      uint32_t* pushed_back = nullptr) {
     const int lane = threadIdx.x & 63;
     bool push = false, heavy = true;
     int t = 0;
@@ -330,13 +333,19 @@ __device__ inline void activate_tiles(uint32_t bits, int tx, int ty, int ntx, in
     const unsigned long long ml = __ballot(push && !heavy);
     if (mh) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(count_out, (uint32_t)__popcll(mh));
+        if (lane == 0) {
+            base = atomicAdd(count_out, (uint32_t)__popcll(mh));
+            if (pushed_front) *pushed_front += (uint32_t)__popcll(mh);
+        }
         base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
         if (push && heavy) work_out[base + lanes_below(mh)] = t;
     }
     if (ml) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(rank->back, (uint32_t)__popcll(ml));
+        if (lane == 0) {
+            base = atomicAdd(rank->back, (uint32_t)__popcll(ml));
+            if (pushed_back) *pushed_back += (uint32_t)__popcll(ml);
+        }
         base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
         if (push && !heavy) work_out[rank->cap - 1 - (int)(base + lanes_below(ml))] = t;
     }
