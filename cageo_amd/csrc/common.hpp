@@ -342,8 +342,8 @@ enum : int {
     CTL_EPOCH = 7,      // epoch counter of the pit-fill worklists
     CTL_ROUNDS = 8,
     CTL_PASSES = 9,
-    CTL_FLAG = 10,      // 3 words, free (pass-end flags now travel with the barrier)
-    CTL_EPOCH_FR = 13,  // epoch counter of the flat-resolution worklists
+    CTL_FLAG = 10,      // 3 words, free (pass-end flags travel with the barrier)
+    CTL_EPOCH_FR = 13,  // free (flat-resolution epochs start over each launch)
     CTL_FR_PASSES = 14, // flat-resolution passes of the last launch
     CTL_TR_ROUNDS = 15, // exit-jump rounds of the last tracing launch
     CTL_PHASE = 16,     // 3 phase-list counts (pit fill)

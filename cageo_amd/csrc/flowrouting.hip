@@ -108,56 +108,54 @@ __device__ inline uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// D8 of one cell (FlowRoutingAlgorithmCPUOpenMP.h:60-150); returns the
-// initial distance: 0 has a direction, D_STREAM, or D_INF (flat).
-// D8 of cell (x, y) from the block's halo tile of elevations `zt` (HALO_W
-// wide; (lx, ly) the cell's place in the block).  Edge cells point out of
-// the grid and read no neighbour, so the halo's cells outside the grid are
-// never used.
-__device__ inline int32_t d8_cell(const float* zt, int8_t* __restrict__ fd, int x, int y, int lx,
-                                  int ly, int W, int H) {
-    const size_t i = (size_t)y * W + x;
+// D8 of one cell (FlowRoutingAlgorithmCPUOpenMP.h:60-150) at grid position
+// (x, y) from a halo tile of elevations `zt` (rows `stride` apart; the cell
+// at index c).  Returns the initial distance: 0 has a direction, D_STREAM,
+// or D_INF (flat); (ox, oy) the direction.  Edge cells point out of the grid
+// and read no neighbour, so halo cells outside the grid are never used.
+__device__ inline int32_t d8_at(const float* zt, int stride, int c, int x, int y, int W, int H,
+                                int8_t& ox, int8_t& oy) {
     int dxi = 0, dyi = 0;
     if (x == 0) dxi = -1;
     else if (x == W - 1) dxi = 1;
     if (y == 0) dyi = -1;
     else if (y == H - 1) dyi = 1;
-    int8_t ox = 0, oy = 0;
-    int32_t d;
+    ox = 0;
+    oy = 0;
     if (dxi != 0 || dyi != 0) {
         ox = (int8_t)dxi;
         oy = (int8_t)dyi;
-        d = 0;
-    } else {
-        const int c = (ly + 1) * HALO_W + lx + 1;
-        const float zc = zt[c];
-        if (zc == 0.0f) {
-            d = D_STREAM;
-        } else {
-            // the direction is carried as its offsets, each a constant of
-            // the unrolled loop (indexing the tables by a variable would load
-            // them from memory)
-            float steepest = 0.0f;
-            d = D_INF;
+        return 0;
+    }
+    const float zc = zt[c];
+    if (zc == 0.0f) return D_STREAM;
+    // the direction is carried as its offsets, each a constant of the
+    // unrolled loop (indexing the tables by a variable would load them from
+    // memory)
+    float steepest = 0.0f;
+    int32_t d = D_INF;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                float s = zc - zt[c + FDY[k] * HALO_W + FDX[k]];
-                if (s > 0.0f) {
-                    if (k & 1) s = s / 1.41421356237309505f;
-                    if (s > steepest) {
-                        steepest = s;
-                        ox = (int8_t)FDX[k];
-                        oy = (int8_t)FDY[k];
-                        d = 0;
-                    }
-                }
+    for (int k = 0; k < 8; ++k) {
+        float s = zc - zt[c + FDY[k] * stride + FDX[k]];
+        if (s > 0.0f) {
+            if (k & 1) s = s / 1.41421356237309505f;
+            if (s > steepest) {
+                steepest = s;
+                ox = (int8_t)FDX[k];
+                oy = (int8_t)FDY[k];
+                d = 0;
             }
         }
     }
-    reinterpret_cast<char2*>(fd)[i] = make_char2(ox, oy);
     return d;
 }
 
+__device__ void bfs_solve(FrShared& S, const FrArgs& args, const FrBlock& b, int tx, int ty,
+                          unsigned long long dirty, int* work_out, uint32_t* count_out,
+                          uint32_t epoch, int pass, bool write_all);
+
+// A listed tile of a flat pass: its halo tile of distances and elevations
+// loaded, then solved (bfs_solve).
 __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int* work_out,
                          uint32_t* count_out, uint32_t epoch, int pass) {
     const FrRaster& a = b.R;
@@ -199,7 +197,20 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     if (threadIdx.x == 0) S.bits = 0;
     __syncthreads();
     clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
+    bfs_solve(S, args, b, tx, ty, dirty, work_out, count_out, epoch, pass, false);
+}
 
+// The flat distances of a tile whose distances and elevations (with the
+// halo) are in LDS, S.rowdirty and S.bits clear: row solves over the rows of
+// `dirty` first, then every row a change reaches; the changed distances
+// written back (with write_all every distance of the block's cells), and
+// the neighbour tiles a changed border cell concerns listed for pass
+// `pass` + 1.
+__device__ void bfs_solve(FrShared& S, const FrArgs& args, const FrBlock& b, int tx, int ty,
+                          unsigned long long dirty, int* work_out, uint32_t* count_out,
+                          uint32_t epoch, int pass, bool write_all) {
+    const FrRaster& a = b.R;
+    const int x0 = tx * TILE, y0 = ty * TILE;
     const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
     uint32_t flat = 0;
 #pragma unroll
@@ -271,10 +282,10 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     uint32_t bits = 0, seen = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
-        if (changed_cells & (1u << k)) {
-            const int ly = strip * STRIP + k;
+        const int ly = strip * STRIP + k;
+        const bool in = x0 + lx < a.W && y0 + ly < a.H;
+        if (in && (write_all || (changed_cells & (1u << k))))
             a.dist[(size_t)(y0 + ly) * a.W + x0 + lx] = S.dz[(ly + 1) * HALO_W + lx + 1].x;
-        }
     }
     // A changed border cell concerns a neighbour tile only if one of its
     // cells in this tile's halo can take the new distance: a flat (d >= 1)
@@ -322,68 +333,96 @@ __device__ void bfs_tile(FrShared& S, const FrArgs& args, const FrBlock& b, int*
     __syncthreads();
 }
 
-// The three worklist counts start at zero (the D8 kernel fills list 0).
-__global__ void k_fr_reset(uint32_t* __restrict__ ctrl) {
-    if (threadIdx.x < 3) ctrl[CTL_COUNT + threadIdx.x] = 0u;
-    if (threadIdx.x == 3) ctrl[CTL_FR_DONE] = 0u;
+// The epoch stamp of pass 0's listings (the D8 kernel's); pass p lists with
+// FR_EPOCH0 + p.  k_fr_reset clears every stamp to 0 before a launch, so the
+// epochs can start over each launch and never come round to 0.
+constexpr uint32_t FR_EPOCH0 = 1u;
+
+// Before a launch's D8 kernel: every block's epoch stamp and dirty rows
+// cleared (an abandoned launch may have left some, and the D8 kernel's
+// blocks list each other, so none can clear its own there) and the worklist
+// counts zeroed.
+__global__ void k_fr_reset(FrArgs a) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < a.total; g += gridDim.x * blockDim.x) {
+        a.mark[g] = 0u;
+        reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 3) a.ctrl[CTL_COUNT + threadIdx.x] = 0u;
+        if (threadIdx.x == 3) a.ctrl[CTL_FR_DONE] = 0u;
+    }
 }
 
-// D8, one workgroup per block, from the block's halo tile of elevations
-// staged in LDS (one batch of loads); blocks holding flats open list 0.  A
-// launch of its own with every block a workgroup: in the persistent kernel
-// each workgroup walked its blocks one after another, a latency chain.
+// D8 and the first flat pass, one workgroup per block.  The block's
+// elevations with a two-cell halo are staged in LDS, D8 is worked out for
+// the 66 x 66 cells of its distance tile (its own and the ring around it:
+// the ring's are the initial distances its neighbours start from), and the
+// first pass solves the tile against that ring as the persistent kernel's
+// first pass did against the distances the D8 kernel had written; then
+// the block's distances and the directions of its cells go out, and the
+// tiles its changed border cells concern are listed for pass 1 (list 1,
+// epoch FR_EPOCH0).  The persistent kernel goes on from pass 1.
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_fr_d8(FrArgs a) {
-    __shared__ float zt[HALO_W * HALO_W];
-    const int g = blockIdx.x;
-    const FrBlock b = fr_block<MULTI>(a, g);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_fr_d8(FrArgs a) {
+    __shared__ FrShared S;
+    const FrBlock b = fr_block<MULTI>(a, blockIdx.x);
     const FrRaster& R = b.R;
-    const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
+    const int tx = b.lb % R.ntx, ty = b.lb / R.ntx;
+    const int x0 = tx * TILE, y0 = ty * TILE;
+    constexpr int ZW = HALO_W + 2, ZN = ZW * ZW;  // elevations with two halo cells
+    constexpr int N = HALO_W * HALO_W;            // the distance tile
+    static_assert(ZN * sizeof(float) <= sizeof(S.dz), "the elevation halo fits the distance tile");
+    float* zt = reinterpret_cast<float*>(S.dz);
     {
-        constexpr int N = HALO_W * HALO_W;
-        constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
-        float v[ITER];
+        constexpr int IT = (ZN + TILE_THREADS - 1) / TILE_THREADS;
+        float v[IT];
 #pragma unroll
-        for (int j = 0; j < ITER; ++j) {
+        for (int j = 0; j < IT; ++j) {  // all loads first (a cell outside loads cell 0)
             const int k = threadIdx.x + j * TILE_THREADS;
-            const int hy = k / HALO_W, hx = k - hy * HALO_W;
-            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-            v[j] = 0.0f;
-            if (k < N && gx >= 0 && gy >= 0 && gx < R.W && gy < R.H)
-                v[j] = R.z[(size_t)gy * R.W + gx];
+            const int hy = k / ZW, hx = k - hy * ZW;
+            const int gx = x0 - 2 + hx, gy = y0 - 2 + hy;
+            const bool in = k < ZN && gx >= 0 && gy >= 0 && gx < R.W && gy < R.H;
+            v[j] = R.z[in ? (size_t)gy * R.W + gx : 0];
+            if (!in) v[j] = 0.0f;
         }
 #pragma unroll
-        for (int j = 0; j < ITER; ++j) {
+        for (int j = 0; j < IT; ++j) {
             const int k = threadIdx.x + j * TILE_THREADS;
-            if (k < N) zt[k] = v[j];
+            if (k < ZN) zt[k] = v[j];
         }
     }
     __syncthreads();
-    const int lx = threadIdx.x & 63;
-    const int x = x0 + lx;
-    int has_flat = 0;
-    if (x < R.W) {
+    constexpr int ITER = (N + TILE_THREADS - 1) / TILE_THREADS;
+    int2 dzv[ITER];
 #pragma unroll
-        for (int k = 0; k < STRIP; ++k) {
-            const int ly = (threadIdx.x >> 6) * STRIP + k;
-            const int y = y0 + ly;
-            if (y < R.H) {
-                const int32_t d = d8_cell(zt, R.fd, x, y, lx, ly, R.W, R.H);
-                R.dist[(size_t)y * R.W + x] = d;
-                has_flat |= d == D_INF;
-            }
+    for (int j = 0; j < ITER; ++j) {
+        const int k = threadIdx.x + j * TILE_THREADS;
+        const int hy = k / HALO_W, hx = k - hy * HALO_W;
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+        dzv[j] = make_int2(D_OUT, 0);
+        if (k < N && gx >= 0 && gy >= 0 && gx < R.W && gy < R.H) {
+            const int c = (hy + 1) * ZW + hx + 1;
+            int8_t ox, oy;
+            const int32_t d = d8_at(zt, ZW, c, gx, gy, R.W, R.H, ox, oy);
+            dzv[j] = make_int2(d, __float_as_int(zt[c]));
+            if (hx >= 1 && hy >= 1 && hx <= TILE && hy <= TILE)  // the block's own cell
+                reinterpret_cast<char2*>(R.fd)[(size_t)gy * R.W + gx] = make_char2(ox, oy);
         }
     }
-    // the block's dirty rows and epoch stamp start clear: a launch that was
-    // abandoned may have left its first pass's (the persistent kernel has no
-    // barrier before that pass, and the epoch counter only advances when a
-    // launch completes)
-    if (threadIdx.x == 0) {
-        reinterpret_cast<uint4*>(a.rows)[g] = make_uint4(0u, 0u, 0u, 0u);
-        a.mark[g] = 0u;
+    __syncthreads();  // every read of the elevation halo is done
+#pragma unroll
+    for (int j = 0; j < ITER; ++j) {
+        const int k = threadIdx.x + j * TILE_THREADS;
+        if (k < N) S.dz[k] = dzv[j];
     }
-    if (__syncthreads_or(has_flat) && threadIdx.x == 0)
-        a.work[atomicAdd(&a.ctrl[CTL_COUNT + 0], 1u)] = g;
+    if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+        S.bits = 0;
+        S.pushed = 0;
+    }
+    __syncthreads();
+    bfs_solve(S, a, b, tx, ty, ~0ull, a.work + (size_t)a.total, &a.ctrl[CTL_COUNT + 1],
+              FR_EPOCH0, 0, true);
 }
 
 // The flat passes, after k_fr_d8 (k_fr_pick follows).
@@ -393,11 +432,11 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_fr_persistent(FrArgs a) {
     __shared__ FrShared S;
     if (grid_aborted(a.gb)) return;
-    uint32_t epoch = ld(&a.ctrl[CTL_EPOCH_FR]);
+    uint32_t epoch = FR_EPOCH0;
 
-    int li = 0;
+    int li = 1;  // pass 0 ran in k_fr_d8 and filled list 1
     bool converged = false;
-    int pass = 0;
+    int pass = 1;
     // after a pass the next list's count comes with the barrier (the sum of
     // what each workgroup listed) where the blocks are fewer than
     // BAR_SUM_LIMIT
@@ -405,7 +444,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     uint32_t next_count = 0;
     for (; pass < a.max_passes; ++pass) {
         const int cin = li, cout = (li + 1) % 3, cclr = (li + 2) % 3;
-        const uint32_t count = pass > 0 && sums ? next_count : ld(&a.ctrl[CTL_COUNT + cin]);
+        const uint32_t count = pass > 1 && sums ? next_count : ld(&a.ctrl[CTL_COUNT + cin]);
         if (count == 0) {
             converged = true;
             break;
@@ -430,7 +469,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.ctrl[CTL_EPOCH_FR] = epoch;
         a.ctrl[CTL_FR_PASSES] = (uint32_t)pass;  // read by the host only
         a.ctrl[CTL_FR_DONE] = 1u;               // k_fr_pick may go on
     }
@@ -549,7 +587,7 @@ int flow_routing(Context* c, const float* z, int8_t* fd) {
     a.ctrl = c->d_ctrl;
     a.gb = context_gridbar(c);
     a.max_passes = 1 << 20;
-    hipLaunchKernelGGL(k_fr_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
+    hipLaunchKernelGGL(k_fr_reset, dim3((a.total + 255) / 256), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_fr_d8<false>, dim3(a.total), dim3(TILE_THREADS), 0, c->stream, a);
     // ordinary launch within the occupancy cap (see pit_fill_quantized)
     hipLaunchKernelGGL(k_fr_persistent<false>, dim3(c->fr_grid), dim3(TILE_THREADS), 0, c->stream,
@@ -591,7 +629,7 @@ int flow_routing_multi(MultiState* m, hipStream_t s) {
     a.ctrl = m->d_ctrl;
     a.gb = multi_gridbar(m);
     a.max_passes = 1 << 20;
-    hipLaunchKernelGGL(k_fr_reset, dim3(1), dim3(64), 0, s, a.ctrl);
+    hipLaunchKernelGGL(k_fr_reset, dim3((a.total + 255) / 256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_fr_d8<true>, dim3(a.total), dim3(TILE_THREADS), 0, s, a);
     hipLaunchKernelGGL(k_fr_persistent<true>, dim3(m->fr_grid), dim3(TILE_THREADS), 0, s, a);
     hipLaunchKernelGGL(k_fr_pick<true>, dim3(a.total), dim3(TILE_THREADS), 0, s, a);
