@@ -264,8 +264,9 @@ __device__ void bfs_solve(FrShared& S, const FrArgs& args, const FrBlock& b, int
             scan_minplus_up(al, kl);
             scan_minplus_down(ar, kr);
             bool ch = false;
+            int nd = D_INF;
             if (isflat) {
-                const int nd = min(al, ar);
+                nd = min(al, ar);
                 if (nd < c.x) {
                     S.dz[li].x = nd;
                     changed_cells |= 1u << k;
@@ -274,7 +275,23 @@ __device__ void bfs_solve(FrShared& S, const FrArgs& args, const FrBlock& b, int
             }
             if (__ballot(ch)) {
                 changed = 1;
-                row_mark(S.rowdirty, rd, k, ly);
+                // The rows above and below are due again only where a cell
+                // can take the new distance: a flat (d >= 1) not lower than
+                // the changed cell whose distance exceeds the new one plus
+                // one (distances only decrease, so a neighbour strip's row
+                // read before that strip lowered it errs towards marking).
+                // Cells of the halo columns belong to the next tiles.
+                bool up = false, dn = false;
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    if (FDY[d] == 0) continue;
+                    if ((lx == 0 && FDX[d] < 0) || (lx == TILE - 1 && FDX[d] > 0)) continue;
+                    const bool t = ch && nb[d].x >= 1 && nb[d].x > nd + 1 &&
+                                   __int_as_float(nb[d].y) >= zc;
+                    if (FDY[d] < 0) up |= t;
+                    else dn |= t;
+                }
+                row_mark_dirs(S.rowdirty, rd, k, ly, __ballot(up) != 0, __ballot(dn) != 0);
             }
         }
         if (!__syncthreads_or(changed)) break;

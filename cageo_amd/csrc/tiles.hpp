@@ -159,6 +159,24 @@ __device__ inline bool row_due(uint32_t* rowdirty, uint32_t& rd, int k, int ly) 
     }
     return due;
 }
+// Row ly (row k of the strip) changed in a way that concerns the row above
+// (`up`) and/or the row below (`down`): those become due again.
+__device__ inline void row_mark_dirs(uint32_t* rowdirty, uint32_t& rd, int k, int ly, bool up,
+                                     bool down) {
+    if (up) {
+        if (k > 0)
+            rd |= 1u << (k - 1);
+        else if (ly > 0 && (threadIdx.x & 63) == 0)
+            rowdirty[ly - 1] = 1u;
+    }
+    if (down) {
+        if (k < STRIP - 1)
+            rd |= 1u << (k + 17);
+        else if (ly < TILE - 1 && (threadIdx.x & 63) == 0)
+            rowdirty[ly + 1] = 1u;
+    }
+}
+
 // Marks the rows next to row k of the strip (row ly of the tile).
 __device__ inline void row_mark(uint32_t* rowdirty, uint32_t& rd, int k, int ly) {
     if (k > 0)
