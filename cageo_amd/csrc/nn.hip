@@ -150,7 +150,14 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn ci, const uint16_t* __re
 // and the 1x1 head, each in the same arithmetic as its own kernel
 // (k_avgpool2, k_head), so a fused layer gives the same bits as the layer
 // followed by that kernel.
-constexpr int CL_TX = 64, CL_PAD = 40;
+constexpr int CL_TX = 64;
+
+// Place (in bf16) of 16-byte chunk c (0..3) of staged row r of the LDS
+// convolution: a row is one pixel's (or one tap and output channel's) 32
+// channels, 64 bytes with no padding, and its chunks are permuted by an XOR
+// with bits 1-2 of the row number, so the rows a fragment read touches at
+// once fall in distinct LDS banks.
+__device__ inline int cl_at(int r, int c) { return r * 32 + 8 * (c ^ ((r >> 1) & 3)); }
 
 struct ConvFuse {
     uint16_t* pool;       // [N][H/2][W/2][Cout]: k_avgpool2 of the output, or null
@@ -160,15 +167,15 @@ struct ConvFuse {
     int sigmoid;
 };
 
-template <int NT, int RW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv3x3_lds(ConvIn ci, const uint16_t* __restrict__ w,
+template <int NT, int RW, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_conv3x3_lds(ConvIn ci, const uint16_t* __restrict__ w,
                                                      const float* __restrict__ bias,
                                                      uint16_t* __restrict__ out,
                                                      float* __restrict__ outf, ConvFuse fz,
                                                      int N, int H, int W, int Cout, int relu) {
     constexpr int TY = 4 * RW, AX = CL_TX + 2, AY = TY + 2, CO = 16 * NT;
     constexpr int TAP_UNROLL = RW == 1 ? 9 : 1;  // two rows a wave: registers run short
-    constexpr int SA_N = AY * AX * CL_PAD, SB_N = 9 * CO * CL_PAD;
+    constexpr int SA_N = AY * AX * 32, SB_N = 9 * CO * 32;
     constexpr int NPX = TY * CL_TX;      // pixels of the output tile
     constexpr int PS = CO + 8;           // staged pixel stride (bf16), padded by 16 bytes
     constexpr int HP = NPX + 2;          // staged channel stride of the head's layout
@@ -238,12 +245,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
         for (int j = 0; j < AIT; ++j) {
             const int i = threadIdx.x + 256 * j;
-            if (i < AN) *reinterpret_cast<uint4*>(&sa[(i >> 2) * CL_PAD + 8 * (i & 3)]) = av[j];
+            if (i < AN) *reinterpret_cast<uint4*>(&sa[cl_at(i >> 2, i & 3)]) = av[j];
         }
 #pragma unroll
         for (int j = 0; j < BIT; ++j) {
             const int i = threadIdx.x + 256 * j;
-            if (i < BN) *reinterpret_cast<uint4*>(&sb[(i >> 2) * CL_PAD + 8 * (i & 3)]) = bv[j];
+            if (i < BN) *reinterpret_cast<uint4*>(&sb[cl_at(i >> 2, i & 3)]) = bv[j];
         }
         __syncthreads();
         if (c + 32 < Cin) load_slice(c + 32);
@@ -254,12 +261,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int m = 0; m < 4 * RW; ++m)
                 a[m] = as_bf16x8(*reinterpret_cast<const uint4*>(
-                    &sa[((wave * RW + m / 4 + dy) * AX + 16 * (m % 4) + row + dx) * CL_PAD +
-                        8 * kb]));
+                    &sa[cl_at((wave * RW + m / 4 + dy) * AX + 16 * (m % 4) + row + dx, kb)]));
 #pragma unroll
             for (int q = 0; q < NT; ++q)
                 b[q] = as_bf16x8(*reinterpret_cast<const uint4*>(
-                    &sb[(tap * CO + 16 * q + row) * CL_PAD + 8 * kb]));
+                    &sb[cl_at(tap * CO + 16 * q + row, kb)]));
 #pragma unroll
             for (int m = 0; m < 4 * RW; ++m)
 #pragma unroll
@@ -701,21 +707,22 @@ inline bool lds_conv_applies(int h, int wdt, int cout) {
 // Output channels of one workgroup of the LDS-staged convolution.
 inline int lds_conv_block(int cout) { return cout % 64 == 0 ? 64 : 32; }
 
-// Workgroups of 4 x 64 pixels x 64 output channels, or 8 x 64 pixels x 32
-// output channels (4 x 64 when the height is not a multiple of 8).
+// Workgroups of 4 x 64 pixels x 64 output channels (two a CU: 210 VGPRs),
+// or of 4 x 64 pixels x 32 output channels, three a CU (146 VGPRs, 43.8 KB of
+// LDS): a 32-channel layer has few MFMAs per staged byte, so its workgroups
+// mostly wait on their loads and stores, and a third one a CU hides more of
+// that than taller tiles would save (one 4096^2 prediction 7.09 -> 6.67 ms
+// against 8 x 64-pixel tiles at two a CU).
 int launch_conv_lds(void* stream, const ConvIn& ci, const uint16_t* w, const float* bias,
                     uint16_t* out, float* outf, const ConvFuse& fz, int n, int h, int wdt, int cout,
                     int relu) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const unsigned tx = (unsigned)(wdt / CL_TX);
     if (cout % 64 == 0)
-        hipLaunchKernelGGL((k_conv3x3_lds<4, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 64),
-                           dim3(256), 0, st, ci, w, bias, out, outf, fz, n, h, wdt, cout, relu);
-    else if (h % 8 == 0)
-        hipLaunchKernelGGL((k_conv3x3_lds<2, 2>), dim3((unsigned)n * (h / 8) * tx, cout / 32),
+        hipLaunchKernelGGL((k_conv3x3_lds<4, 1, 2>), dim3((unsigned)n * (h / 4) * tx, cout / 64),
                            dim3(256), 0, st, ci, w, bias, out, outf, fz, n, h, wdt, cout, relu);
     else
-        hipLaunchKernelGGL((k_conv3x3_lds<2, 1>), dim3((unsigned)n * (h / 4) * tx, cout / 32),
+        hipLaunchKernelGGL((k_conv3x3_lds<2, 1, 3>), dim3((unsigned)n * (h / 4) * tx, cout / 32),
                            dim3(256), 0, st, ci, w, bias, out, outf, fz, n, h, wdt, cout, relu);
     CAGEO_HIP_TRY(hipGetLastError());
     return CAGEO_OK;
