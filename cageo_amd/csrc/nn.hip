@@ -711,8 +711,8 @@ inline int lds_conv_block(int cout) { return cout % 64 == 0 ? 64 : 32; }
 // or of 4 x 64 pixels x 32 output channels, three a CU (146 VGPRs, 43.8 KB of
 // LDS): a 32-channel layer has few MFMAs per staged byte, so its workgroups
 // mostly wait on their loads and stores, and a third one a CU hides more of
-// that than taller tiles would save (one 4096^2 prediction 7.09 -> 6.67 ms
-// against 8 x 64-pixel tiles at two a CU).
+// that than taller tiles would save (the three such layers of a 4096^2
+// prediction: 891 -> 843 us a call against 8 x 64-pixel tiles at two a CU).
 int launch_conv_lds(void* stream, const ConvIn& ci, const uint16_t* w, const float* bias,
                     uint16_t* out, float* outf, const ConvFuse& fz, int n, int h, int wdt, int cout,
                     int relu) {
