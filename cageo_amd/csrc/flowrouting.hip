@@ -211,7 +211,7 @@ __device__ void bfs_solve(FrShared& S, const FrArgs& args, const FrBlock& b, int
                           uint32_t epoch, int pass, bool write_all) {
     const FrRaster& a = b.R;
     const int x0 = tx * TILE, y0 = ty * TILE;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t flat = 0;
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) {
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(256) void k_fr_pick(FrArgs a) {
     const FrBlock b = fr_block<MULTI>(a, blockIdx.x);
     const FrRaster& R = b.R;
     const int x0 = (b.lb % R.ntx) * TILE, y0 = (b.lb / R.ntx) * TILE;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int x = x0 + lx;
     uint32_t flat = 0;
     {

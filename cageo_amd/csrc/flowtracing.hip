@@ -82,7 +82,7 @@ __device__ __forceinline__ void trace_tile(uint64_t* L, const int8_t* __restrict
                                            uint64_t* __restrict__ bent, int W, int H, int ntx,
                                            int t) {
     const int x0 = (t % ntx) * TILE, y0 = (t / ntx) * TILE;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gx = x0 + lx;
 
     // The strip's cells' flags and directions are loaded together, then

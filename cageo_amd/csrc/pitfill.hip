@@ -253,7 +253,7 @@ __device__ inline uint32_t load_tile_own(PfShared& S, const PfRaster& a, int x0,
                                         bool restart, const float* zq) {
     constexpr int RING = 4 * HALO_W - 4;  // 260 cells
     constexpr int RIT = (RING + TILE_THREADS - 1) / TILE_THREADS;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gx = x0 + lx;
     auto relay = [&](uint8_t st, float w) {
         const float r = (st == PF_FIXED_LIVE || (st == PF_OPEN && !restart)) ? w : PF_INF;
@@ -410,7 +410,7 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
     //     (their value if live, nothing if dead).  The fixed point of the row,
     //     W(x) = max(Z, min(c, W(x-1), W(x+1))), is min(L(x), R(x)) with L, R
     //     the map chains from the left and right halo cells.
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int hl = 0, hr = HALO_W - 1;  // halo columns
     uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
@@ -510,7 +510,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         return;
     }
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gx = x0 + lx;
     uint32_t open = load_tile_own(S, a, x0, y0, restart, a.zq);  // OPEN cells
     __syncthreads();
@@ -557,7 +557,7 @@ __device__ inline void cert_sweeps(PfShared& S, unsigned long long dirty, uint32
     // than their value; along the row, fixing spreads from a cell fixed live
     // to an OPEN neighbour whose value is not lower (segmented propagation
     // over the lane masks).
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t rd = strip_rows(dirty, strip);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
@@ -611,7 +611,7 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     if (pass == 0 && __builtin_amdgcn_readfirstlane(ld(&args.topen[b.gbase + b.lb])) == 0u)
         return;  // closed since it joined the phase list
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gx = x0 + lx;
     const uint32_t sopen = load_tile_own(S, a, x0, y0, false, nullptr);
     uint32_t open = 0, open_inf = 0;
@@ -866,7 +866,7 @@ __device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round
     const uint32_t nf = ld(&a.ctrl[CTL_COMPACT]);
     const int t = ranked_entry(a.work + (size_t)6 * a.total, a.total, nf, blockIdx.x);
     const int x0 = (t % R.ntx) * TILE, y0 = (t / R.ntx) * TILE;
-    const int lx = threadIdx.x & 63, strip = threadIdx.x >> 6;
+    const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gx = x0 + lx;
     // the whole tile once, as a first pass of a round loads it
     load_tile(S, R, x0, y0, true);
