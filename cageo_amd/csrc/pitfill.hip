@@ -55,7 +55,6 @@ constexpr float PF_INF = __builtin_huge_valf();
 
 struct PfShared {
     float2 rel[HALO_W * HALO_W];  // x: relay (OPEN or live FIXED), y: live FIXED only
-    float zt[TILE * TILE];        // quantized elevation Z of the tile's OPEN cells
     uint32_t rowdirty[TILE];      // strip edge rows marked by the neighbouring strips
     uint32_t wcol[TILE_THREADS / 64], ecol[TILE_THREADS / 64];  // changed rows of columns 0, TILE-1
     uint32_t bits;
@@ -64,6 +63,19 @@ struct PfShared {
     uint32_t steps; // row steps of the tile's solve, all waves
     uint32_t ringrows[2];  // resident rounds: tile rows next to changed halo cells
     uint32_t pfront, pback;  // entries this workgroup listed in the pass (thread 0's)
+};
+
+// Z of a thread's 16 cells (row k of its strip), held in registers by the
+// persistent kernel: a vector, so that the sweeps' read of row k (the same k
+// across the wave) is an indexed register move, where an array would live
+// in scratch.  Without a Z tile in LDS (35.4 KB a workgroup) four workgroups
+// share a CU.
+typedef float ZRow __attribute__((ext_vector_type(STRIP)));
+
+// The resident rounds keep the tile's Z in LDS (their kernel has no
+// registers to spare).
+struct PfSharedRes : PfShared {
+    float zt[TILE * TILE];
 };
 
 // run_passes result of a launch being abandoned at a barrier.
@@ -246,11 +258,11 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0,
 
 // load_tile and the tile's own states in fewer dependent rounds: each
 // thread loads its strip's 16 cells (state, value and, with zq, the
-// quantized elevation into S.zt) in two batches of 8 rows, the first with
+// quantized elevation into zr) in two batches of 8 rows, the first with
 // the halo ring's cells, so a tile costs two rounds of global loads.
 // Returns the thread's OPEN cells (bit k = row k of its strip).
 __device__ inline uint32_t load_tile_own(PfShared& S, const PfRaster& a, int x0, int y0,
-                                        bool restart, const float* zq) {
+                                        bool restart, const float* zq, ZRow& zr) {
     constexpr int RING = 4 * HALO_W - 4;  // 260 cells
     constexpr int RIT = (RING + TILE_THREADS - 1) / TILE_THREADS;
     const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -313,7 +325,7 @@ __device__ inline uint32_t load_tile_own(PfShared& S, const PfRaster& a, int x0,
         for (int k = 0; k < HB; ++k) {
             const int ly = strip * STRIP + h * HB + k;
             S.rel[(ly + 1) * HALO_W + lx + 1] = relay(sv[k], wv[k]);
-            if (zq) S.zt[ly * TILE + lx] = zv[k];
+            zr[h * HB + k] = zv[k];
             if (sv[k] == PF_OPEN) open |= 1u << (h * HB + k);
         }
     }
@@ -391,15 +403,18 @@ __device__ inline void keep_quantized(const PfRaster& R, int lb) {
     }
 }
 
-// Fill sweeps of a tile whose relays (S.rel) and Z (S.zt) are in LDS, over
+// Fill sweeps of a tile whose relays (S.rel) are in LDS, with Z of row k of
+// the thread's strip (tile row ly) from z_at(k, ly), over
 // the rows of `dirty` first (a 64-bit row mask) and then every row a change
 // reaches.  open / zlm: the thread's OPEN cells and those whose Z is live
 // (bit k = row k of its strip); lowered / fixed / fixed_live collect its
 // cells whose value dropped, that were fixed, that were fixed live; nrow
 // counts the wave's row steps.
+template <class ZAt>
 __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float mn,
                                    uint32_t& open, uint32_t zlm, uint32_t& lowered,
-                                   uint32_t& fixed, uint32_t& fixed_live, uint32_t& nrow) {
+                                   uint32_t& fixed, uint32_t& fixed_live, uint32_t& nrow,
+                                   const ZAt& z_at) {
     // Row step (row ly, all 64 columns at once):
     //  1. Z-rule: an OPEN cell with a live FIXED neighbour no higher than Z
     //     is fixed at Z and, if Z is live, relays Z; along the row this
@@ -432,7 +447,7 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
             const float2 a0 = S.rel[li - HALO_W - 1], a1 = S.rel[li - HALO_W], a2 = S.rel[li - HALO_W + 1];
             const float2 b0 = S.rel[li + HALO_W - 1], b1 = S.rel[li + HALO_W], b2 = S.rel[li + HALO_W + 1];
             const float yl = S.rel[li - 1].y, yr = S.rel[li + 1].y;
-            const float z = S.zt[ly * TILE + lx];
+            const float z = z_at(k, ly);
             const float wcur = c.x;
             const float m8y = fminf(fminf(fminf(a0.y, a1.y), fminf(a2.y, yl)),
                                     fminf(fminf(yr, b0.y), fminf(b1.y, b2.y)));
@@ -512,19 +527,20 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gx = x0 + lx;
-    uint32_t open = load_tile_own(S, a, x0, y0, restart, a.zq);  // OPEN cells
+    ZRow zr;  // Z of the thread's cells (row k of its strip)
+    uint32_t open = load_tile_own(S, a, x0, y0, restart, a.zq, zr);  // OPEN cells
     __syncthreads();
     uint32_t zlm = 0;  // OPEN cells whose Z is live
 #pragma unroll
     for (int k = 0; k < STRIP; ++k)
-        if (((open >> k) & 1u) && pf_live(S.zt[(strip * STRIP + k) * TILE + lx], mn))
-            zlm |= 1u << k;
+        if (((open >> k) & 1u) && pf_live(zr[k], mn)) zlm |= 1u << k;
     clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
     const uint32_t rewrite = restart && round > 0 ? open : 0u;  // stale values of the last round
 
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
     uint32_t nrow = 0;  // rows solved by this wave
-    fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow);
+    fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
+                [&](int k, int) { return zr[k]; });
     if ((threadIdx.x & 63) == 0 && nrow) atomicAdd(&S.steps, nrow);
 
     uint32_t bits = 0, nfixed = 0;
@@ -533,8 +549,7 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
         const int ly = strip * STRIP + k;
         const size_t gi = (size_t)(y0 + ly) * a.W + gx;
         if ((lowered | rewrite) & (1u << k))
-            a.w[gi] = (fixed & (1u << k)) ? S.zt[ly * TILE + lx]
-                                          : S.rel[(ly + 1) * HALO_W + lx + 1].x;
+            a.w[gi] = (fixed & (1u << k)) ? zr[k] : S.rel[(ly + 1) * HALO_W + lx + 1].x;
         if (fixed & (1u << k)) {
             a.st[gi] = (fixed_live & (1u << k)) ? PF_FIXED_LIVE : PF_FIXED_DEAD;
             ++nfixed;
@@ -613,7 +628,8 @@ __device__ void cert_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
     const int x0 = (b.lb % a.ntx) * TILE, y0 = (b.lb / a.ntx) * TILE;
     const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gx = x0 + lx;
-    const uint32_t sopen = load_tile_own(S, a, x0, y0, false, nullptr);
+    ZRow zr_unused;
+    const uint32_t sopen = load_tile_own(S, a, x0, y0, false, nullptr, zr_unused);
     uint32_t open = 0, open_inf = 0;
     __syncthreads();  // the relays are in LDS
     clear_dirty_rows(args.rows, b.gbase + b.lb, pass);
@@ -861,7 +877,7 @@ __device__ inline void resident_reduce(PfShared& S, uint32_t nfixed, uint32_t bi
 // (one raster; workgroups 0 .. n-1 take the n tiles of the compacted list).
 // Returns the round it ended in, -1 when a phase hit the pass cap, or
 // PF_ABANDONED.
-__device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round, uint32_t n) {
+__device__ int resident_rounds(PfSharedRes& S, const PfArgs& a, float mn, int round, uint32_t n) {
     const PfRaster& R = a.one;
     const uint32_t nf = ld(&a.ctrl[CTL_COMPACT]);
     const int t = ranked_entry(a.work + (size_t)6 * a.total, a.total, nf, blockIdx.x);
@@ -922,7 +938,8 @@ __device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round
                 if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
                 __syncthreads();
                 uint32_t lowered = 0, fixed = 0, fixed_live = 0, nrow = 0;
-                fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow);
+                fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
+                            [&](int, int ly) { return S.zt[ly * TILE + lx]; });
                 uint32_t bits = 0, nfixed = 0;
 #pragma unroll
                 for (int k = 0; k < STRIP; ++k) {
@@ -1005,7 +1022,7 @@ __device__ int resident_rounds(PfShared& S, const PfArgs& a, float mn, int round
 // MULTI: a table of rasters (else the one raster of the arguments; two
 // instantiations keep the one-raster code free of table lookups).
 template <bool MULTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pf_persistent(PfArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pf_persistent(PfArgs a) {
     __shared__ PfShared S;
     if (grid_aborted(a.gb)) return;
 
@@ -1144,7 +1161,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // over (ctrl[CTL_RESIDENT] open tiles, one workgroup each; the other
 // workgroups end at once).  Nothing to do otherwise.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pf_resident(PfArgs a) {
-    __shared__ PfShared S;
+    __shared__ PfSharedRes S;
     const uint32_t n = __builtin_amdgcn_readfirstlane(ld(&a.ctrl[CTL_RESIDENT]));
     if (blockIdx.x >= n) return;
     if (grid_aborted(a.gb)) return;
