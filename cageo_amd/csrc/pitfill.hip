@@ -101,9 +101,11 @@ __device__ inline float min8x(const float2* s, int i) {
 // A few scalar instructions instead of a doubling loop.
 __device__ inline unsigned long long spread_up(unsigned long long s, unsigned long long g) {
     const unsigned long long x = s | g;
-    unsigned long long sum;
-    const bool out = __builtin_add_overflow(x, s, &sum);
-    return ((sum ^ x ^ s) >> 1) | ((unsigned long long)out << 63);
+    const unsigned long long sum = x + s;
+    // carry out of bit 63 from the operands' and the sum's top bits (scalar
+    // bit operations, where an overflow test became a vector compare)
+    const unsigned long long out = ((x & s) | ((x | s) & ~sum)) >> 63;
+    return ((sum ^ x ^ s) >> 1) | (out << 63);
 }
 __device__ inline unsigned long long spread_row(unsigned long long seed, unsigned long long gate_up,
                                                 unsigned long long gate_down) {
@@ -447,6 +449,11 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
             const float2 a0 = S.rel[li - HALO_W - 1], a1 = S.rel[li - HALO_W], a2 = S.rel[li - HALO_W + 1];
             const float2 b0 = S.rel[li + HALO_W - 1], b1 = S.rel[li + HALO_W], b2 = S.rel[li + HALO_W + 1];
             const float yl = S.rel[li - 1].y, yr = S.rel[li + 1].y;
+            // the row's two halo cells (one address for the whole wave)
+            const float hlv = __int_as_float(__builtin_amdgcn_readfirstlane(
+                __float_as_int(S.rel[(ly + 1) * HALO_W + hl].x)));
+            const float hrv = __int_as_float(__builtin_amdgcn_readfirstlane(
+                __float_as_int(S.rel[(ly + 1) * HALO_W + hr].x)));
             const float z = z_at(k, ly);
             const float wcur = c.x;
             const float m8y = fminf(fminf(fminf(a0.y, a1.y), fminf(a2.y, yl)),
@@ -455,12 +462,15 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
             const bool zlive = op && ((zlm >> k) & 1u);
             const bool seed = op && m8y <= z;
             const float zl = lane_prev(z), zrt = lane_next(z);
-            const int livel = lane_prev((int)zlive), liver = lane_next((int)zlive);
             const unsigned long long sd = __ballot(seed);
             unsigned long long f = sd;
-            if (sd)
-                f = spread_row(sd, __ballot(op && lx > 0 && livel && zl <= z),
-                               __ballot(op && lx < TILE - 1 && liver && zrt <= z));
+            if (sd) {
+                // a step x-1 -> x needs x-1 live (lane masks shifted by one
+                // lane; the end lanes get 0 from the shift)
+                const unsigned long long lv = __ballot(zlive);
+                f = spread_row(sd, __ballot(op && zl <= z) & (lv << 1),
+                               __ballot(op && zrt <= z) & (lv >> 1));
+            }
             bool ch = false;
             float lo, hi;
             if ((f >> lx) & 1ull) {
@@ -486,8 +496,8 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
             scan_clamp_up(llo, lhi);
             scan_clamp_down(rlo, rhi);
             if (open & bit) {
-                const float L = clamp3(S.rel[(ly + 1) * HALO_W + hl].x, llo, lhi);
-                const float R = clamp3(S.rel[(ly + 1) * HALO_W + hr].x, rlo, rhi);
+                const float L = clamp3(hlv, llo, lhi);
+                const float R = clamp3(hrv, rlo, rhi);
                 const float nw = fminf(L, R);
                 if (nw < wcur) {
                     S.rel[li].x = nw;
@@ -591,12 +601,15 @@ __device__ inline void cert_sweeps(PfShared& S, unsigned long long dirty, uint32
             const bool seed = op && min8y(S.rel, li) <= w;
             // from the left neighbour x-1 (open, live) into x, and from x+1
             const float wl = lane_prev(w), wrt = lane_next(w);
-            const int livel = lane_prev((int)live), liver = lane_next((int)live);
-            const bool gl = op && lx > 0 && livel && wl <= w;
-            const bool gr = op && lx < TILE - 1 && liver && wrt <= w;
             const unsigned long long sd = __ballot(seed);
             unsigned long long f = sd;
-            if (sd) f = spread_row(sd, __ballot(gl), __ballot(gr));
+            if (sd) {
+                // a step x-1 -> x needs x-1 live (lane masks shifted by one
+                // lane; the end lanes get 0 from the shift)
+                const unsigned long long lv = __ballot(live);
+                f = spread_row(sd, __ballot(op && wl <= w) & (lv << 1),
+                               __ballot(op && wrt <= w) & (lv >> 1));
+            }
             bool ch = false;
             if ((f >> lx) & 1ull) {
                 open &= ~bit;
