@@ -412,7 +412,7 @@ __device__ inline void keep_quantized(const PfRaster& R, int lb) {
 // (bit k = row k of its strip); lowered / fixed / fixed_live collect its
 // cells whose value dropped, that were fixed, that were fixed live; nrow
 // counts the wave's row steps.
-template <class ZAt>
+template <bool CARRY, class ZAt>
 __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float mn,
                                    uint32_t& open, uint32_t zlm, uint32_t& lowered,
                                    uint32_t& fixed, uint32_t& fixed_live, uint32_t& nrow,
@@ -430,6 +430,8 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
     const int lx = threadIdx.x & 63, strip = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int hl = 0, hr = HALO_W - 1;  // halo columns
     uint32_t rd = strip_rows(dirty, strip);
+    ClampCarry cc;
+    clamp_carry_init(cc);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
         const bool upward = sweep_upward(S.rowdirty, rd, strip, sweep);
@@ -493,8 +495,13 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
                 lo = hi = c.y;  // live FIXED: its value; otherwise no relay
             }
             float llo = lo, lhi = hi, rlo = lo, rhi = hi;
-            scan_clamp_up(llo, lhi);
-            scan_clamp_down(rlo, rhi);
+            if constexpr (CARRY) {
+                scan_clamp_up(llo, lhi, cc);
+                scan_clamp_down(rlo, rhi, cc);
+            } else {
+                scan_clamp_up(llo, lhi);
+                scan_clamp_down(rlo, rhi);
+            }
             if (open & bit) {
                 const float L = clamp3(hlv, llo, lhi);
                 const float R = clamp3(hrv, rlo, rhi);
@@ -549,8 +556,8 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
 
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
     uint32_t nrow = 0;  // rows solved by this wave
-    fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
-                [&](int k, int) { return zr[k]; });
+    fill_sweeps<true>(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
+                      [&](int k, int) { return zr[k]; });
     if ((threadIdx.x & 63) == 0 && nrow) atomicAdd(&S.steps, nrow);
 
     uint32_t bits = 0, nfixed = 0;
@@ -951,8 +958,9 @@ __device__ int resident_rounds(PfSharedRes& S, const PfArgs& a, float mn, int ro
                 if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
                 __syncthreads();
                 uint32_t lowered = 0, fixed = 0, fixed_live = 0, nrow = 0;
-                fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
-                            [&](int, int ly) { return S.zt[ly * TILE + lx]; });
+                // the plain scans: this kernel has no registers to spare
+                fill_sweeps<false>(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
+                                   [&](int, int ly) { return S.zt[ly * TILE + lx]; });
                 uint32_t bits = 0, nfixed = 0;
 #pragma unroll
                 for (int k = 0; k < STRIP; ++k) {
@@ -1035,7 +1043,7 @@ __device__ int resident_rounds(PfSharedRes& S, const PfArgs& a, float mn, int ro
 // MULTI: a table of rasters (else the one raster of the arguments; two
 // instantiations keep the one-raster code free of table lookups).
 template <bool MULTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pf_persistent(PfArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pf_persistent(PfArgs a) {
     __shared__ PfShared S;
     if (grid_aborted(a.gb)) return;
 
