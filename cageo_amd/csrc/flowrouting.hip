@@ -58,6 +58,15 @@ __device__ inline void scan_minplus_up(int& a, int& k) {
 __device__ inline void scan_minplus_down(int& a, int& k) {
     CAGEO_SCAN_DOWN(dpp_i, lane_read, int, a, k, D_INF, 0, minplus_after);
 }
+// The same with the lane moves' destinations carried across row steps.
+typedef ScanCarry<int> MinplusCarry;
+__device__ inline void minplus_carry_init(MinplusCarry& c) { scan_carry_init(c, (int)D_INF, 0); }
+__device__ inline void scan_minplus_up(int& a, int& k, MinplusCarry& c) {
+    scan_up_carried(a, k, c, minplus_after);
+}
+__device__ inline void scan_minplus_down(int& a, int& k, MinplusCarry& c) {
+    scan_down_carried(a, k, c, (int)D_INF, 0, minplus_after);
+}
 
 // One raster of a launch.
 struct FrRaster {
@@ -226,6 +235,8 @@ __device__ void bfs_solve(FrShared& S, const FrArgs& args, const FrBlock& b, int
     // nothing.  Scans from both sides give the row's exact fixed point.
     uint32_t changed_cells = 0;
     uint32_t rd = strip_rows(dirty, strip);
+    MinplusCarry mc;
+    minplus_carry_init(mc);
     for (int sweep = 0;; ++sweep) {
         int changed = 0;
         const bool upward = sweep_upward(S.rowdirty, rd, strip, sweep);
@@ -261,8 +272,8 @@ __device__ void bfs_solve(FrShared& S, const FrArgs& args, const FrBlock& b, int
                 a = c.x >= 0 ? c.x : D_INF;  // a direction relays 0
             }
             int al = a, ar = a;
-            scan_minplus_up(al, kl);
-            scan_minplus_down(ar, kr);
+            scan_minplus_up(al, kl, mc);
+            scan_minplus_down(ar, kr, mc);
             bool ch = false;
             int nd = D_INF;
             if (isflat) {

@@ -122,30 +122,43 @@ __device__ inline void scan_clamp_down(float& lo, float& hi) {
     CAGEO_SCAN_DOWN(dpp_f, lane_read, float, lo, hi, NI, PI, clamp_after);
 }
 
-// The clamp scans with their lane moves' destinations carried across row
-// steps.  Each destination is written by its own lane move only, so the
-// lanes that move leaves alone (outside the shift, or in rows it masks off)
-// keep the identity they were given once, and no step has to write it
-// again (20 moves a row step fewer, for 20 registers).
-struct ClampCarry {
-    float u[12];  // upward scan: levels 1, 2, 4, 8, bcast15, bcast31 (lo, hi)
-    float d[8];   // downward scan: levels 1, 2, 4, 8 (lo, hi)
+// Scans with their lane moves' destinations carried across row steps.
+// Each destination is written by its own lane move only, so the lanes that
+// move leaves alone (outside the shift, or in rows it masks off) keep the
+// identity they were given once, and no step has to write it again (20
+// moves a row step fewer, for 20 registers).
+template <class T>
+struct ScanCarry {
+    T u[12];  // upward scan: levels 1, 2, 4, 8, bcast15, bcast31 (first, second)
+    T d[8];   // downward scan: levels 1, 2, 4, 8 (first, second)
 };
-__device__ inline void clamp_carry_init(ClampCarry& c) {
-    const float NI = -__builtin_huge_valf(), PI = __builtin_huge_valf();
+template <class T>
+__device__ inline void scan_carry_init(ScanCarry<T>& c, T ea, T eb) {
 #pragma unroll
     for (int i = 0; i < 12; i += 2) {
-        c.u[i] = NI;
-        c.u[i + 1] = PI;
+        c.u[i] = ea;
+        c.u[i + 1] = eb;
     }
 #pragma unroll
     for (int i = 0; i < 8; i += 2) {
-        c.d[i] = NI;
-        c.d[i + 1] = PI;
+        c.d[i] = ea;
+        c.d[i + 1] = eb;
     }
 }
-#define CAGEO_CARRY_STEP(ARR, I, CTRL, ROWS)                                                       do {                                                                                               ARR[I] = dpp_f<CTRL, ROWS>(ARR[I], lo);                                                        ARR[I + 1] = dpp_f<CTRL, ROWS>(ARR[I + 1], hi);                                                clamp_after(lo, hi, ARR[I], ARR[I + 1]);                                                   } while (0)
-__device__ inline void scan_clamp_up(float& lo, float& hi, ClampCarry& c) {
+template <int CTRL, int ROWS>
+__device__ inline float dpp_t(float old, float src) { return dpp_f<CTRL, ROWS>(old, src); }
+template <int CTRL, int ROWS>
+__device__ inline int dpp_t(int old, int src) { return dpp_i<CTRL, ROWS>(old, src); }
+#define CAGEO_CARRY_STEP(ARR, I, CTRL, ROWS)                                                   \
+    do {                                                                                       \
+        ARR[I] = dpp_t<CTRL, ROWS>(ARR[I], a);                                                 \
+        ARR[I + 1] = dpp_t<CTRL, ROWS>(ARR[I + 1], b);                                         \
+        compose(a, b, ARR[I], ARR[I + 1]);                                                     \
+    } while (0)
+// Inclusive scans of "self after prev" (compose(self_a, self_b, prev_a,
+// prev_b)) with identity (ea, eb), as CAGEO_SCAN_UP / CAGEO_SCAN_DOWN.
+template <class T, class C>
+__device__ inline void scan_up_carried(T& a, T& b, ScanCarry<T>& c, C compose) {
     CAGEO_CARRY_STEP(c.u, 0, DPP_ROW_SHR + 1, 0xf);
     CAGEO_CARRY_STEP(c.u, 2, DPP_ROW_SHR + 2, 0xf);
     CAGEO_CARRY_STEP(c.u, 4, DPP_ROW_SHR + 4, 0xf);
@@ -153,22 +166,33 @@ __device__ inline void scan_clamp_up(float& lo, float& hi, ClampCarry& c) {
     CAGEO_CARRY_STEP(c.u, 8, DPP_ROW_BCAST15, 0xa);
     CAGEO_CARRY_STEP(c.u, 10, DPP_ROW_BCAST31, 0xc);
 }
-__device__ inline void scan_clamp_down(float& lo, float& hi, ClampCarry& c) {
-    const float NI = -__builtin_huge_valf(), PI = __builtin_huge_valf();
+template <class T, class C>
+__device__ inline void scan_down_carried(T& a, T& b, ScanCarry<T>& c, T ea, T eb, C compose) {
     CAGEO_CARRY_STEP(c.d, 0, DPP_ROW_SHL + 1, 0xf);
     CAGEO_CARRY_STEP(c.d, 2, DPP_ROW_SHL + 2, 0xf);
     CAGEO_CARRY_STEP(c.d, 4, DPP_ROW_SHL + 4, 0xf);
     CAGEO_CARRY_STEP(c.d, 6, DPP_ROW_SHL + 8, 0xf);
-    float s3a = lane_read(lo, 48), s3b = lane_read(hi, 48);
-    float s2a = lane_read(lo, 32), s2b = lane_read(hi, 32);
-    float s1a = lane_read(lo, 16), s1b = lane_read(hi, 16);
-    clamp_after(s2a, s2b, s3a, s3b);
-    clamp_after(s1a, s1b, s2a, s2b);
+    T s3a = lane_read(a, 48), s3b = lane_read(b, 48);
+    T s2a = lane_read(a, 32), s2b = lane_read(b, 32);
+    T s1a = lane_read(a, 16), s1b = lane_read(b, 16);
+    compose(s2a, s2b, s3a, s3b);
+    compose(s1a, s1b, s2a, s2b);
     const int row = (threadIdx.x & 63) >> 4;
-    clamp_after(lo, hi, row == 0 ? s1a : row == 1 ? s2a : row == 2 ? s3a : NI,
-                row == 0 ? s1b : row == 1 ? s2b : row == 2 ? s3b : PI);
+    compose(a, b, row == 0 ? s1a : row == 1 ? s2a : row == 2 ? s3a : ea,
+            row == 0 ? s1b : row == 1 ? s2b : row == 2 ? s3b : eb);
 }
 #undef CAGEO_CARRY_STEP
+
+typedef ScanCarry<float> ClampCarry;
+__device__ inline void clamp_carry_init(ClampCarry& c) {
+    scan_carry_init(c, -__builtin_huge_valf(), __builtin_huge_valf());
+}
+__device__ inline void scan_clamp_up(float& lo, float& hi, ClampCarry& c) {
+    scan_up_carried(lo, hi, c, clamp_after);
+}
+__device__ inline void scan_clamp_down(float& lo, float& hi, ClampCarry& c) {
+    scan_down_carried(lo, hi, c, -__builtin_huge_valf(), __builtin_huge_valf(), clamp_after);
+}
 
 // Rows due for a row solve.  Each wave keeps the rows of its own strip in
 // one wave-uniform word (bit k = row k of the strip): the low half holds
