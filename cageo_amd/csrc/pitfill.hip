@@ -1043,7 +1043,8 @@ __device__ int resident_rounds(PfSharedRes& S, const PfArgs& a, float mn, int ro
 // MULTI: a table of rasters (else the one raster of the arguments; two
 // instantiations keep the one-raster code free of table lookups).  At least
 // three waves a SIMD; the one-raster kernel (126 VGPRs, 35.4 KB of LDS) runs
-// four workgroups a CU, the table one (130 VGPRs) three: both time the same.
+// four workgroups a CU, the table one (130 VGPRs) three (the one-raster
+// kernel times the same at three and at four).
 template <bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pf_persistent(PfArgs a) {
     __shared__ PfShared S;
