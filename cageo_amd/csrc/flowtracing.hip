@@ -149,26 +149,22 @@ __device__ __forceinline__ void trace_tile(uint64_t* L, const int8_t* __restrict
     }
     __syncthreads();
 
-    // Jacobi doubling rounds: read all targets, barrier, write.
+    // Doubling rounds in place: a cell's entry is written by its own thread
+    // only, and the target entry it reads is either last round's or already
+    // this round's, both exact summaries of the target's path.  So every
+    // entry at least doubles its hops a round (paths inside a tile resolve
+    // within 13), step counts stay exact sums, and one barrier a round
+    // suffices.  Entries are 64-bit LDS words, each read or written whole.
     for (int round = 0; round < 13; ++round) {
         if (!__syncthreads_or(local != 0)) break;
-        uint64_t nv[STRIP];
-#pragma unroll
-        for (int k = 0; k < STRIP; ++k) {
-            const int li = (strip * STRIP + k) * TILE + lx;
-            nv[k] = 0;
-            if (local & (1u << k)) {
-                const uint64_t e = L[li];
-                nv[k] = hop(e, L[value_of(e)]);
-            }
-        }
-        __syncthreads();
 #pragma unroll
         for (int k = 0; k < STRIP; ++k) {
             if (local & (1u << k)) {
                 const int li = (strip * STRIP + k) * TILE + lx;
-                L[li] = nv[k];
-                if (kind_of(nv[k]) != K_LOCAL) local &= ~(1u << k);
+                const uint64_t e = L[li];
+                const uint64_t nv = hop(e, L[value_of(e)]);
+                L[li] = nv;
+                if (kind_of(nv) != K_LOCAL) local &= ~(1u << k);
             }
         }
     }
