@@ -958,9 +958,10 @@ __device__ int resident_rounds(PfSharedRes& S, const PfArgs& a, float mn, int ro
                 if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
                 __syncthreads();
                 uint32_t lowered = 0, fixed = 0, fixed_live = 0, nrow = 0;
-                // the plain scans: this kernel has no registers to spare
-                fill_sweeps<false>(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
-                                   [&](int, int ly) { return S.zt[ly * TILE + lx]; });
+                // carried scans here too: their 20 registers push 3 values of
+                // the rounds' last write-back to scratch, off the sweep loop
+                fill_sweeps<true>(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
+                                  [&](int, int ly) { return S.zt[ly * TILE + lx]; });
                 uint32_t bits = 0, nfixed = 0;
 #pragma unroll
                 for (int k = 0; k < STRIP; ++k) {
