@@ -78,6 +78,10 @@ struct PfSharedRes : PfShared {
     float zt[TILE * TILE];
 };
 
+// Batches of a tile's own rows in load_tile_own (each a round of global
+// loads; the first carries the halo ring too).
+constexpr int PF_LOAD_BATCHES = 1;
+
 // run_passes result of a launch being abandoned at a barrier.
 constexpr int PF_ABANDONED = -2;
 
@@ -260,8 +264,8 @@ __device__ inline void load_tile(PfShared& S, const PfRaster& a, int x0, int y0,
 
 // load_tile and the tile's own states in fewer dependent rounds: each
 // thread loads its strip's 16 cells (state, value and, with zq, the
-// quantized elevation into zr) in two batches of 8 rows, the first with
-// the halo ring's cells, so a tile costs two rounds of global loads.
+// quantized elevation into zr) in PF_LOAD_BATCHES batches of rows, the
+// first with the halo ring's cells: one round of global loads a batch.
 // Returns the thread's OPEN cells (bit k = row k of its strip).
 __device__ inline uint32_t load_tile_own(PfShared& S, const PfRaster& a, int x0, int y0,
                                         bool restart, const float* zq, ZRow& zr) {
@@ -306,8 +310,8 @@ __device__ inline uint32_t load_tile_own(PfShared& S, const PfRaster& a, int x0,
     }
     uint32_t open = 0;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        constexpr int HB = STRIP / 2;
+    for (int h = 0; h < PF_LOAD_BATCHES; ++h) {
+        constexpr int HB = STRIP / PF_LOAD_BATCHES;
         uint8_t sv[HB];
         float wv[HB], zv[HB];
 #pragma unroll
