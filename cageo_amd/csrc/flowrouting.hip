@@ -51,14 +51,8 @@ __device__ inline void minplus_after(int& a, int& k, int pa, int pk) {
     a = min(a, sat_add(pa, k));
     k = sat_add(pk, k);
 }
-// Inclusive min-plus scans towards higher / lower lanes.
-__device__ inline void scan_minplus_up(int& a, int& k) {
-    CAGEO_SCAN_UP(dpp_i, a, k, D_INF, 0, minplus_after);
-}
-__device__ inline void scan_minplus_down(int& a, int& k) {
-    CAGEO_SCAN_DOWN(dpp_i, lane_read, int, a, k, D_INF, 0, minplus_after);
-}
-// The same with the lane moves' destinations carried across row steps.
+// Inclusive min-plus scans towards higher / lower lanes, their lane moves'
+// destinations carried across row steps (ScanCarry).
 typedef ScanCarry<int> MinplusCarry;
 __device__ inline void minplus_carry_init(MinplusCarry& c) { scan_carry_init(c, (int)D_INF, 0); }
 __device__ inline void scan_minplus_up(int& a, int& k, MinplusCarry& c) {

@@ -416,7 +416,7 @@ __device__ inline void keep_quantized(const PfRaster& R, int lb) {
 // (bit k = row k of its strip); lowered / fixed / fixed_live collect its
 // cells whose value dropped, that were fixed, that were fixed live; nrow
 // counts the wave's row steps.
-template <bool CARRY, class ZAt>
+template <class ZAt>
 __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float mn,
                                    uint32_t& open, uint32_t zlm, uint32_t& lowered,
                                    uint32_t& fixed, uint32_t& fixed_live, uint32_t& nrow,
@@ -499,13 +499,8 @@ __device__ inline void fill_sweeps(PfShared& S, unsigned long long dirty, float 
                 lo = hi = c.y;  // live FIXED: its value; otherwise no relay
             }
             float llo = lo, lhi = hi, rlo = lo, rhi = hi;
-            if constexpr (CARRY) {
-                scan_clamp_up(llo, lhi, cc);
-                scan_clamp_down(rlo, rhi, cc);
-            } else {
-                scan_clamp_up(llo, lhi);
-                scan_clamp_down(rlo, rhi);
-            }
+            scan_clamp_up(llo, lhi, cc);
+            scan_clamp_down(rlo, rhi, cc);
             if (open & bit) {
                 const float L = clamp3(hlv, llo, lhi);
                 const float R = clamp3(hrv, rlo, rhi);
@@ -560,8 +555,8 @@ __device__ void fill_tile(PfShared& S, const PfArgs& args, const PfBlock& b, flo
 
     uint32_t lowered = 0, fixed = 0, fixed_live = 0;
     uint32_t nrow = 0;  // rows solved by this wave
-    fill_sweeps<true>(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
-                      [&](int k, int) { return zr[k]; });
+    fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
+                [&](int k, int) { return zr[k]; });
     if ((threadIdx.x & 63) == 0 && nrow) atomicAdd(&S.steps, nrow);
 
     uint32_t bits = 0, nfixed = 0;
@@ -962,10 +957,10 @@ __device__ int resident_rounds(PfSharedRes& S, const PfArgs& a, float mn, int ro
                 if (threadIdx.x < TILE) S.rowdirty[threadIdx.x] = 0u;
                 __syncthreads();
                 uint32_t lowered = 0, fixed = 0, fixed_live = 0, nrow = 0;
-                // carried scans here too: their 20 registers push 3 values of
-                // the rounds' last write-back to scratch, off the sweep loop
-                fill_sweeps<true>(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
-                                  [&](int, int ly) { return S.zt[ly * TILE + lx]; });
+                // the carried scans' 20 registers push 3 values of the rounds'
+                // last write-back to scratch here, off the sweep loop
+                fill_sweeps(S, dirty, mn, open, zlm, lowered, fixed, fixed_live, nrow,
+                            [&](int, int ly) { return S.zt[ly * TILE + lx]; });
                 uint32_t bits = 0, nfixed = 0;
 #pragma unroll
                 for (int k = 0; k < STRIP; ++k) {

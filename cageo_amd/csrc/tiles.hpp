@@ -80,12 +80,13 @@ __device__ inline float lane_read(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Inclusive scans of an associative "self after prev" composition C(self,
-// prev) with identity element E, over the 64 lanes of a wave.
-//   up:   lane x ends with f_x o ... o f_0   (row shifts, then the row
-//         broadcasts carry rows 0 / 0-1 into the rows after them);
-//   down: lane x ends with f_x o ... o f_63  (row shifts, then the totals
-//         of the rows after each row, read from lanes 16, 32, 48).
+// Inclusive scan of an associative "self after prev" composition C(self,
+// prev) with identity element E, over the 64 lanes of a wave: lane x ends
+// with f_x o ... o f_0 (row shifts, then the row broadcasts carry rows 0 /
+// 0-1 into the rows after them).  The row solves use the carried form below,
+// which also scans towards lower lanes (lane x ends with f_x o ... o f_63:
+// row shifts, then the totals of the rows after each row, read from lanes
+// 16, 32, 48).
 #define CAGEO_SCAN_UP(DPP, a, b, ea, eb, C)                                                    \
     do {                                                                                       \
         C(a, b, DPP<DPP_ROW_SHR + 1>(ea, a), DPP<DPP_ROW_SHR + 1>(eb, b));                     \
@@ -95,32 +96,8 @@ __device__ inline float lane_read(float v, int lane) {
         C(a, b, (DPP<DPP_ROW_BCAST15, 0xa>(ea, a)), (DPP<DPP_ROW_BCAST15, 0xa>(eb, b)));       \
         C(a, b, (DPP<DPP_ROW_BCAST31, 0xc>(ea, a)), (DPP<DPP_ROW_BCAST31, 0xc>(eb, b)));       \
     } while (0)
-#define CAGEO_SCAN_DOWN(DPP, READ, T, a, b, ea, eb, C)                                         \
-    do {                                                                                       \
-        C(a, b, DPP<DPP_ROW_SHL + 1>(ea, a), DPP<DPP_ROW_SHL + 1>(eb, b));                     \
-        C(a, b, DPP<DPP_ROW_SHL + 2>(ea, a), DPP<DPP_ROW_SHL + 2>(eb, b));                     \
-        C(a, b, DPP<DPP_ROW_SHL + 4>(ea, a), DPP<DPP_ROW_SHL + 4>(eb, b));                     \
-        C(a, b, DPP<DPP_ROW_SHL + 8>(ea, a), DPP<DPP_ROW_SHL + 8>(eb, b));                     \
-        T s3a_ = READ(a, 48), s3b_ = READ(b, 48);                                              \
-        T s2a_ = READ(a, 32), s2b_ = READ(b, 32);                                              \
-        T s1a_ = READ(a, 16), s1b_ = READ(b, 16);                                              \
-        C(s2a_, s2b_, s3a_, s3b_);                                                             \
-        C(s1a_, s1b_, s2a_, s2b_);                                                             \
-        const int row_ = (threadIdx.x & 63) >> 4;                                              \
-        C(a, b, row_ == 0 ? s1a_ : row_ == 1 ? s2a_ : row_ == 2 ? s3a_ : (ea),                 \
-          row_ == 0 ? s1b_ : row_ == 1 ? s2b_ : row_ == 2 ? s3b_ : (eb));                      \
-    } while (0)
 __device__ inline int lane_read(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-// Inclusive clamp-map scans towards higher / lower lanes.
-__device__ inline void scan_clamp_up(float& lo, float& hi) {
-    const float NI = -__builtin_huge_valf(), PI = __builtin_huge_valf();
-    CAGEO_SCAN_UP(dpp_f, lo, hi, NI, PI, clamp_after);
-}
-__device__ inline void scan_clamp_down(float& lo, float& hi) {
-    const float NI = -__builtin_huge_valf(), PI = __builtin_huge_valf();
-    CAGEO_SCAN_DOWN(dpp_f, lane_read, float, lo, hi, NI, PI, clamp_after);
-}
 
 // Scans with their lane moves' destinations carried across row steps.
 // Each destination is written by its own lane move only, so the lanes that
@@ -156,7 +133,7 @@ __device__ inline int dpp_t(int old, int src) { return dpp_i<CTRL, ROWS>(old, sr
         compose(a, b, ARR[I], ARR[I + 1]);                                                     \
     } while (0)
 // Inclusive scans of "self after prev" (compose(self_a, self_b, prev_a,
-// prev_b)) with identity (ea, eb), as CAGEO_SCAN_UP / CAGEO_SCAN_DOWN.
+// prev_b)) with identity (ea, eb), towards higher and towards lower lanes.
 template <class T, class C>
 __device__ inline void scan_up_carried(T& a, T& b, ScanCarry<T>& c, C compose) {
     CAGEO_CARRY_STEP(c.u, 0, DPP_ROW_SHR + 1, 0xf);
